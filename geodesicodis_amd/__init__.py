@@ -1,0 +1,359 @@
+"""geodesicodis_amd -- GeodesicODIS's Laplace-Tidal-Equation time step on MI355X.
+
+Python binding (ctypes) of ``libgeodesicodis_amd.so``, whose C ABI is
+declared in ``include/odis.h``.  The library holds the HIP kernels (gfx950)
+and the host C++ that mirrors the reference's program flow
+(Globals -> Mesh -> solveODIS -> ab3Explicit).
+
+There is no CPU fallback: every stepping entry point runs the HIP kernels and
+raises if the library or a GPU is unavailable.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgeodesicodis_amd.so")
+
+FP64, FP32 = 0, 1
+OBLIQ, OBLIQ_WEST, OBLIQ_EAST, ECC_RAD, ECC_LIB, ECC, ECC_WEST, ECC_EAST, FULL, FULL2, \
+    TOTAL, ECC_W3, OBLIQ_W3, PLANET, PLANET_OBL, GENERAL, NONE = range(17)
+FREE, FREE_LOADING, LID_LOVE, LID_MEMBR, LID_NUM, LID_INF = range(6)
+LINEAR, QUADRATIC = range(2)
+
+ERRORS = {0: "OK", -1: "ODIS_E_INVALID", -2: "ODIS_E_HIP", -3: "ODIS_E_NONFINITE",
+          -4: "ODIS_E_UNSUPPORTED", -5: "ODIS_E_NOMEM", -6: "ODIS_E_RANGE", -7: "ODIS_E_STOPPED",
+          -8: "ODIS_E_IO"}
+
+# Symbols declared in include/odis.h (kept in sync by tests/test_capi.py).
+EXPORTS = [
+    "odis_create", "odis_destroy", "odis_last_error", "odis_set_state", "odis_get_state",
+    "odis_step", "odis_sync", "odis_get_dissipation", "odis_get_dissipation_series",
+    "odis_series_capacity", "odis_get_velocity_en", "odis_request_stop", "odis_enable_timing",
+    "odis_get_timing", "odis_grid_read", "odis_grid_generate", "odis_grid_write", "odis_grid_size",
+    "odis_grid_arrays", "odis_grid_from_arrays", "odis_grid_free", "odis_mesh_build",
+    "odis_mesh_sizes", "odis_mesh_describe", "odis_mesh_get", "odis_mesh_free",
+    "odis_round_period", "odis_quantise_dt", "odis_config_read", "odis_host_last_error",
+    "odis_run_case",
+]
+
+
+class OdisError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (ERRORS.get(code, "?"), code, msg))
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    """odis_params (include/odis.h)."""
+    _fields_ = [(n, ctypes.c_double) for n in
+                ("g", "h", "alpha", "omega", "radius", "love_reduct", "ecc", "obliquity", "dt",
+                 "shell_thickness", "semimajor")] + \
+               [(n, ctypes.c_int) for n in
+                ("tide", "surface", "friction", "init_load", "quad_drag", "reserved")]
+
+    def to_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class MeshDesc(ctypes.Structure):
+    _fields_ = [("n_cells", ctypes.c_int), ("n_faces", ctypes.c_int),
+                ("face_nodes", ctypes.c_void_p), ("grad_coef", ctypes.c_void_p),
+                ("trisk_count", ctypes.c_void_p), ("trisk_faces", ctypes.c_void_p),
+                ("trisk_coef", ctypes.c_void_p), ("face_lat", ctypes.c_void_p),
+                ("face_area", ctypes.c_void_p), ("cell_faces", ctypes.c_void_p),
+                ("div_coef", ctypes.c_void_p), ("cell_lat", ctypes.c_void_p),
+                ("cell_lon", ctypes.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    """Load the in-tree library; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("geodesicodis_amd: %s is missing; run __graft_entry__.build() "
+                              "(make -C geodesicodis_amd)" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        vp, ip, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
+        llp = ctypes.POINTER(ctypes.c_longlong)
+        vpp = ctypes.POINTER(ctypes.c_void_p)
+        sig = {
+            "odis_create": ([ctypes.POINTER(MeshDesc), ctypes.POINTER(Params), ctypes.c_int,
+                             ctypes.c_int, vpp], ctypes.c_int),
+            "odis_destroy": ([vp], ctypes.c_int),
+            "odis_last_error": ([vp], ctypes.c_char_p),
+            "odis_set_state": ([vp, ctypes.c_int, dp, dp, dp, dp, ctypes.c_longlong], ctypes.c_int),
+            "odis_get_state": ([vp, ctypes.c_int, dp, dp, dp, dp, llp], ctypes.c_int),
+            "odis_step": ([vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+            "odis_sync": ([vp], ctypes.c_int),
+            "odis_get_dissipation": ([vp, dp], ctypes.c_int),
+            "odis_get_dissipation_series": ([vp, ctypes.c_int, ctypes.c_longlong, ctypes.c_int, dp],
+                                            ctypes.c_int),
+            "odis_series_capacity": ([vp], ctypes.c_int),
+            "odis_get_velocity_en": ([vp, ctypes.c_int, dp, dp], ctypes.c_int),
+            "odis_request_stop": ([vp], ctypes.c_int),
+            "odis_enable_timing": ([vp, ctypes.c_int], ctypes.c_int),
+            "odis_get_timing": ([vp, dp, dp, dp], ctypes.c_int),
+            "odis_grid_read": ([ctypes.c_char_p, vpp], ctypes.c_int),
+            "odis_grid_generate": ([ctypes.c_int, ctypes.c_int, vpp], ctypes.c_int),
+            "odis_grid_write": ([vp, ctypes.c_char_p], ctypes.c_int),
+            "odis_grid_size": ([vp], ctypes.c_int),
+            "odis_grid_arrays": ([vp, dp, dp, ip, dp, dp], ctypes.c_int),
+            "odis_grid_from_arrays": ([ctypes.c_int, dp, dp, ip, dp, dp, vpp], ctypes.c_int),
+            "odis_grid_free": ([vp], ctypes.c_int),
+            "odis_mesh_build": ([vp, ctypes.c_double, vpp], ctypes.c_int),
+            "odis_mesh_sizes": ([vp, ip, ip, ip], ctypes.c_int),
+            "odis_mesh_describe": ([vp, ctypes.POINTER(MeshDesc)], ctypes.c_int),
+            "odis_mesh_get": ([vp, ctypes.c_char_p, vp], ctypes.c_int),
+            "odis_mesh_free": ([vp], ctypes.c_int),
+            "odis_round_period": ([ctypes.c_double, dp, dp], ctypes.c_int),
+            "odis_quantise_dt": ([ctypes.c_double, ctypes.c_double, dp, ip], ctypes.c_int),
+            "odis_config_read": ([ctypes.c_char_p, ctypes.POINTER(Params), ip, dp, ip, ip],
+                                 ctypes.c_int),
+            "odis_host_last_error": ([], ctypes.c_char_p),
+            "odis_run_case": ([ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def _host_check(rc):
+    if rc < 0:
+        raise OdisError(rc, lib().odis_host_last_error().decode())
+    return rc
+
+
+# ---------------------------------------------------------------------------
+# grid and mesh (host)
+# ---------------------------------------------------------------------------
+
+class Grid:
+    """Grid in the reference's file layout (src/mesh.cpp:4016-4101)."""
+
+    def __init__(self, handle):
+        self._h = handle
+        self.n = lib().odis_grid_size(handle)
+
+    @classmethod
+    def generate(cls, level, lloyd_iters=64):
+        h = ctypes.c_void_p()
+        _host_check(lib().odis_grid_generate(int(level), int(lloyd_iters), ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def read(cls, path):
+        h = ctypes.c_void_p()
+        _host_check(lib().odis_grid_read(str(path).encode(), ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_arrays(cls, lat, lon, friends, clat, clon):
+        lat = np.ascontiguousarray(lat, np.float64)
+        lon = np.ascontiguousarray(lon, np.float64)
+        fr = np.ascontiguousarray(friends, np.int32)
+        clat = np.ascontiguousarray(clat, np.float64)
+        clon = np.ascontiguousarray(clon, np.float64)
+        h = ctypes.c_void_p()
+        _host_check(lib().odis_grid_from_arrays(len(lat), _dp(lat), _dp(lon), _ip(fr), _dp(clat),
+                                                _dp(clon), ctypes.byref(h)))
+        return cls(h)
+
+    def arrays(self):
+        n = self.n
+        lat, lon = np.zeros(n), np.zeros(n)
+        fr = np.zeros((n, 6), np.int32)
+        clat, clon = np.zeros((n, 6)), np.zeros((n, 6))
+        lib().odis_grid_arrays(self._h, _dp(lat), _dp(lon), _ip(fr), _dp(clat), _dp(clon))
+        return dict(lat=lat, lon=lon, friends=fr, clat=clat, clon=clon)
+
+    def write(self, path):
+        _host_check(lib().odis_grid_write(self._h, str(path).encode()))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().odis_grid_free(self._h)
+            self._h = None
+
+
+_MESH_INT = {"faces", "node_face_dir", "face_nodes", "interp_count", "interp_friends"}
+
+
+class Mesh:
+    """TRiSK C-grid operands built on the host (Mesh::Mesh, src/mesh.cpp:32-197)."""
+
+    def __init__(self, grid, radius):
+        h = ctypes.c_void_p()
+        _host_check(lib().odis_mesh_build(grid._h, float(radius), ctypes.byref(h)))
+        self._h = h
+        n, e, v = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        lib().odis_mesh_sizes(h, ctypes.byref(n), ctypes.byref(e), ctypes.byref(v))
+        self.N, self.E, self.V = n.value, e.value, v.value
+        self.radius = float(radius)
+
+    def get(self, name):
+        cnt = _host_check(lib().odis_mesh_get(self._h, name.encode(), None))
+        a = np.zeros(cnt, np.int32 if name in _MESH_INT else np.float64)
+        lib().odis_mesh_get(self._h, name.encode(), a.ctypes.data_as(ctypes.c_void_p))
+        return a
+
+    def describe(self):
+        d = MeshDesc()
+        lib().odis_mesh_describe(self._h, ctypes.byref(d))
+        return d
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().odis_mesh_free(self._h)
+            self._h = None
+
+
+# ---------------------------------------------------------------------------
+# configuration
+# ---------------------------------------------------------------------------
+
+def round_period(ang_vel):
+    p, o = ctypes.c_double(), ctypes.c_double()
+    _host_check(lib().odis_round_period(float(ang_vel), ctypes.byref(p), ctypes.byref(o)))
+    return p.value, o.value
+
+
+def quantise_dt(period, target):
+    d, n = ctypes.c_double(), ctypes.c_int()
+    _host_check(lib().odis_quantise_dt(float(period), float(target), ctypes.byref(d), ctypes.byref(n)))
+    return d.value, n.value
+
+
+def read_config(case_dir):
+    """Parse input.in (Globals semantics); returns (Params, info dict)."""
+    p = Params()
+    steps, lvl, outt = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    end = ctypes.c_double()
+    _host_check(lib().odis_config_read(str(case_dir).encode(), ctypes.byref(p), ctypes.byref(steps),
+                                       ctypes.byref(end), ctypes.byref(lvl), ctypes.byref(outt)))
+    return p, dict(steps_per_orbit=steps.value, end_time=end.value, grid_level=lvl.value,
+                   output_time=outt.value)
+
+
+def make_params(radius=252.1e3, g=0.113, ang_vel=5.307e-5, h=38e3, alpha=1e-8, ecc=0.0047,
+                obliquity=0.0, tide=ECC, surface=FREE, friction=LINEAR, love_reduct=1.0,
+                time_step=30.0, shell_thickness=0.0, semimajor=0.0, init_load=0, quad_drag=0):
+    """Params with the reference's derived values: period rounding and dt
+    quantisation (src/globals.cpp:209-214, src/mesh.cpp:1566-1623).
+    Defaults are the Enceladus free-surface ocean of the plain-method case."""
+    period, omega = round_period(ang_vel)
+    dt, n = quantise_dt(period, time_step)
+    p = Params(g=g, h=h, alpha=alpha, omega=omega, radius=radius, love_reduct=love_reduct,
+               ecc=ecc, obliquity=obliquity, dt=dt, shell_thickness=shell_thickness,
+               semimajor=semimajor, tide=tide, surface=surface, friction=friction,
+               init_load=init_load, quad_drag=quad_drag)
+    return p, n
+
+
+# ---------------------------------------------------------------------------
+# device context (the time loop)
+# ---------------------------------------------------------------------------
+
+class Context:
+    """GPU LTE stepper over one mesh and one or more instances
+    (odis_create ... odis_destroy)."""
+
+    def __init__(self, mesh, params, precision=FP64):
+        if isinstance(params, Params):
+            params = [params]
+        self.mesh = mesh
+        self.B = len(params)
+        arr = (Params * self.B)(*params)
+        self.params = list(params)
+        d = mesh.describe()
+        h = ctypes.c_void_p()
+        rc = lib().odis_create(ctypes.byref(d), arr, self.B, int(precision), ctypes.byref(h))
+        if rc:
+            raise OdisError(rc, lib().odis_last_error(None).decode())
+        self._h = h
+        self.precision = precision
+
+    def _check(self, rc):
+        if rc:
+            raise OdisError(rc, lib().odis_last_error(self._h).decode())
+
+    def set_state(self, inst=0, v=None, dvdt=None, eta=None, detadt=None, iter=0):
+        arrs = [None if a is None else np.ascontiguousarray(a, np.float64).ravel()
+                for a in (v, dvdt, eta, detadt)]
+        self._check(lib().odis_set_state(self._h, int(inst), *[_dp(a) for a in arrs], int(iter)))
+
+    def get_state(self, inst=0):
+        m = self.mesh
+        v, dv = np.zeros(m.E), np.zeros(m.E * 3)
+        eta, de = np.zeros(m.N), np.zeros(m.N * 3)
+        it = ctypes.c_longlong()
+        self._check(lib().odis_get_state(self._h, int(inst), _dp(v), _dp(dv), _dp(eta), _dp(de),
+                                         ctypes.byref(it)))
+        return dict(v=v, dvdt=dv.reshape(m.E, 3), eta=eta, detadt=de.reshape(m.N, 3),
+                    iter=it.value)
+
+    def step(self, n, sync=True):
+        self._check(lib().odis_step(self._h, int(n), 1 if sync else 0))
+
+    def sync(self):
+        self._check(lib().odis_sync(self._h))
+
+    def dissipation(self):
+        out = np.zeros(self.B)
+        self._check(lib().odis_get_dissipation(self._h, _dp(out)))
+        return out
+
+    def dissipation_series(self, inst, first, count):
+        out = np.zeros(count)
+        self._check(lib().odis_get_dissipation_series(self._h, int(inst), int(first), int(count),
+                                                      _dp(out)))
+        return out
+
+    def velocity_en(self, inst=0):
+        normal = np.ascontiguousarray(self.mesh.get("face_normal"))
+        uv = np.zeros(self.mesh.E * 2)
+        self._check(lib().odis_get_velocity_en(self._h, int(inst), _dp(normal), _dp(uv)))
+        return uv.reshape(-1, 2)
+
+    def request_stop(self):
+        self._check(lib().odis_request_stop(self._h))
+
+    def enable_timing(self, on=True):
+        self._check(lib().odis_enable_timing(self._h, 1 if on else 0))
+
+    def timing(self):
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._check(lib().odis_get_timing(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return dict(face_ms=a.value, cell_ms=b.value, step_ms=c.value)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().odis_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def run_case(case_dir, grid_dir=None, precision=FP64):
+    """solveODIS-shaped driver over a case directory holding input.in."""
+    rc = lib().odis_run_case(str(case_dir).encode(),
+                             None if grid_dir is None else str(grid_dir).encode(), int(precision))
+    if rc < 0:
+        raise OdisError(rc, "see %s/DATA/ERROR.txt" % case_dir)
+    return rc

@@ -1,0 +1,943 @@
+// odis_device.hip -- HIP kernels and the device context behind odis.h.
+//
+// Replaces the body of ab3Explicit (src/timeIntegrator.cpp:57-322) for one
+// or more ocean instances sharing one grid.  See odis_kernels.hpp for the
+// per-step dataflow.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "odis.h"
+#include "odis_kernels.hpp"
+
+namespace odis {
+
+// ===========================================================================
+// kernels
+// ===========================================================================
+
+template <typename T>
+__device__ __forceinline__ double block_sum(double x, double *sh) {
+  // fixed-shape tree: wave-level shuffle then LDS across the 4 waves
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) sh[wave] = x;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  return r;
+}
+
+// Face kernel.  DISS_ONLY: evaluate the dissipation partials of the current
+// state without stepping (output cadence / final state).
+template <typename T, bool DISS_ONLY>
+__global__ __launch_bounds__(kBlock) void k_face(FaceArgs<T> a, int energy_quad) {
+  __shared__ double sh[4];
+  const int b = blockIdx.y;
+  const int tile = xcd_tile(blockIdx.x, a.nb);
+  const int E = a.E;
+  const int e = tile * kBlock + threadIdx.x;
+  const StepInfo *inf = a.info + b;
+  const long long n = inf->step;
+  const double *ip = a.inst + b * 8;
+  double local = 0.0;
+  if (e < E) {
+    const T *v = a.v_cur + b * a.v_stride;
+    T vt = T(0);
+#pragma unroll
+    for (int j = 0; j < 10; ++j) vt += a.tri_coef[j * E + e] * v[a.tri_idx[j * E + e]];
+    const T ve = v[e];
+    const T sp = ve * ve + vt * vt;
+    const T en = energy_quad ? sqrt(sp) * sp : sp;
+    local = double(a.farea[e]) * double(en);
+    if (!DISS_ONLY) {
+      const T g = T(ip[0]), alpha = T(ip[1]), dt = T(ip[2]), qd = T(ip[4]);
+      const bool ab3 = (n >= 2) || (ip[3] != 0.0);
+      const T *eta = a.eta + b * a.eta_stride;
+      const T ge = a.g_coef[e] * eta[a.g_idx[e]] + a.g_coef[E + e] * eta[a.g_idx[E + e]];
+      T f0 = -g * ge + a.fcor[e] * vt;
+      if (a.quad_drag) f0 -= qd * sqrt(sp) * ve;
+      T fu = T(0);
+      for (int k = 0; k < a.nfields; ++k) fu += T(inf->coef[k]) * a.fgs[k * E + e];
+      const T drag = -alpha * ve + fu;
+      T *ring = a.ring + b * a.ring_stride;
+      const int s0 = int(n % 3), s1 = int((n + 2) % 3), s2 = int((n + 1) % 3);
+      T upd;
+      if (ab3)
+        upd = (T(23. / 12.) * f0 + T(-16. / 12.) * ring[s1 * E + e] + T(5. / 12.) * ring[s2 * E + e]) * dt;
+      else
+        upd = f0 * dt;
+      ring[s0 * E + e] = f0;
+      a.v_nxt[b * a.v_stride + e] = (ve + upd) + dt * drag;
+    }
+  }
+  const double s = block_sum<T>(local, sh);
+  if (threadIdx.x == 0) a.partials[b * a.nb + tile] = s;
+}
+
+// Sum the face kernel's partials of instance b in a fixed order.
+__device__ __forceinline__ double sum_partials(const double *p, int count, double *sh) {
+  double acc = 0.0;
+  for (int j = threadIdx.x; j < count; j += kBlock) acc += p[j];
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) sh[wave] = acc;
+  __syncthreads();
+  return (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cell(CellArgs<T> a) {
+  __shared__ double sh[4];
+  __shared__ unsigned int s_ticket;
+  const int b = blockIdx.y;
+  const int tile = xcd_tile(blockIdx.x, a.nb);
+  const int N = a.N;
+  const int i = tile * kBlock + threadIdx.x;
+  StepInfo *inf = a.info + b;
+  const long long n = inf->step;
+  const double *ip = a.inst + b * 8;
+  if (i < N) {
+    const T *v = a.v_new + b * a.v_stride;
+    T s = T(0);
+#pragma unroll
+    for (int j = 0; j < 6; ++j) s += a.div_coef[j * N + i] * v[a.div_idx[j * N + i]];
+    const T h = T(ip[5]), dt = T(ip[2]);
+    const bool ab3 = (n >= 2) || (ip[3] != 0.0);
+    const T f0 = h * s;
+    T *ring = a.ring + b * a.ring_stride;
+    const int s0 = int(n % 3), s1 = int((n + 2) % 3), s2 = int((n + 1) % 3);
+    T upd;
+    if (ab3)
+      upd = (T(23. / 12.) * f0 + T(-16. / 12.) * ring[s1 * N + i] + T(5. / 12.) * ring[s2 * N + i]) * dt;
+    else
+      upd = f0 * dt;
+    ring[s0 * N + i] = f0;
+    T *eta = a.eta + b * a.eta_stride;
+    eta[i] = eta[i] + upd;
+  }
+  // last workgroup of this instance finalises the step
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) s_ticket = atomicAdd(&inf->ticket, 1u);
+  __syncthreads();
+  if (s_ticket != unsigned(a.nb - 1)) return;
+  __threadfence();
+  const double tot = sum_partials(a.partials + size_t(b) * a.nb_face, a.nb_face, sh);
+  if (threadIdx.x == 0) {
+    const double d = tot * a.dconst[b];
+    a.series[size_t(b) * a.cap + size_t(n % a.cap)] = d;
+    inf->last_diss = d;
+    if (!isfinite(d)) inf->nonfinite = 1;
+    double c[kMaxForcing] = {0, 0, 0, 0, 0, 0, 0};
+    const double dt = ip[2];
+    tide_coeffs(a.tide[b], dt * double(n + 1) + dt, c);
+    for (int k = 0; k < kMaxForcing; ++k) inf->coef[k] = c[k];
+    inf->ticket = 0u;
+    inf->step = n + 1;
+  }
+}
+
+// Forcing coefficients of the pending step (after create / set_state).
+__global__ void k_prologue(StepInfo *info, const TideDev *tide, const double *inst, int B) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  double c[kMaxForcing] = {0, 0, 0, 0, 0, 0, 0};
+  const double dt = inst[b * 8 + 2];
+  tide_coeffs(tide[b], dt * double(info[b].step) + dt, c);
+  for (int k = 0; k < kMaxForcing; ++k) info[b].coef[k] = c[k];
+  info[b].ticket = 0u;
+}
+
+// Sum dissipation partials produced by k_face<DISS_ONLY>.
+__global__ __launch_bounds__(kBlock) void k_diss_final(const double *partials, int nb, const double *dconst,
+                                                       double *out) {
+  __shared__ double sh[4];
+  const int b = blockIdx.x;
+  const double t = sum_partials(partials + size_t(b) * nb, nb, sh);
+  if (threadIdx.x == 0) out[b] = t * dconst[b];
+}
+
+// (east, north) velocity per face: n v_n + t v_t with t = (n_y, -n_x)
+// (src/interpolation.cpp:49-58).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_velocity_en(int E, const int *tri_idx, const T *tri_coef,
+                                                        const T *v, const double *normal, double *uv) {
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= E) return;
+  double vt = 0.0;
+  for (int j = 0; j < 10; ++j) vt += double(tri_coef[j * E + e]) * double(v[tri_idx[j * E + e]]);
+  const double nx = normal[2 * e], ny = normal[2 * e + 1], vn = double(v[e]);
+  uv[2 * e] = nx * vn + ny * vt;
+  uv[2 * e + 1] = ny * vn + (-nx) * vt;
+}
+
+}  // namespace odis
+
+// ===========================================================================
+// context
+// ===========================================================================
+
+using namespace odis;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct DevBuf {
+  void *p = nullptr;
+};
+
+template <typename T>
+struct Ops {
+  int *tri_idx = nullptr;
+  T *tri_coef = nullptr;
+  int *g_idx = nullptr;
+  T *g_coef = nullptr;
+  T *fcor = nullptr, *farea = nullptr, *fgs = nullptr;
+  int *div_idx = nullptr;
+  T *div_coef = nullptr;
+  T *v[2] = {nullptr, nullptr};
+  T *ring_v = nullptr, *eta = nullptr, *ring_e = nullptr;
+};
+
+uint64_t hilbert3(uint32_t X0, uint32_t X1, uint32_t X2, int bits) {
+  uint32_t X[3] = {X0, X1, X2};
+  uint32_t M = 1u << (bits - 1), P, Q, t;
+  for (Q = M; Q > 1; Q >>= 1) {
+    P = Q - 1;
+    for (int i = 0; i < 3; i++) {
+      if (X[i] & Q) {
+        X[0] ^= P;
+      } else {
+        t = (X[0] ^ X[i]) & P;
+        X[0] ^= t;
+        X[i] ^= t;
+      }
+    }
+  }
+  for (int i = 1; i < 3; i++) X[i] ^= X[i - 1];
+  t = 0;
+  for (Q = M; Q > 1; Q >>= 1)
+    if (X[2] & Q) t ^= Q - 1;
+  for (int i = 0; i < 3; i++) X[i] ^= t;
+  uint64_t key = 0;
+  for (int b = bits - 1; b >= 0; b--)
+    for (int i = 0; i < 3; i++) key = (key << 1) | ((X[i] >> b) & 1u);
+  return key;
+}
+
+}  // namespace
+
+struct odis_ctx {
+  int N = 0, E = 0, B = 0, precision = ODIS_FP64;
+  int nb_face = 0, nb_cell = 0;
+  int nfields = 0;
+  int energy_quad = 0, quad_drag = 0;
+  hipStream_t stream = nullptr;
+  std::vector<int> cell_perm, cell_inv, face_perm, face_inv;  // ref -> dev, dev -> ref
+  Ops<double> d;
+  Ops<float> f;
+  int cur = 0;
+  StepInfo *info = nullptr;
+  double *partials = nullptr;
+  double *series = nullptr;
+  int cap = 1 << 16;
+  double *inst = nullptr;
+  TideDev *tide = nullptr;
+  double *dconst = nullptr;
+  double *scratch_out = nullptr;
+  std::vector<odis_params> params;
+  std::vector<std::vector<double>> init_hist_v, init_hist_e;  // history as last set ([M][3])
+  std::vector<long long> host_step;
+  std::vector<void *> allocs;
+  volatile int stop_flag = 0;
+  int timing = 0;
+  std::vector<hipEvent_t> ev;
+  double t_face = 0, t_cell = 0, t_step = 0;
+  std::string err;
+};
+
+#define CK(x)                                                                              \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess) {                                                                \
+      return fail(ctx, ODIS_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_));        \
+    }                                                                                      \
+  } while (0)
+
+static int fail(odis_ctx *ctx, int code, const std::string &msg) {
+  if (ctx) ctx->err = msg;
+  g_last_error = msg;
+  return code;
+}
+
+template <typename T>
+static int dev_alloc(odis_ctx *ctx, T **p, size_t count) {
+  void *q = nullptr;
+  hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+  if (e != hipSuccess) return fail(ctx, ODIS_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  ctx->allocs.push_back(q);
+  *p = static_cast<T *>(q);
+  return ODIS_OK;
+}
+
+template <typename T>
+static int upload(odis_ctx *ctx, T **p, const std::vector<T> &h) {
+  int rc = dev_alloc(ctx, p, h.size());
+  if (rc) return rc;
+  CK(hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return ODIS_OK;
+}
+
+template <typename T>
+static std::vector<T> cast_vec(const std::vector<double> &x) {
+  return std::vector<T>(x.begin(), x.end());
+}
+
+// Spatial fields S_k of each potential, evaluated from the reference's trig
+// tables (src/mesh.cpp:2122-2152); the time coefficients are tide_coeffs.
+static int tide_fields(int tide, double lat, double lon, double *S) {
+  const double cl = std::cos(lat), sl = std::sin(lat), co = std::cos(lon), so = std::sin(lon);
+  const double c2l = std::cos(2.0 * lat), s2l = std::sin(2.0 * lat);
+  const double c2o = std::cos(2.0 * lon), s2o = std::sin(2.0 * lon);
+  const double cq = std::pow(std::cos(lat), 2.0), sq = std::pow(std::sin(lat), 2.0);
+  const double coq = std::pow(std::cos(lon), 2.0), soq = std::pow(std::sin(lon), 2.0);
+  switch (tide) {
+    case ODIS_ECC:
+      S[0] = (1. - 3. * sq) + cq * (3. * c2o);
+      S[1] = cq * (4. * s2o);
+      return 2;
+    case ODIS_OBLIQ:
+      S[0] = s2l * co;
+      return 1;
+    case ODIS_OBLIQ_WEST:
+      S[0] = sl * cl * co;
+      S[1] = sl * cl * so;
+      return 2;
+    case ODIS_FULL:
+      S[0] = (1. - 3. * sq) + cq * (3. * c2o);
+      S[1] = cq * (4. * s2o);
+      S[2] = s2l * co;
+      return 3;
+    case ODIS_FULL2:
+      S[0] = -(1 - 3 * c2l);
+      S[1] = cl * sl * co;
+      S[2] = cl * sl * so;
+      S[3] = cq * co;
+      S[4] = cq * so;
+      S[5] = cq * c2o;
+      S[6] = cq * s2o;
+      return 7;
+    case ODIS_PLANET:
+      S[0] = cl * cl * coq;
+      S[1] = cl * cl * so * co;
+      S[2] = cl * cl * soq;
+      S[3] = 1.0;
+      return 4;
+    default:
+      return 0;
+  }
+}
+
+static TideDev make_tide(const odis_params &p) {
+  TideDev t{};
+  t.tide = p.tide;
+  t.omega = p.omega;
+  t.dt = p.dt;
+  double R = p.radius;
+  if (p.surface == ODIS_LID_LOVE || p.surface == ODIS_LID_MEMBR) R += p.shell_thickness;
+  const double base = p.love_reduct * std::pow(p.omega, 2.0) * std::pow(R, 2.0);
+  t.f_ecc = 0.75 * base * p.ecc;
+  t.f_obl = -3. / 2. * base * p.obliquity;
+  t.f_west = 0.5 * base * p.obliquity;
+  t.f_full2 = 1 / 32. * base;
+  t.ecc = p.ecc;
+  t.obl = p.obliquity;
+  t.pl_a1 = 421800000.0;
+  t.pl_a2 = p.semimajor;
+  t.pl_gm = 6.67408e-11 * 8.931938e+22;
+  t.radius = R;
+  return t;
+}
+
+template <typename T>
+static int build_ops(odis_ctx *ctx, Ops<T> &o, const std::vector<double> &tri_c, const std::vector<int> &tri_i,
+                     const std::vector<double> &gc, const std::vector<int> &gi, const std::vector<double> &fcor,
+                     const std::vector<double> &farea, const std::vector<double> &fgs,
+                     const std::vector<double> &dc, const std::vector<int> &di) {
+  int rc;
+  if ((rc = upload(ctx, &o.tri_idx, tri_i))) return rc;
+  if ((rc = upload(ctx, &o.tri_coef, cast_vec<T>(tri_c)))) return rc;
+  if ((rc = upload(ctx, &o.g_idx, gi))) return rc;
+  if ((rc = upload(ctx, &o.g_coef, cast_vec<T>(gc)))) return rc;
+  if ((rc = upload(ctx, &o.fcor, cast_vec<T>(fcor)))) return rc;
+  if ((rc = upload(ctx, &o.farea, cast_vec<T>(farea)))) return rc;
+  if ((rc = upload(ctx, &o.fgs, cast_vec<T>(fgs)))) return rc;
+  if ((rc = upload(ctx, &o.div_idx, di))) return rc;
+  if ((rc = upload(ctx, &o.div_coef, cast_vec<T>(dc)))) return rc;
+  const size_t E = ctx->E, N = ctx->N, B = ctx->B;
+  if ((rc = dev_alloc(ctx, &o.v[0], B * E))) return rc;
+  if ((rc = dev_alloc(ctx, &o.v[1], B * E))) return rc;
+  if ((rc = dev_alloc(ctx, &o.ring_v, B * 3 * E))) return rc;
+  if ((rc = dev_alloc(ctx, &o.eta, B * N))) return rc;
+  if ((rc = dev_alloc(ctx, &o.ring_e, B * 3 * N))) return rc;
+  CK(hipMemset(o.v[0], 0, B * E * sizeof(T)));
+  CK(hipMemset(o.v[1], 0, B * E * sizeof(T)));
+  CK(hipMemset(o.ring_v, 0, B * 3 * E * sizeof(T)));
+  CK(hipMemset(o.eta, 0, B * N * sizeof(T)));
+  CK(hipMemset(o.ring_e, 0, B * 3 * N * sizeof(T)));
+  return ODIS_OK;
+}
+
+static int launch_prologue(odis_ctx *ctx) {
+  hipLaunchKernelGGL(k_prologue, dim3((ctx->B + 63) / 64), dim3(64), 0, ctx->stream, ctx->info, ctx->tide,
+                     ctx->inst, ctx->B);
+  CK(hipGetLastError());
+  return ODIS_OK;
+}
+
+extern "C" int odis_create(const odis_mesh_desc *m, const odis_params *params, int n_inst, int precision,
+                           odis_ctx **out) {
+  odis_ctx *ctx = nullptr;
+  if (!m || !params || !out || n_inst < 1) return fail(nullptr, ODIS_E_INVALID, "null argument or n_inst < 1");
+  if (precision != ODIS_FP64 && precision != ODIS_FP32) return fail(nullptr, ODIS_E_INVALID, "bad precision");
+  const int N = m->n_cells, E = m->n_faces;
+  if (N < 12 || E != 3 * N - 6) return fail(nullptr, ODIS_E_INVALID, "mesh sizes: need E = 3N-6");
+  if (!m->face_nodes || !m->grad_coef || !m->trisk_count || !m->trisk_faces || !m->trisk_coef ||
+      !m->face_lat || !m->face_area || !m->cell_faces || !m->div_coef || !m->cell_lat || !m->cell_lon)
+    return fail(nullptr, ODIS_E_INVALID, "mesh descriptor has a null array");
+  for (int b = 0; b < n_inst; b++) {
+    const odis_params &p = params[b];
+    if (p.tide == ODIS_GENERAL)
+      return fail(nullptr, ODIS_E_UNSUPPORTED,
+                  "GENERAL forcing needs the Legendre tables of CalcLegendreFuncs, which the reference leaves "
+                  "commented out");
+    if (p.tide != params[0].tide || p.omega != params[0].omega || p.friction != params[0].friction ||
+        p.quad_drag != params[0].quad_drag)
+      return fail(nullptr, ODIS_E_UNSUPPORTED, "instances of one context must share tide type, omega and drag law");
+    if (!(p.dt > 0) || !(p.h > 0) || !(p.radius > 0))
+      return fail(nullptr, ODIS_E_INVALID, "dt, h and radius must be positive");
+  }
+  // validate index ranges on the host before anything reaches a kernel
+  for (int e = 0; e < E; e++) {
+    for (int k = 0; k < 2; k++)
+      if (m->face_nodes[2 * e + k] < 0 || m->face_nodes[2 * e + k] >= N)
+        return fail(nullptr, ODIS_E_INVALID, "face_nodes out of range");
+    const int c = m->trisk_count[e];
+    if (c < 0 || c > 10) return fail(nullptr, ODIS_E_INVALID, "trisk_count out of range");
+    for (int j = 0; j < c; j++)
+      if (m->trisk_faces[10 * e + j] < 0 || m->trisk_faces[10 * e + j] >= E)
+        return fail(nullptr, ODIS_E_INVALID, "trisk_faces out of range");
+  }
+  for (int i = 0; i < N; i++)
+    for (int j = 0; j < 6; j++) {
+      const int f = m->cell_faces[6 * i + j];
+      if (f >= E || (f < 0 && j < 5)) return fail(nullptr, ODIS_E_INVALID, "cell_faces out of range");
+    }
+
+  ctx = new odis_ctx();
+  ctx->N = N;
+  ctx->E = E;
+  ctx->B = n_inst;
+  ctx->precision = precision;
+  ctx->params.assign(params, params + n_inst);
+  ctx->energy_quad = params[0].friction == ODIS_QUADRATIC;
+  ctx->quad_drag = params[0].quad_drag != 0;
+  ctx->init_hist_v.assign(n_inst, std::vector<double>(size_t(3) * E, 0.0));
+  ctx->init_hist_e.assign(n_inst, std::vector<double>(size_t(3) * N, 0.0));
+  ctx->host_step.assign(n_inst, 0);
+
+  // -- locality renumbering: cells along a 3-D Hilbert curve, faces by the
+  //    ranks of their two cells
+  std::vector<std::pair<uint64_t, int>> ck(N);
+  for (int i = 0; i < N; i++) {
+    const double la = m->cell_lat[i], lo = m->cell_lon[i];
+    const double x = std::cos(la) * std::cos(lo), y = std::cos(la) * std::sin(lo), z = std::sin(la);
+    const double s = double((1u << 20) - 1);
+    auto q = [&](double v) { return uint32_t(std::lround((v + 1.0) * 0.5 * s)); };
+    ck[i] = {hilbert3(q(x), q(y), q(z), 20), i};
+  }
+  std::stable_sort(ck.begin(), ck.end());
+  ctx->cell_perm.resize(N);
+  ctx->cell_inv.resize(N);
+  for (int r = 0; r < N; r++) {
+    ctx->cell_inv[r] = ck[r].second;
+    ctx->cell_perm[ck[r].second] = r;
+  }
+  std::vector<std::pair<std::pair<int, int>, int>> fk(E);
+  for (int e = 0; e < E; e++) {
+    const int a = ctx->cell_perm[m->face_nodes[2 * e]], b = ctx->cell_perm[m->face_nodes[2 * e + 1]];
+    fk[e] = {{std::min(a, b), std::max(a, b)}, e};
+  }
+  std::stable_sort(fk.begin(), fk.end());
+  ctx->face_perm.resize(E);
+  ctx->face_inv.resize(E);
+  for (int r = 0; r < E; r++) {
+    ctx->face_inv[r] = fk[r].second;
+    ctx->face_perm[fk[r].second] = r;
+  }
+
+  // -- operands in device order ----------------------------------------------
+  const odis_params &p0 = params[0];
+  std::vector<double> tri_c(size_t(10) * E, 0.0), gc(size_t(2) * E), fcor(E), farea(E);
+  std::vector<int> tri_i(size_t(10) * E), gi(size_t(2) * E);
+  double S[kMaxForcing];
+  const int K = tide_fields(p0.tide, 0.0, 0.0, S);
+  ctx->nfields = K;
+  std::vector<double> cellS(size_t(K) * N);
+  for (int i = 0; i < N; i++) {
+    tide_fields(p0.tide, m->cell_lat[i], m->cell_lon[i], S);
+    for (int k = 0; k < K; k++) cellS[size_t(k) * N + i] = S[k];
+  }
+  std::vector<double> fgs(size_t(std::max(K, 1)) * E, 0.0);
+  for (int r = 0; r < E; r++) {
+    const int e = ctx->face_inv[r];
+    const int c = m->trisk_count[e];
+    for (int j = 0; j < 10; j++) {
+      if (j < c) {
+        tri_i[size_t(j) * E + r] = ctx->face_perm[m->trisk_faces[10 * e + j]];
+        tri_c[size_t(j) * E + r] = m->trisk_coef[10 * e + j];
+      } else {
+        tri_i[size_t(j) * E + r] = r;  // padded slot: own face, zero weight
+        tri_c[size_t(j) * E + r] = 0.0;
+      }
+    }
+    const int n0 = m->face_nodes[2 * e], n1 = m->face_nodes[2 * e + 1];
+    gi[r] = ctx->cell_perm[n0];
+    gi[size_t(E) + r] = ctx->cell_perm[n1];
+    gc[r] = m->grad_coef[2 * e];
+    gc[size_t(E) + r] = m->grad_coef[2 * e + 1];
+    fcor[r] = -2.0 * p0.omega * std::sin(m->face_lat[e]);
+    farea[r] = m->face_area[e];
+    for (int k = 0; k < K; k++)
+      fgs[size_t(k) * E + r] = m->grad_coef[2 * e] * cellS[size_t(k) * N + n0] +
+                               m->grad_coef[2 * e + 1] * cellS[size_t(k) * N + n1];
+  }
+  std::vector<double> dc(size_t(6) * N, 0.0);
+  std::vector<int> di(size_t(6) * N, 0);
+  for (int r = 0; r < N; r++) {
+    const int i = ctx->cell_inv[r];
+    const int f0 = ctx->face_perm[m->cell_faces[6 * i]];
+    for (int j = 0; j < 6; j++) {
+      const int f = m->cell_faces[6 * i + j];
+      if (f >= 0) {
+        di[size_t(j) * N + r] = ctx->face_perm[f];
+        dc[size_t(j) * N + r] = m->div_coef[6 * i + j];
+      } else {
+        di[size_t(j) * N + r] = f0;
+        dc[size_t(j) * N + r] = 0.0;
+      }
+    }
+  }
+
+  int rc = ODIS_OK;
+  hipError_t he = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (he != hipSuccess) {
+    rc = fail(ctx, ODIS_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
+  }
+  if (!rc) {
+    if (precision == ODIS_FP64)
+      rc = build_ops(ctx, ctx->d, tri_c, tri_i, gc, gi, fcor, farea, fgs, dc, di);
+    else
+      rc = build_ops(ctx, ctx->f, tri_c, tri_i, gc, gi, fcor, farea, fgs, dc, di);
+  }
+  ctx->nb_face = (E + kBlock - 1) / kBlock;
+  ctx->nb_cell = (N + kBlock - 1) / kBlock;
+  // per-instance parameters
+  std::vector<double> inst(size_t(8) * n_inst, 0.0), dconst(n_inst);
+  std::vector<TideDev> tides(n_inst);
+  for (int b = 0; b < n_inst && !rc; b++) {
+    const odis_params &p = params[b];
+    double *q = &inst[size_t(8) * b];
+    q[0] = p.g;
+    q[1] = p.quad_drag ? 0.0 : p.alpha;
+    q[2] = p.dt;
+    q[3] = p.init_load ? 1.0 : 0.0;
+    q[4] = p.quad_drag ? p.alpha / p.h : 0.0;
+    q[5] = p.h;
+    const double area = 4 * kPi * std::pow(p.radius, 2.0);
+    dconst[b] = (p.friction == ODIS_QUADRATIC ? p.alpha / p.h : p.alpha * 1000.0 * p.h) / area;
+    tides[b] = make_tide(p);
+    tides[b].nfields = K;
+  }
+  if (!rc) rc = upload(ctx, &ctx->inst, inst);
+  if (!rc) rc = upload(ctx, &ctx->dconst, dconst);
+  if (!rc) rc = upload(ctx, &ctx->tide, tides);
+  if (!rc) rc = dev_alloc(ctx, &ctx->info, size_t(n_inst));
+  if (!rc) rc = dev_alloc(ctx, &ctx->partials, size_t(n_inst) * ctx->nb_face);
+  if (!rc) rc = dev_alloc(ctx, &ctx->series, size_t(n_inst) * ctx->cap);
+  if (!rc) rc = dev_alloc(ctx, &ctx->scratch_out, size_t(n_inst));
+  if (!rc) {
+    he = hipMemset(ctx->info, 0, sizeof(StepInfo) * n_inst);
+    if (he == hipSuccess) he = hipMemset(ctx->series, 0, sizeof(double) * n_inst * ctx->cap);
+    if (he != hipSuccess) rc = fail(ctx, ODIS_E_HIP, hipGetErrorString(he));
+  }
+  if (!rc) rc = launch_prologue(ctx);
+  if (!rc) {
+    he = hipStreamSynchronize(ctx->stream);
+    if (he != hipSuccess) rc = fail(ctx, ODIS_E_HIP, hipGetErrorString(he));
+  }
+  if (rc) {
+    g_last_error = ctx->err;
+    odis_destroy(ctx);
+    return rc;
+  }
+  *out = ctx;
+  return ODIS_OK;
+}
+
+extern "C" int odis_destroy(odis_ctx *ctx) {
+  if (!ctx) return ODIS_E_INVALID;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto &e : ctx->ev) (void)hipEventDestroy(e);
+  for (void *p : ctx->allocs) (void)hipFree(p);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return ODIS_OK;
+}
+
+extern "C" const char *odis_last_error(const odis_ctx *ctx) {
+  if (ctx && !ctx->err.empty()) return ctx->err.c_str();
+  return g_last_error.c_str();
+}
+
+// ---------------------------------------------------------------------------
+// state in / out (reference numbering and Array2D history layout)
+// ---------------------------------------------------------------------------
+
+template <typename T>
+static int put_state(odis_ctx *ctx, Ops<T> &o, int b, const double *v, const double *dv, const double *eta,
+                     const double *de, long long k) {
+  const int E = ctx->E, N = ctx->N;
+  const bool load = ctx->params[b].init_load != 0;
+  const size_t bE = size_t(b) * E, bN = size_t(b) * N;
+  if (v) {
+    std::vector<T> h(E);
+    for (int r = 0; r < E; r++) h[r] = T(v[ctx->face_inv[r]]);
+    CK(hipMemcpy(o.v[ctx->cur] + bE, h.data(), E * sizeof(T), hipMemcpyHostToDevice));
+  }
+  if (eta) {
+    std::vector<T> h(N);
+    for (int r = 0; r < N; r++) h[r] = T(eta[ctx->cell_inv[r]]);
+    CK(hipMemcpy(o.eta + bN, h.data(), N * sizeof(T), hipMemcpyHostToDevice));
+  }
+  // ring slots consumed by the next step k (see odis_get_state for the inverse)
+  auto put_hist = [&](T *ring, int M, const std::vector<int> &inv, const double *ds,
+                      std::vector<double> &keep) -> int {
+    std::vector<T> r1(M), r2(M);
+    for (int r = 0; r < M; r++) {
+      r1[r] = T(ds[3 * inv[r] + 1]);
+      r2[r] = T(ds[3 * inv[r] + 2]);
+    }
+    keep.assign(ds, ds + size_t(3) * M);
+    const size_t base = size_t(b) * 3 * M;
+    if (k == 1 && !load) {
+      CK(hipMemcpy(ring + base + 0 * size_t(M), r2.data(), M * sizeof(T), hipMemcpyHostToDevice));
+    } else {
+      const int s1 = int((k + 2) % 3), s2 = int((k + 1) % 3);
+      CK(hipMemcpy(ring + base + size_t(s1) * M, r1.data(), M * sizeof(T), hipMemcpyHostToDevice));
+      CK(hipMemcpy(ring + base + size_t(s2) * M, r2.data(), M * sizeof(T), hipMemcpyHostToDevice));
+    }
+    return ODIS_OK;
+  };
+  int rc;
+  if (dv && (rc = put_hist(o.ring_v, E, ctx->face_inv, dv, ctx->init_hist_v[b]))) return rc;
+  if (de && (rc = put_hist(o.ring_e, N, ctx->cell_inv, de, ctx->init_hist_e[b]))) return rc;
+  return ODIS_OK;
+}
+
+extern "C" int odis_set_state(odis_ctx *ctx, int inst, const double *v, const double *dvdt, const double *eta,
+                              const double *detadt, long long iter) {
+  if (!ctx) return ODIS_E_INVALID;
+  if (inst < 0 || inst >= ctx->B) return fail(ctx, ODIS_E_RANGE, "instance out of range");
+  if (iter < 0) return fail(ctx, ODIS_E_INVALID, "iter < 0");
+  CK(hipStreamSynchronize(ctx->stream));
+  int rc = ctx->precision == ODIS_FP64 ? put_state(ctx, ctx->d, inst, v, dvdt, eta, detadt, iter)
+                                       : put_state(ctx, ctx->f, inst, v, dvdt, eta, detadt, iter);
+  if (rc) return rc;
+  StepInfo si{};
+  si.step = iter;
+  CK(hipMemcpy(ctx->info + inst, &si, sizeof(StepInfo), hipMemcpyHostToDevice));
+  ctx->host_step[inst] = iter;
+  return launch_prologue(ctx);
+}
+
+template <typename T>
+static int get_state_t(odis_ctx *ctx, Ops<T> &o, int b, double *v, double *dv, double *eta, double *de,
+                       long long k) {
+  const int E = ctx->E, N = ctx->N;
+  const bool load = ctx->params[b].init_load != 0;
+  const size_t bE = size_t(b) * E, bN = size_t(b) * N;
+  if (v) {
+    std::vector<T> h(E);
+    CK(hipMemcpy(h.data(), o.v[ctx->cur] + bE, E * sizeof(T), hipMemcpyDeviceToHost));
+    for (int r = 0; r < E; r++) v[ctx->face_inv[r]] = double(h[r]);
+  }
+  if (eta) {
+    std::vector<T> h(N);
+    CK(hipMemcpy(h.data(), o.eta + bN, N * sizeof(T), hipMemcpyDeviceToHost));
+    for (int r = 0; r < N; r++) eta[ctx->cell_inv[r]] = double(h[r]);
+  }
+  auto get_hist = [&](T *ring, int M, const std::vector<int> &inv, double *ds,
+                      const std::vector<double> &keep) -> int {
+    std::vector<T> all(size_t(3) * M);
+    CK(hipMemcpy(all.data(), ring + size_t(b) * 3 * M, all.size() * sizeof(T), hipMemcpyDeviceToHost));
+    for (int r = 0; r < M; r++) {
+      const int i = inv[r];
+      if (k == 0) {  // no step taken since set_state: history as set
+        for (int q = 0; q < 3; q++) ds[3 * i + q] = keep[3 * i + q];
+        continue;
+      }
+      const double last = double(all[size_t((k - 1) % 3) * M + r]);
+      ds[3 * i + 0] = last;
+      if (k == 1 && !load) {
+        ds[3 * i + 1] = keep[3 * i + 1];
+        ds[3 * i + 2] = last;
+      } else {
+        ds[3 * i + 1] = last;
+        ds[3 * i + 2] = double(all[size_t((k + 1) % 3) * M + r]);
+      }
+    }
+    return ODIS_OK;
+  };
+  int rc;
+  if (dv && (rc = get_hist(o.ring_v, E, ctx->face_inv, dv, ctx->init_hist_v[b]))) return rc;
+  if (de && (rc = get_hist(o.ring_e, N, ctx->cell_inv, de, ctx->init_hist_e[b]))) return rc;
+  return ODIS_OK;
+}
+
+extern "C" int odis_get_state(odis_ctx *ctx, int inst, double *v, double *dvdt, double *eta, double *detadt,
+                              long long *iter) {
+  if (!ctx) return ODIS_E_INVALID;
+  if (inst < 0 || inst >= ctx->B) return fail(ctx, ODIS_E_RANGE, "instance out of range");
+  CK(hipStreamSynchronize(ctx->stream));
+  StepInfo si;
+  CK(hipMemcpy(&si, ctx->info + inst, sizeof(StepInfo), hipMemcpyDeviceToHost));
+  const long long k = si.step;
+  if (iter) *iter = k;
+  return ctx->precision == ODIS_FP64 ? get_state_t(ctx, ctx->d, inst, v, dvdt, eta, detadt, k)
+                                     : get_state_t(ctx, ctx->f, inst, v, dvdt, eta, detadt, k);
+}
+
+// ---------------------------------------------------------------------------
+// stepping
+// ---------------------------------------------------------------------------
+
+template <typename T>
+static FaceArgs<T> face_args(odis_ctx *ctx, Ops<T> &o) {
+  FaceArgs<T> a;
+  a.E = ctx->E;
+  a.nb = ctx->nb_face;
+  a.tri_idx = o.tri_idx;
+  a.tri_coef = o.tri_coef;
+  a.g_idx = o.g_idx;
+  a.g_coef = o.g_coef;
+  a.fcor = o.fcor;
+  a.farea = o.farea;
+  a.fgs = o.fgs;
+  a.nfields = ctx->nfields;
+  a.v_cur = o.v[ctx->cur];
+  a.v_nxt = o.v[ctx->cur ^ 1];
+  a.ring = o.ring_v;
+  a.eta = o.eta;
+  a.v_stride = ctx->E;
+  a.ring_stride = 3LL * ctx->E;
+  a.eta_stride = ctx->N;
+  a.info = ctx->info;
+  a.partials = ctx->partials;
+  a.inst = ctx->inst;
+  a.quad_drag = ctx->quad_drag;
+  return a;
+}
+
+template <typename T>
+static CellArgs<T> cell_args(odis_ctx *ctx, Ops<T> &o) {
+  CellArgs<T> a;
+  a.N = ctx->N;
+  a.nb = ctx->nb_cell;
+  a.nb_face = ctx->nb_face;
+  a.div_idx = o.div_idx;
+  a.div_coef = o.div_coef;
+  a.v_new = o.v[ctx->cur ^ 1];
+  a.eta = o.eta;
+  a.ring = o.ring_e;
+  a.v_stride = ctx->E;
+  a.eta_stride = ctx->N;
+  a.ring_stride = 3LL * ctx->N;
+  a.info = ctx->info;
+  a.partials = ctx->partials;
+  a.series = ctx->series;
+  a.cap = ctx->cap;
+  a.inst = ctx->inst;
+  a.tide = ctx->tide;
+  a.dconst = ctx->dconst;
+  return a;
+}
+
+template <typename T>
+static int step_t(odis_ctx *ctx, Ops<T> &o, int n_steps) {
+  const dim3 gf(ctx->nb_face, ctx->B), gc(ctx->nb_cell, ctx->B), blk(kBlock);
+  // timing mode: three events per step on the context stream, read once at
+  // the end so the host never stalls the queue inside the timed region
+  const size_t need = ctx->timing ? size_t(3) * n_steps : 0;
+  while (ctx->ev.size() < need) {
+    hipEvent_t e;
+    CK(hipEventCreate(&e));
+    ctx->ev.push_back(e);
+  }
+  int done = 0;
+  for (int s = 0; s < n_steps; s++) {
+    if (ctx->stop_flag) {
+      ctx->stop_flag = 0;
+      return fail(ctx, ODIS_E_STOPPED, "stop requested");
+    }
+    FaceArgs<T> fa = face_args(ctx, o);
+    CellArgs<T> ca = cell_args(ctx, o);
+    if (ctx->timing) CK(hipEventRecord(ctx->ev[3 * s], ctx->stream));
+    hipLaunchKernelGGL((k_face<T, false>), gf, blk, 0, ctx->stream, fa, ctx->energy_quad);
+    if (ctx->timing) CK(hipEventRecord(ctx->ev[3 * s + 1], ctx->stream));
+    hipLaunchKernelGGL((k_cell<T>), gc, blk, 0, ctx->stream, ca);
+    if (ctx->timing) CK(hipEventRecord(ctx->ev[3 * s + 2], ctx->stream));
+    CK(hipGetLastError());
+    ctx->cur ^= 1;
+    done++;
+  }
+  if (ctx->timing && done > 0) {
+    CK(hipEventSynchronize(ctx->ev[3 * (done - 1) + 2]));
+    double tf = 0, tc = 0;
+    for (int s = 0; s < done; s++) {
+      float a = 0, b = 0;
+      CK(hipEventElapsedTime(&a, ctx->ev[3 * s], ctx->ev[3 * s + 1]));
+      CK(hipEventElapsedTime(&b, ctx->ev[3 * s + 1], ctx->ev[3 * s + 2]));
+      tf += a;
+      tc += b;
+    }
+    float all = 0;
+    CK(hipEventElapsedTime(&all, ctx->ev[0], ctx->ev[3 * (done - 1) + 2]));
+    ctx->t_face = tf / done;
+    ctx->t_cell = tc / done;
+    ctx->t_step = double(all) / done;
+  }
+  return ODIS_OK;
+}
+
+extern "C" int odis_step(odis_ctx *ctx, int n_steps, int flags) {
+  if (!ctx) return ODIS_E_INVALID;
+  if (n_steps < 0) return fail(ctx, ODIS_E_INVALID, "n_steps < 0");
+  int rc = ctx->precision == ODIS_FP64 ? step_t(ctx, ctx->d, n_steps) : step_t(ctx, ctx->f, n_steps);
+  if (rc) return rc;
+  for (auto &s : ctx->host_step) s += n_steps;
+  if (flags & ODIS_SYNC) CK(hipStreamSynchronize(ctx->stream));
+  return ODIS_OK;
+}
+
+extern "C" int odis_sync(odis_ctx *ctx) {
+  if (!ctx) return ODIS_E_INVALID;
+  CK(hipStreamSynchronize(ctx->stream));
+  return ODIS_OK;
+}
+
+template <typename T>
+static int diss_t(odis_ctx *ctx, Ops<T> &o) {
+  FaceArgs<T> fa = face_args(ctx, o);
+  const dim3 gf(ctx->nb_face, ctx->B), blk(kBlock);
+  hipLaunchKernelGGL((k_face<T, true>), gf, blk, 0, ctx->stream, fa, ctx->energy_quad);
+  hipLaunchKernelGGL(k_diss_final, dim3(ctx->B), blk, 0, ctx->stream, ctx->partials, ctx->nb_face, ctx->dconst,
+                     ctx->scratch_out);
+  CK(hipGetLastError());
+  return ODIS_OK;
+}
+
+extern "C" int odis_get_dissipation(odis_ctx *ctx, double *per_inst) {
+  if (!ctx || !per_inst) return ODIS_E_INVALID;
+  int rc = ctx->precision == ODIS_FP64 ? diss_t(ctx, ctx->d) : diss_t(ctx, ctx->f);
+  if (rc) return rc;
+  CK(hipMemcpyAsync(per_inst, ctx->scratch_out, sizeof(double) * ctx->B, hipMemcpyDeviceToHost, ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  for (int b = 0; b < ctx->B; b++)
+    if (!std::isfinite(per_inst[b])) return fail(ctx, ODIS_E_NONFINITE, "non-finite dissipation");
+  return ODIS_OK;
+}
+
+extern "C" int odis_series_capacity(const odis_ctx *ctx) { return ctx ? ctx->cap : ODIS_E_INVALID; }
+
+extern "C" int odis_get_dissipation_series(odis_ctx *ctx, int inst, long long first, int count, double *out) {
+  if (!ctx || !out || count < 0) return ODIS_E_INVALID;
+  if (inst < 0 || inst >= ctx->B) return fail(ctx, ODIS_E_RANGE, "instance out of range");
+  CK(hipStreamSynchronize(ctx->stream));
+  StepInfo si;
+  CK(hipMemcpy(&si, ctx->info + inst, sizeof(StepInfo), hipMemcpyDeviceToHost));
+  // series[k] exists for k < si.step and k >= si.step - cap
+  if (first < 0 || first + count > si.step || first < si.step - ctx->cap)
+    return fail(ctx, ODIS_E_RANGE, "series range not available");
+  std::vector<double> all(ctx->cap);
+  CK(hipMemcpy(all.data(), ctx->series + size_t(inst) * ctx->cap, sizeof(double) * ctx->cap,
+               hipMemcpyDeviceToHost));
+  for (int k = 0; k < count; k++) out[k] = all[size_t((first + k) % ctx->cap)];
+  return ODIS_OK;
+}
+
+template <typename T>
+static int vel_t(odis_ctx *ctx, Ops<T> &o, int b, const double *normal, double *uv) {
+  const int E = ctx->E;
+  std::vector<double> nd(size_t(2) * E);
+  for (int r = 0; r < E; r++) {
+    nd[2 * r] = normal[2 * ctx->face_inv[r]];
+    nd[2 * r + 1] = normal[2 * ctx->face_inv[r] + 1];
+  }
+  double *dn = nullptr, *duv = nullptr;
+  CK(hipMalloc(&dn, sizeof(double) * 2 * E));
+  hipError_t e = hipMalloc(&duv, sizeof(double) * 2 * E);
+  if (e != hipSuccess) {
+    (void)hipFree(dn);
+    return fail(ctx, ODIS_E_NOMEM, "hipMalloc");
+  }
+  std::vector<double> h(size_t(2) * E);
+  e = hipMemcpy(dn, nd.data(), sizeof(double) * 2 * E, hipMemcpyHostToDevice);
+  hipLaunchKernelGGL((k_velocity_en<T>), dim3((E + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, E,
+                     o.tri_idx, o.tri_coef, o.v[ctx->cur] + size_t(b) * E, dn, duv);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) e = hipMemcpy(h.data(), duv, sizeof(double) * 2 * E, hipMemcpyDeviceToHost);
+  (void)hipFree(dn);
+  (void)hipFree(duv);
+  if (e != hipSuccess) return fail(ctx, ODIS_E_HIP, hipGetErrorString(e));
+  for (int r = 0; r < E; r++) {
+    uv[2 * ctx->face_inv[r]] = h[2 * r];
+    uv[2 * ctx->face_inv[r] + 1] = h[2 * r + 1];
+  }
+  return ODIS_OK;
+}
+
+extern "C" int odis_get_velocity_en(odis_ctx *ctx, int inst, const double *face_normal, double *uv) {
+  if (!ctx || !face_normal || !uv) return ODIS_E_INVALID;
+  if (inst < 0 || inst >= ctx->B) return fail(ctx, ODIS_E_RANGE, "instance out of range");
+  CK(hipStreamSynchronize(ctx->stream));
+  return ctx->precision == ODIS_FP64 ? vel_t(ctx, ctx->d, inst, face_normal, uv)
+                                     : vel_t(ctx, ctx->f, inst, face_normal, uv);
+}
+
+extern "C" int odis_request_stop(odis_ctx *ctx) {
+  if (!ctx) return ODIS_E_INVALID;
+  ctx->stop_flag = 1;
+  return ODIS_OK;
+}
+
+extern "C" int odis_enable_timing(odis_ctx *ctx, int on) {
+  if (!ctx) return ODIS_E_INVALID;
+  ctx->timing = on;
+  return ODIS_OK;
+}
+
+extern "C" int odis_get_timing(odis_ctx *ctx, double *face_ms, double *cell_ms, double *step_ms) {
+  if (!ctx) return ODIS_E_INVALID;
+  if (face_ms) *face_ms = ctx->t_face;
+  if (cell_ms) *cell_ms = ctx->t_cell;
+  if (step_ms) *step_ms = ctx->t_step;
+  return ODIS_OK;
+}

@@ -1,0 +1,207 @@
+/*
+ * odis.h -- C ABI of geodesicodis_amd: the Laplace-Tidal-Equation (LTE) time
+ * step of GeodesicODIS on the icosahedral-geodesic (SCVT) C-grid, run by
+ * hand-written HIP kernels on one MI355X (gfx950).
+ *
+ * The reference has no FFI: its hot path is the pair of free functions
+ *     int solveODIS(Globals *, Mesh *)    include/solver.h:10, src/solver.cpp:15-61
+ *     int ab3Explicit(Globals *, Mesh *)  include/timeIntegrator.h:12, src/timeIntegrator.cpp:57-322
+ * called from main (src/main.cpp:64).  This header is the seam beneath them:
+ * plain pointers and sizes, no C++ types, int error codes (0 = OK, < 0 = error),
+ * never exits or throws across the boundary.  The library owns every device
+ * buffer; the caller owns host buffers, which are copied in and out and never
+ * retained.
+ *
+ * Numbering: every array crossing this ABI uses the reference's numbering of
+ * cells (grid-file node order), faces (first-visit order of AssignFaces,
+ * src/mesh.cpp:524-666) and its face orientation (face_nodes[e][0] is the
+ * "inner" node).  The device layout is renumbered internally for locality.
+ */
+#ifndef ODIS_H
+#define ODIS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ODIS_OK              0
+#define ODIS_E_INVALID      -1   /* bad argument / shape                     */
+#define ODIS_E_HIP          -2   /* HIP runtime error                        */
+#define ODIS_E_NONFINITE    -3   /* non-finite dissipation (blow-up)         */
+#define ODIS_E_UNSUPPORTED  -4   /* option with no live reference semantics  */
+#define ODIS_E_NOMEM        -5
+#define ODIS_E_RANGE        -6   /* index / size outside what was created    */
+#define ODIS_E_STOPPED      -7   /* odis_request_stop() honoured             */
+#define ODIS_E_IO           -8   /* file not found / unreadable              */
+
+/* Arithmetic of the stencil state.  ODIS_FP64 is the plain method (drop-in);
+ * ODIS_FP32 runs the stencils and state in FP32 with FP64 dissipation sums. */
+#define ODIS_FP64 0
+#define ODIS_FP32 1
+
+/* Potential / surface / friction codes: same ordinal values as the
+ * reference enums (include/globals.h:46-82). */
+enum odis_potential {
+    ODIS_OBLIQ = 0, ODIS_OBLIQ_WEST, ODIS_OBLIQ_EAST, ODIS_ECC_RAD, ODIS_ECC_LIB, ODIS_ECC,
+    ODIS_ECC_WEST, ODIS_ECC_EAST, ODIS_FULL, ODIS_FULL2, ODIS_TOTAL, ODIS_ECC_W3,
+    ODIS_OBLIQ_W3, ODIS_PLANET, ODIS_PLANET_OBL, ODIS_GENERAL, ODIS_NONE
+};
+enum odis_surface { ODIS_FREE = 0, ODIS_FREE_LOADING, ODIS_LID_LOVE, ODIS_LID_MEMBR,
+                    ODIS_LID_NUM, ODIS_LID_INF };
+enum odis_friction { ODIS_LINEAR = 0, ODIS_QUADRATIC };
+
+typedef struct odis_ctx odis_ctx;
+typedef struct odis_grid odis_grid;
+typedef struct odis_mesh odis_mesh;
+
+/* ------------------------------------------------------------------------ */
+/* Hot-path operands.  Replaces the Mesh members the step reads             */
+/* (include/mesh.h:77-221: face_nodes, face_interp_friends/weights,         */
+/* face_len, face_node_dist, face_centre_pos_sph, face_area, faces,         */
+/* node_face_dir, control_volume_surf_area_map, operatorGradient,           */
+/* operatorDivergence, operatorCoriolis, trig tables).                      */
+/* ------------------------------------------------------------------------ */
+typedef struct odis_mesh_desc {
+    int n_cells;                /* N                                                 */
+    int n_faces;                /* E = 3N - 6                                        */
+    const int    *face_nodes;   /* [E][2] inner, outer cell                          */
+    const double *grad_coef;    /* [E][2] G row (-m_in/d_e, m_out/d_e), src/mesh.cpp:3076-3080 */
+    const int    *trisk_count;  /* [E]    9 or 10                                    */
+    const int    *trisk_faces;  /* [E][10] face_interp_friends, -1 padded            */
+    const double *trisk_coef;   /* [E][10] w_ee' l_e' / d_e (src/interpolation.cpp:43-46) */
+    const double *face_lat;     /* [E]    face centre latitude (Coriolis)            */
+    const double *face_area;    /* [E]    d_e l_e (src/mesh.cpp:1093)                */
+    const int    *cell_faces;   /* [N][6] faces, -1 padded                           */
+    const double *div_coef;     /* [N][6] -dir l_e / A_i (src/mesh.cpp:3242-3247)    */
+    const double *cell_lat;     /* [N]    node latitude  (forcing fields)            */
+    const double *cell_lon;     /* [N]    node longitude                             */
+} odis_mesh_desc;
+
+/* Per-instance physics, the Globals members the step reads
+ * (include/globals.h:120-179) after applySurfaceBCs and period rounding. */
+typedef struct odis_params {
+    double g;               /* gravity at the ocean top                    */
+    double h;               /* ocean thickness                             */
+    double alpha;           /* linear drag (s^-1) or quadratic c_D         */
+    double omega;           /* rotation rate after period rounding         */
+    double radius;          /* ocean-top radius                            */
+    double love_reduct;     /* forcing prefactor                           */
+    double ecc;
+    double obliquity;       /* radians                                     */
+    double dt;              /* quantised time step (src/mesh.cpp:1566-1623)*/
+    double shell_thickness; /* added back to the forcing radius for LID_*  */
+    double semimajor;       /* PLANET forcing                              */
+    int tide;               /* enum odis_potential                         */
+    int surface;            /* enum odis_surface                           */
+    int friction;           /* enum odis_friction                          */
+    int init_load;          /* 1: AB3 from step 0 (INIT_LOAD semantics,
+                               src/temporalOperators.cpp:36)               */
+    int quad_drag;          /* 1: quadratic drag in the momentum tendency
+                               (src/drag.cpp:19-38; never live in the
+                               reference AB3 loop)                         */
+    int reserved;
+} odis_params;
+
+/* ---- device context: the time loop (src/timeIntegrator.cpp:57-322) ------ */
+
+/* Allocate device operands and n_inst instance states (all at rest, step 0).
+ * All instances share the mesh; params holds n_inst entries. */
+int odis_create(const odis_mesh_desc *mesh, const odis_params *params, int n_inst,
+                int precision, odis_ctx **out);
+int odis_destroy(odis_ctx *ctx);
+const char *odis_last_error(const odis_ctx *ctx);
+
+/* Replaces getInitialConditions / loadInitialConditions
+ * (src/initialConditions.cpp:19-144, 279-313): v[E], dvdt[E][3], eta[N],
+ * detadt[N][3] in the reference's Array2D row layout; iter = step counter.
+ * Null pointers leave that array unchanged. */
+int odis_set_state(odis_ctx *ctx, int inst, const double *v, const double *dvdt,
+                   const double *eta, const double *detadt, long long iter);
+int odis_get_state(odis_ctx *ctx, int inst, double *v, double *dvdt, double *eta,
+                   double *detadt, long long *iter);
+
+/* Advance every instance n_steps iterations of the ab3Explicit loop body
+ * (src/timeIntegrator.cpp:207-277).  Stream-ordered; returns after enqueueing
+ * unless ODIS_SYNC is set in flags. */
+#define ODIS_SYNC 1
+int odis_step(odis_ctx *ctx, int n_steps, int flags);
+int odis_sync(odis_ctx *ctx);
+
+/* FP64 dissipation of the current state of each instance (updateEnergy,
+ * src/energy.cpp:13-62; W/m^2, multiply by 4 pi R^2 for W).  Returns
+ * ODIS_E_NONFINITE if any instance is non-finite. */
+int odis_get_dissipation(odis_ctx *ctx, double *per_inst);
+/* Per-step dissipation series of instance inst: out[k] = dissipation of the
+ * state at step first + k (ring of the last odis_series_capacity() steps). */
+int odis_get_dissipation_series(odis_ctx *ctx, int inst, long long first, int count,
+                                double *out);
+int odis_series_capacity(const odis_ctx *ctx);
+/* Reconstructed (east, north) velocity per face, interpolateVelocity
+ * (src/interpolation.cpp:26-62); uv[E][2]. */
+int odis_get_velocity_en(odis_ctx *ctx, int inst, const double *face_normal, double *uv);
+/* Async-signal-safe stop request (replaces CatchExit, src/timeIntegrator.cpp:34-37):
+ * odis_step returns ODIS_E_STOPPED at the next step boundary. */
+int odis_request_stop(odis_ctx *ctx);
+
+/* Timing helper for benchmarks: average device time (ms) of the last
+ * odis_step call's kernels of the given kind (0 = face kernel, 1 = cell
+ * kernel, 2 = whole step), measured with HIP events on the context stream. */
+int odis_enable_timing(odis_ctx *ctx, int on);
+int odis_get_timing(odis_ctx *ctx, double *face_ms, double *cell_ms, double *step_ms);
+
+/* ---- host: grid and mesh (src/mesh.cpp:32-197, 436-1154, 2808-3287) ---- */
+
+/* Read a reference-format grid file (src/mesh.cpp:4016-4101). */
+int odis_grid_read(const char *path, odis_grid **out);
+/* Icosahedral bisection grid at reference level `level` (N = 10*4^(level-1)+2)
+ * with `lloyd_iters` fixed-topology Lloyd (centroidal Voronoi) sweeps. */
+int odis_grid_generate(int level, int lloyd_iters, odis_grid **out);
+int odis_grid_write(const odis_grid *g, const char *path);
+int odis_grid_size(const odis_grid *g);
+/* lat, lon [N] radians; friends [N][6] (-1 pad); clat, clon [N][6] Voronoi vertices. */
+int odis_grid_arrays(const odis_grid *g, double *lat, double *lon, int *friends,
+                     double *clat, double *clon);
+int odis_grid_from_arrays(int n, const double *lat, const double *lon, const int *friends,
+                          const double *clat, const double *clon, odis_grid **out);
+int odis_grid_free(odis_grid *g);
+
+/* Assemble the TRiSK C-grid operands on the host (Mesh::Mesh). */
+int odis_mesh_build(const odis_grid *g, double radius, odis_mesh **out);
+int odis_mesh_sizes(const odis_mesh *m, int *n_cells, int *n_faces, int *n_vertices);
+/* Fill a descriptor whose pointers stay valid while m lives. */
+int odis_mesh_describe(const odis_mesh *m, odis_mesh_desc *desc);
+/* Copy a named array (same names as the Mesh members, e.g. "face_len",
+ * "face_node_dist", "face_normal", "interp_weights", "cv_area"); returns the
+ * element count, or < 0.  dst may be null to query the count. */
+int odis_mesh_get(const odis_mesh *m, const char *name, void *dst);
+int odis_mesh_free(odis_mesh *m);
+
+/* ---- host: configuration (Globals, src/globals.cpp) ---------------------- */
+
+/* Period rounded to an even number of seconds and the matching rate,
+ * src/globals.cpp:209-214. */
+int odis_round_period(double ang_vel, double *period, double *omega);
+/* CalcMaxTimeStep, src/mesh.cpp:1601-1618: dt = P / n_steps, n_steps the
+ * smallest multiple of 100 with dt <= target. */
+int odis_quantise_dt(double period, double target_dt, double *dt, int *n_steps);
+/* Read case_dir/input.in with the reference semantics, apply applySurfaceBCs
+ * and the dt quantisation, and fill the step parameters.  Any of the
+ * optional outputs may be null. */
+int odis_config_read(const char *case_dir, odis_params *params, int *steps_per_orbit,
+                     double *end_time_orbits, int *grid_level, int *output_time);
+/* Last error of a host entry point (grid, mesh, config). */
+const char *odis_host_last_error(void);
+
+/* ---- host: drivers (solveODIS / ab3Explicit) ----------------------------- */
+
+/* Run the reference's program flow from a case directory holding input.in
+ * (src/main.cpp:46-68): Globals, Mesh, solveODIS.  Writes DATA/OUTPUT.txt with
+ * the reference's log lines and InitialConditions/{vel,pres}_init.txt.
+ * grid_dir (nullable) overrides where grid_l<g>.txt is looked up; when the
+ * file is absent the grid is generated. */
+int odis_run_case(const char *case_dir, const char *grid_dir, int precision);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ODIS_H */

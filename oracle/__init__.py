@@ -1,0 +1,275 @@
+"""ctypes binding of the CPU FP64 oracle (test infrastructure only).
+
+Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+bench.py may import this module.  The product package never does.
+
+The C source restates the reference's mesh assembly and AB3 step
+(src/mesh.cpp, src/timeIntegrator.cpp, src/updateMomentum.cpp,
+src/updateEta.cpp, src/temporalOperators.cpp, src/tidalPotentials.cpp,
+src/interpolation.cpp:26-62, src/energy.cpp:13-62).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+OBLIQ, OBLIQ_WEST, OBLIQ_EAST, ECC_RAD, ECC_LIB, ECC, ECC_WEST, ECC_EAST, FULL, FULL2, \
+    TOTAL, ECC_W3, OBLIQ_W3, PLANET, PLANET_OBL, GENERAL, NONE = range(17)
+FREE, FREE_LOADING, LID_LOVE, LID_MEMBR, LID_NUM, LID_INF = range(6)
+LINEAR, QUADRATIC = range(2)
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in
+                ("g", "h", "alpha", "omega", "radius", "love_reduct", "ecc", "obliquity",
+                 "dt", "shell_thickness", "semimajor")] + \
+               [(n, ctypes.c_int) for n in
+                ("tide", "surface", "friction", "init_load", "quad_drag_momentum", "pad_")]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        vp, ip, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
+        L.orc_mesh_build.restype = vp
+        L.orc_mesh_build.argtypes = [ctypes.c_int, dp, dp, ip, dp, dp, ctypes.c_double, ip]
+        L.orc_mesh_free.argtypes = [vp]
+        L.orc_mesh_sizes.argtypes = [vp, ip, ip, ip]
+        L.orc_mesh_get.argtypes = [vp, ctypes.c_char_p, vp]
+        L.orc_mesh_get.restype = ctypes.c_int
+        for nm in ("orc_mesh_grad_csr", "orc_mesh_div_csr"):
+            getattr(L, nm).argtypes = [vp, ip, ip, dp]
+        L.orc_mesh_coriolis_nnz.argtypes = [vp]
+        L.orc_mesh_coriolis_csr.argtypes = [vp, ctypes.c_double, ip, ip, dp]
+        L.orc_sim_create.restype = vp
+        L.orc_sim_create.argtypes = [vp, ctypes.POINTER(OrcParams)]
+        L.orc_sim_free.argtypes = [vp]
+        L.orc_sim_set_state.argtypes = [vp, dp, dp, dp, dp, ctypes.c_int]
+        L.orc_sim_get_state.argtypes = [vp, dp, dp, dp, dp, ip]
+        L.orc_sim_step.argtypes = [vp, ctypes.c_int, dp]
+        L.orc_sim_dissipation.argtypes = [vp]
+        L.orc_sim_dissipation.restype = ctypes.c_double
+        L.orc_sim_forcing.argtypes = [vp, ctypes.c_double, dp]
+        L.orc_sim_interp_velocity.argtypes = [vp, dp, dp]
+        L.orc_sim_momentum.argtypes = [vp, dp, dp, dp]
+        L.orc_round_period.argtypes = [ctypes.c_double, dp, dp]
+        L.orc_quantise_dt.argtypes = [ctypes.c_double, ctypes.c_double, dp, ip]
+        L.orc_analytical_obliq_west.argtypes = [ctypes.POINTER(OrcParams), ctypes.c_double,
+                                                ctypes.c_double, ctypes.c_double, dp]
+        L.orc_membrane_nu_beta.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_int, dp, dp, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if a is not None else None
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+_INT_ARRAYS = {"faces", "node_face_dir", "vertexes", "vertex_nodes", "face_nodes",
+               "interp_count", "interp_friends"}
+
+
+class Mesh:
+    """Oracle mesh built from a grid (node lat/lon in radians, 6 friends,
+    6 Voronoi vertices per node), src/mesh.cpp:32-197."""
+
+    def __init__(self, grid, radius):
+        L = lib()
+        self.lat = np.ascontiguousarray(grid["lat"], dtype=np.float64)
+        self.lon = np.ascontiguousarray(grid["lon"], dtype=np.float64)
+        fr = np.ascontiguousarray(grid["friends"], dtype=np.int32)
+        clat = np.ascontiguousarray(grid["clat"], dtype=np.float64)
+        clon = np.ascontiguousarray(grid["clon"], dtype=np.float64)
+        err = ctypes.c_int(0)
+        self.h = L.orc_mesh_build(len(self.lat), _dp(self.lat), _dp(self.lon), _ip(fr),
+                                  _dp(clat), _dp(clon), float(radius), ctypes.byref(err))
+        if not self.h:
+            raise RuntimeError("oracle mesh build failed: %d" % err.value)
+        n, e, v = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        L.orc_mesh_sizes(self.h, ctypes.byref(n), ctypes.byref(e), ctypes.byref(v))
+        self.N, self.E, self.V = n.value, e.value, v.value
+        self.radius = float(radius)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_mesh_free(self.h)
+            self.h = None
+
+    def get(self, name):
+        L = lib()
+        cnt = L.orc_mesh_get(self.h, name.encode(), None)
+        if cnt < 0:
+            raise KeyError(name)
+        a = np.zeros(cnt, dtype=np.int32 if name in _INT_ARRAYS else np.float64)
+        L.orc_mesh_get(self.h, name.encode(), a.ctypes.data_as(ctypes.c_void_p))
+        return a
+
+    def grad_csr(self):
+        rp = np.zeros(self.E + 1, np.int32)
+        col = np.zeros(2 * self.E, np.int32)
+        val = np.zeros(2 * self.E)
+        lib().orc_mesh_grad_csr(self.h, _ip(rp), _ip(col), _dp(val))
+        return rp, col, val
+
+    def div_csr(self):
+        rp = np.zeros(self.N + 1, np.int32)
+        col = np.zeros(6 * self.N, np.int32)
+        val = np.zeros(6 * self.N)
+        nnz = lib().orc_mesh_div_csr(self.h, _ip(rp), _ip(col), _dp(val))
+        return rp, col[:nnz], val[:nnz]
+
+    def coriolis_csr(self, omega):
+        nnz = lib().orc_mesh_coriolis_nnz(self.h)
+        rp = np.zeros(self.E + 1, np.int32)
+        col = np.zeros(nnz, np.int32)
+        val = np.zeros(nnz)
+        lib().orc_mesh_coriolis_csr(self.h, float(omega), _ip(rp), _ip(col), _dp(val))
+        return rp, col, val
+
+
+def make_params(**kw):
+    p = OrcParams()
+    defaults = dict(g=0.113, h=38e3, alpha=1e-8, omega=5.307e-5, radius=252.1e3,
+                    love_reduct=1.0, ecc=0.0047, obliquity=0.0, dt=30.0,
+                    shell_thickness=0.0, semimajor=0.0, tide=ECC, surface=FREE,
+                    friction=LINEAR, init_load=0, quad_drag_momentum=0)
+    defaults.update(kw)
+    for k, v in defaults.items():
+        setattr(p, k, v)
+    return p
+
+
+class Sim:
+    """Oracle AB3 stepper, src/timeIntegrator.cpp:57-322."""
+
+    def __init__(self, mesh, params):
+        self.mesh = mesh
+        self.params = params
+        self.h = lib().orc_sim_create(mesh.h, ctypes.byref(params))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_sim_free(self.h)
+            self.h = None
+
+    def set_state(self, v=None, dvdt=None, eta=None, detadt=None, iter=0):
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+                for a in (v, dvdt, eta, detadt)]
+        lib().orc_sim_set_state(self.h, *[_dp(a) for a in arrs], int(iter))
+
+    def get_state(self):
+        m = self.mesh
+        v, dv = np.zeros(m.E), np.zeros(m.E * 3)
+        eta, de = np.zeros(m.N), np.zeros(m.N * 3)
+        it = ctypes.c_int()
+        lib().orc_sim_get_state(self.h, _dp(v), _dp(dv), _dp(eta), _dp(de), ctypes.byref(it))
+        return dict(v=v, dvdt=dv.reshape(m.E, 3), eta=eta, detadt=de.reshape(m.N, 3),
+                    iter=it.value)
+
+    def step(self, n, record=False):
+        d = np.zeros(n) if record else None
+        lib().orc_sim_step(self.h, int(n), _dp(d))
+        return d
+
+    def dissipation(self):
+        return lib().orc_sim_dissipation(self.h)
+
+    def forcing(self, t):
+        U = np.zeros(self.mesh.N)
+        lib().orc_sim_forcing(self.h, float(t), _dp(U))
+        return U
+
+    def interp_velocity(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        uv = np.zeros(self.mesh.E * 2)
+        lib().orc_sim_interp_velocity(self.h, _dp(v), _dp(uv))
+        return uv.reshape(-1, 2)
+
+    def momentum(self, v, eta):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        eta = np.ascontiguousarray(eta, dtype=np.float64)
+        out = np.zeros(self.mesh.E)
+        lib().orc_sim_momentum(self.h, _dp(v), _dp(eta), _dp(out))
+        return out
+
+
+def round_period(ang_vel):
+    p, o = ctypes.c_double(), ctypes.c_double()
+    lib().orc_round_period(float(ang_vel), ctypes.byref(p), ctypes.byref(o))
+    return p.value, o.value
+
+
+def quantise_dt(period, target):
+    d, n = ctypes.c_double(), ctypes.c_int()
+    lib().orc_quantise_dt(float(period), float(target), ctypes.byref(d), ctypes.byref(n))
+    return d.value, n.value
+
+
+def analytical_obliq_west(params, lat, lon, t):
+    o = np.zeros(6)
+    lib().orc_analytical_obliq_west(ctypes.byref(params), float(lat), float(lon), float(t), _dp(o))
+    return o
+
+
+def membrane_nu_beta(radius, hs, ho, l_max):
+    nu, beta = np.zeros(l_max + 1), np.zeros(l_max + 1)
+    ro, go = ctypes.c_double(), ctypes.c_double()
+    lib().orc_membrane_nu_beta(float(radius), float(hs), float(ho), int(l_max), _dp(nu),
+                               _dp(beta), ctypes.byref(ro), ctypes.byref(go))
+    return nu, beta, ro.value, go.value
+
+
+def read_reference_grid(path):
+    """Parse a reference-format grid file (src/mesh.cpp:4016-4101) with the same
+    deg->rad conversion (atof(str) * pi/180)."""
+    rad = 3.1415926535897932384626433832795028841971693993751058 / 180.0
+    rows = {}
+    with open(path) as fh:
+        fh.readline()
+        for line in fh:
+            if not line.strip():
+                continue
+            head, rest = line.split("{", 1)
+            parts = head.split()
+            nid = int(parts[0])
+            fr_s, rest = rest.split("}", 1)
+            fr = [int(x) for x in fr_s.split(",")]
+            cen = rest.split("{", 1)[1].split("}", 1)[0]
+            pts = [p.strip().strip("(").strip() for p in cen.split(")") if p.strip(" ,")]
+            cl = []
+            for p in pts:
+                p = p.lstrip(",").strip().lstrip("(")
+                a, b = p.split(",")
+                cl.append((float(a) * rad, float(b) * rad))
+            rows[nid] = (float(parts[1]) * rad, float(parts[2]) * rad, fr, cl)
+    n = len(rows)
+    lat = np.zeros(n)
+    lon = np.zeros(n)
+    friends = np.zeros((n, 6), np.int32)
+    clat = np.zeros((n, 6))
+    clon = np.zeros((n, 6))
+    for i, (a, b, fr, cl) in rows.items():
+        lat[i], lon[i] = a, b
+        friends[i] = fr
+        for j, (x, y) in enumerate(cl):
+            clat[i, j], clon[i, j] = x, y
+    return dict(lat=lat, lon=lon, friends=friends, clat=clat, clon=clon)

@@ -1,0 +1,229 @@
+"""GPU (HIP, gfx950) LTE step vs the CPU FP64 oracle, through the C ABI.
+
+Tolerances (FP64 plain method): relative L2 <= 1e-12 after a few hundred
+steps, <= 1e-10 after one full orbit (the bar of the plain method).  The two
+paths share the same operators (the product's mesh assembly is checked
+bit-for-bit against the oracle's in test_host_mesh.py); they differ only in
+summation order, in the factoring of the Coriolis coefficient and of the
+forcing gradient, so the differences are rounding-level.
+"""
+import numpy as np
+import pytest
+
+from conftest import generated_grid, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL_SHORT = 1e-12
+TOL_ORBIT = 1e-10
+
+
+def _pair(product, oracle, level, params_kw, precision=0):
+    G, O = product, oracle
+    g, arrs = generated_grid(level)
+    pkw = dict(params_kw)
+    p, nsteps = G.make_params(**pkw)
+    mesh = G.Mesh(g, p.radius)
+    ctx = G.Context(mesh, p, precision=precision)
+    om = O.Mesh(arrs, p.radius)
+    op = O.make_params(**{k: getattr(p, k) for k in
+                          ("g", "h", "alpha", "omega", "radius", "love_reduct", "ecc", "obliquity",
+                           "dt", "shell_thickness", "semimajor", "tide", "surface", "friction",
+                           "init_load")}, quad_drag_momentum=p.quad_drag)
+    sim = O.Sim(om, op)
+    return ctx, sim, mesh, p, nsteps
+
+
+def _compare(ctx, sim, tol, inst=0):
+    a = ctx.get_state(inst)
+    b = sim.get_state()
+    assert a["iter"] == b["iter"]
+    errs = {k: rel_l2(a[k], b[k]) for k in ("v", "eta", "dvdt", "detadt")}
+    for k, e in errs.items():
+        assert e <= tol, (k, e, errs)
+    return errs
+
+
+def test_ecc_rest_start_short(product, oracle):
+    ctx, sim, mesh, p, _ = _pair(product, oracle, 4, dict(tide=product.ECC))
+    for n in (1, 1, 1, 5, 50):
+        ctx.step(n)
+        sim.step(n)
+        _compare(ctx, sim, TOL_SHORT)
+    d_gpu = ctx.dissipation()[0]
+    d_orc = sim.dissipation()
+    assert abs(d_gpu - d_orc) <= 1e-12 * abs(d_orc)
+
+
+def test_ecc_one_orbit_level5(product, oracle):
+    """Config-1 physics (Enceladus, h=38 km, e=0.0047, alpha=1e-8) at 2562
+    cells for one full orbit (4000 steps of dt = 29.5985 s)."""
+    ctx, sim, mesh, p, nsteps = _pair(product, oracle, 5, dict(tide=product.ECC))
+    assert nsteps == 4000 and abs(p.dt - 118394.0 / 4000) < 1e-12
+    ctx.step(nsteps)
+    d_orc = sim.step(nsteps, record=True)
+    errs = _compare(ctx, sim, TOL_ORBIT)
+    # per-step dissipation series: series[k] is the state at step k
+    series = ctx.dissipation_series(0, 1, nsteps - 1)
+    assert rel_l2(series, d_orc[:-1]) <= TOL_ORBIT
+    assert abs(ctx.dissipation()[0] - d_orc[-1]) <= TOL_ORBIT * abs(d_orc[-1])
+    print("one-orbit errors", errs)
+
+
+@pytest.mark.parametrize("tide", ["OBLIQ", "OBLIQ_WEST", "FULL", "FULL2", "PLANET", "NONE", "ECC_RAD"])
+def test_forcing_types(product, oracle, tide):
+    G = product
+    kw = dict(tide=getattr(G, tide), obliquity=0.01, ecc=0.0047)
+    if tide == "PLANET":
+        kw.update(semimajor=671100000.0)
+    ctx, sim, mesh, p, _ = _pair(product, oracle, 4, kw)
+    ctx.step(120)
+    sim.step(120)
+    if tide in ("NONE", "ECC_RAD"):
+        # no live forcing: a rest start stays exactly at rest
+        s = ctx.get_state()
+        assert np.all(s["v"] == 0) and np.all(s["eta"] == 0)
+        return
+    _compare(ctx, sim, TOL_SHORT)
+
+
+def test_obliq_west_analytical_start(product, oracle):
+    """INIT_ANALYTICAL start (src/initialConditions.cpp:146-206) with the
+    OBLIQ_WEST closed form; the Euler start discards the analytic history."""
+    G, O = product, oracle
+    kw = dict(tide=G.OBLIQ_WEST, obliquity=0.1 * np.pi / 180, alpha=1e-7)
+    ctx, sim, mesh, p, _ = _pair(product, oracle, 4, kw)
+    op = sim.params
+    flat, flon = mesh.get("face_centre_lat"), mesh.get("face_centre_lon")
+    nrm = mesh.get("face_normal").reshape(-1, 2)
+    clat, clon = mesh.get("cell_lat"), mesh.get("cell_lon")
+    v = np.zeros(mesh.E)
+    dv = np.zeros((mesh.E, 3))
+    eta = np.zeros(mesh.N)
+    de = np.zeros((mesh.N, 3))
+    for e in range(mesh.E):
+        for k, t in enumerate((0.0, -p.dt, -2 * p.dt)):
+            o = O.analytical_obliq_west(op, flat[e], flon[e], t)
+            if k == 0:
+                v[e] = o[0] * nrm[e, 0] + o[1] * nrm[e, 1]
+            dv[e, k] = o[2] * nrm[e, 0] + o[3] * nrm[e, 1]
+    for i in range(mesh.N):
+        for k, t in enumerate((0.0, -p.dt, -2 * p.dt)):
+            o = O.analytical_obliq_west(op, clat[i], clon[i], t)
+            if k == 0:
+                eta[i] = o[4]
+            de[i, k] = o[5]
+    ctx.set_state(0, v, dv, eta, de, 0)
+    sim.set_state(v, dv, eta, de, 0)
+    s0 = ctx.get_state()
+    assert np.array_equal(s0["v"], v) and np.array_equal(s0["dvdt"], dv)
+    for n in (1, 1, 1, 40):
+        ctx.step(n)
+        sim.step(n)
+        _compare(ctx, sim, TOL_SHORT)
+
+
+def test_history_layout_and_load_resume(product, oracle):
+    """Reference Array2D history layout after each of the first steps, and an
+    INIT_LOAD resume (AB3 from step 0 with the written history) matching an
+    uninterrupted run."""
+    G, O = product, oracle
+    ctx, sim, mesh, p, _ = _pair(product, oracle, 4, dict(tide=G.ECC))
+    for _ in range(4):
+        ctx.step(1)
+        sim.step(1)
+        _compare(ctx, sim, TOL_SHORT)
+    ctx.step(30)
+    sim.step(30)
+    ck = ctx.get_state()
+    # resume in a fresh context with INIT_LOAD semantics: time restarts at 0,
+    # so compare against an oracle resumed from the same checkpoint
+    p2 = G.Params(**p.to_dict())
+    p2.init_load = 1
+    ctx2 = G.Context(mesh, p2)
+    ctx2.set_state(0, ck["v"], ck["dvdt"], ck["eta"], ck["detadt"], 0)
+    g, arrs = generated_grid(4)
+    op = sim.params
+    op.init_load = 1
+    sim2 = O.Sim(O.Mesh(arrs, p.radius), op)
+    sim2.set_state(ck["v"], ck["dvdt"], ck["eta"], ck["detadt"], 0)
+    ctx2.step(25)
+    sim2.step(25)
+    _compare(ctx2, sim2, TOL_SHORT)
+
+
+def test_quadratic_drag_and_energy(product, oracle):
+    G = product
+    kw = dict(tide=G.ECC, friction=G.QUADRATIC, alpha=2e-3, quad_drag=1, ecc=0.05)
+    ctx, sim, mesh, p, _ = _pair(product, oracle, 4, kw)
+    ctx.step(200)
+    d = sim.step(200, record=True)
+    _compare(ctx, sim, TOL_SHORT)
+    assert abs(ctx.dissipation()[0] - d[-1]) <= 1e-12 * abs(d[-1])
+
+
+def test_batch_instances_match_single_runs(product, oracle):
+    """Several instances with different thickness and drag in one context."""
+    G, O = product, oracle
+    g, arrs = generated_grid(4)
+    ps = []
+    for h, a in ((38e3, 1e-8), (5e3, 1e-7), (90e3, 1e-10)):
+        p, _ = G.make_params(h=h, alpha=a)
+        ps.append(p)
+    mesh = G.Mesh(g, ps[0].radius)
+    ctx = G.Context(mesh, ps)
+    ctx.step(150)
+    om = O.Mesh(arrs, ps[0].radius)
+    for b, p in enumerate(ps):
+        sim = O.Sim(om, O.make_params(g=p.g, h=p.h, alpha=p.alpha, omega=p.omega, radius=p.radius,
+                                      ecc=p.ecc, dt=p.dt, tide=p.tide))
+        sim.step(150)
+        _compare(ctx, sim, TOL_SHORT, inst=b)
+
+
+def test_fp32_stencils_error_vs_fp64(product, oracle):
+    """FP32 state and stencils with FP64 dissipation sums: error vs the FP64
+    oracle after 400 steps is reported and bounded."""
+    ctx, sim, mesh, p, _ = _pair(product, oracle, 5, dict(tide=product.ECC), precision=product.FP32)
+    ctx.step(400)
+    d = sim.step(400, record=True)
+    a, b = ctx.get_state(), sim.get_state()
+    ev, ee = rel_l2(a["v"], b["v"]), rel_l2(a["eta"], b["eta"])
+    ed = abs(ctx.dissipation()[0] - d[-1]) / abs(d[-1])
+    print("fp32 rel errors: v %.2e eta %.2e diss %.2e" % (ev, ee, ed))
+    assert ev < 1e-4 and ee < 1e-4 and ed < 1e-4
+
+
+def test_reference_format_grid_roundtrip(product, oracle, tmp_path):
+    """A grid written in the reference text format and read back gives the
+    same GPU trajectory as the oracle on the re-read arrays."""
+    G, O = product, oracle
+    g, _ = generated_grid(3)
+    path = tmp_path / "grid_l3.txt"
+    g.write(path)
+    g2 = G.Grid.read(path)
+    arrs = O.read_reference_grid(str(path))
+    a2 = g2.arrays()
+    for k in ("lat", "lon", "clat", "clon"):
+        assert np.array_equal(arrs[k], a2[k])
+    p, _ = G.make_params()
+    ctx = G.Context(G.Mesh(g2, p.radius), p)
+    sim = O.Sim(O.Mesh(arrs, p.radius), O.make_params(g=p.g, h=p.h, alpha=p.alpha, omega=p.omega,
+                                                      radius=p.radius, ecc=p.ecc, dt=p.dt))
+    ctx.step(60)
+    sim.step(60)
+    _compare(ctx, sim, TOL_SHORT)
+
+
+def test_stop_request(product):
+    G = product
+    g, _ = generated_grid(3)
+    p, _ = G.make_params()
+    ctx = G.Context(G.Mesh(g, p.radius), p)
+    ctx.request_stop()
+    with pytest.raises(G.OdisError) as ei:
+        ctx.step(10)
+    assert ei.value.code == -7
+    assert ctx.get_state()["iter"] == 0
+    ctx.step(3)
+    assert ctx.get_state()["iter"] == 3
