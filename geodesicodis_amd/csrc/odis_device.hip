@@ -36,14 +36,13 @@ __device__ __forceinline__ double block_sum(double x, double *sh) {
 // Face kernel.  DISS_ONLY: evaluate the dissipation partials of the current
 // state without stepping (output cadence / final state).
 template <typename T, bool DISS_ONLY>
-__global__ __launch_bounds__(kBlock) void k_face(FaceArgs<T> a, int energy_quad) {
+__global__ __launch_bounds__(kBlock) void k_face(FaceArgs<T> a, int energy_quad, long long n,
+                                                 const double *coef) {
   __shared__ double sh[4];
   const int b = blockIdx.y;
   const int tile = xcd_tile(blockIdx.x, a.nb);
   const int E = a.E;
   const int e = tile * kBlock + threadIdx.x;
-  const StepInfo *inf = a.info + b;
-  const long long n = inf->step;
   const double *ip = a.inst + b * 8;
   double local = 0.0;
   if (e < E) {
@@ -63,7 +62,7 @@ __global__ __launch_bounds__(kBlock) void k_face(FaceArgs<T> a, int energy_quad)
       T f0 = -g * ge + a.fcor[e] * vt;
       if (a.quad_drag) f0 -= qd * sqrt(sp) * ve;
       T fu = T(0);
-      for (int k = 0; k < a.nfields; ++k) fu += T(inf->coef[k]) * a.fgs[k * E + e];
+      for (int k = 0; k < a.nfields; ++k) fu += T(coef[b * 8 + k]) * a.fgs[k * E + e];
       const T drag = -alpha * ve + fu;
       T *ring = a.ring + b * a.ring_stride;
       const int s0 = int(n % 3), s1 = int((n + 2) % 3), s2 = int((n + 1) % 3);
@@ -92,15 +91,12 @@ __device__ __forceinline__ double sum_partials(const double *p, int count, doubl
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cell(CellArgs<T> a) {
+__global__ __launch_bounds__(kBlock) void k_cell(CellArgs<T> a, long long n) {
   __shared__ double sh[4];
-  __shared__ unsigned int s_ticket;
   const int b = blockIdx.y;
   const int tile = xcd_tile(blockIdx.x, a.nb);
   const int N = a.N;
   const int i = tile * kBlock + threadIdx.x;
-  StepInfo *inf = a.info + b;
-  const long long n = inf->step;
   const double *ip = a.inst + b * 8;
   if (i < N) {
     const T *v = a.v_new + b * a.v_stride;
@@ -121,37 +117,15 @@ __global__ __launch_bounds__(kBlock) void k_cell(CellArgs<T> a) {
     T *eta = a.eta + b * a.eta_stride;
     eta[i] = eta[i] + upd;
   }
-  // last workgroup of this instance finalises the step
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) s_ticket = atomicAdd(&inf->ticket, 1u);
-  __syncthreads();
-  if (s_ticket != unsigned(a.nb - 1)) return;
-  __threadfence();
+  // the first-dispatched workgroup of each instance reduces the face
+  // kernel's dissipation partials of this step (complete: previous kernel)
+  if (blockIdx.x != 0) return;
   const double tot = sum_partials(a.partials + size_t(b) * a.nb_face, a.nb_face, sh);
   if (threadIdx.x == 0) {
     const double d = tot * a.dconst[b];
     a.series[size_t(b) * a.cap + size_t(n % a.cap)] = d;
-    inf->last_diss = d;
-    if (!isfinite(d)) inf->nonfinite = 1;
-    double c[kMaxForcing] = {0, 0, 0, 0, 0, 0, 0};
-    const double dt = ip[2];
-    tide_coeffs(a.tide[b], dt * double(n + 1) + dt, c);
-    for (int k = 0; k < kMaxForcing; ++k) inf->coef[k] = c[k];
-    inf->ticket = 0u;
-    inf->step = n + 1;
+    if (!isfinite(d)) a.nonfinite[b] = 1;
   }
-}
-
-// Forcing coefficients of the pending step (after create / set_state).
-__global__ void k_prologue(StepInfo *info, const TideDev *tide, const double *inst, int B) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B) return;
-  double c[kMaxForcing] = {0, 0, 0, 0, 0, 0, 0};
-  const double dt = inst[b * 8 + 2];
-  tide_coeffs(tide[b], dt * double(info[b].step) + dt, c);
-  for (int k = 0; k < kMaxForcing; ++k) info[b].coef[k] = c[k];
-  info[b].ticket = 0u;
 }
 
 // Sum dissipation partials produced by k_face<DISS_ONLY>.
@@ -189,10 +163,6 @@ namespace {
 
 thread_local std::string g_last_error;
 
-struct DevBuf {
-  void *p = nullptr;
-};
-
 template <typename T>
 struct Ops {
   int *tri_idx = nullptr;
@@ -207,6 +177,7 @@ struct Ops {
 };
 
 uint64_t hilbert3(uint32_t X0, uint32_t X1, uint32_t X2, int bits) {
+  // Skilling's transpose form of the Hilbert index, bits per axis
   uint32_t X[3] = {X0, X1, X2};
   uint32_t M = 1u << (bits - 1), P, Q, t;
   for (Q = M; Q > 1; Q >>= 1) {
@@ -244,17 +215,24 @@ struct odis_ctx {
   Ops<double> d;
   Ops<float> f;
   int cur = 0;
-  StepInfo *info = nullptr;
-  double *partials = nullptr;
-  double *series = nullptr;
+  long long step = 0;          // next step index n, shared by all instances
+  double *partials = nullptr;  // [B][nb_face]
+  double *series = nullptr;    // [B][cap]
+  int *nonfinite = nullptr;    // [B]
   int cap = 1 << 16;
-  double *inst = nullptr;
-  TideDev *tide = nullptr;
-  double *dconst = nullptr;
+  double *inst = nullptr;      // [B][8]
+  double *dconst = nullptr;    // [B]
   double *scratch_out = nullptr;
+  // forcing coefficient tables: 2 device + 2 pinned host buffers of
+  // chunk x B x 8 doubles, alternated per chunk
+  int chunk = 0;
+  double *coef_dev[2] = {nullptr, nullptr};
+  double *coef_host[2] = {nullptr, nullptr};
+  hipEvent_t coef_ev[2] = {nullptr, nullptr};
+  int coef_next = 0;
+  std::vector<TideDev> tides;
   std::vector<odis_params> params;
   std::vector<std::vector<double>> init_hist_v, init_hist_e;  // history as last set ([M][3])
-  std::vector<long long> host_step;
   std::vector<void *> allocs;
   volatile int stop_flag = 0;
   int timing = 0;
@@ -300,8 +278,9 @@ static std::vector<T> cast_vec(const std::vector<double> &x) {
   return std::vector<T>(x.begin(), x.end());
 }
 
-// Spatial fields S_k of each potential, evaluated from the reference's trig
-// tables (src/mesh.cpp:2122-2152); the time coefficients are tide_coeffs.
+// Spatial fields S_k of each potential, from the reference's trig tables
+// (src/mesh.cpp:2122-2152); U(x, t) = sum_k c_k(t) S_k(x) with c_k from
+// tide_coeffs.  Regrouping of src/tidalPotentials.cpp:82-225.
 static int tide_fields(int tide, double lat, double lon, double *S) {
   const double cl = std::cos(lat), sl = std::sin(lat), co = std::cos(lon), so = std::sin(lon);
   const double c2l = std::cos(2.0 * lat), s2l = std::sin(2.0 * lat);
@@ -395,16 +374,22 @@ static int build_ops(odis_ctx *ctx, Ops<T> &o, const std::vector<double> &tri_c,
   return ODIS_OK;
 }
 
-static int launch_prologue(odis_ctx *ctx) {
-  hipLaunchKernelGGL(k_prologue, dim3((ctx->B + 63) / 64), dim3(64), 0, ctx->stream, ctx->info, ctx->tide,
-                     ctx->inst, ctx->B);
-  CK(hipGetLastError());
+extern "C" int odis_destroy(odis_ctx *ctx) {
+  if (!ctx) return ODIS_E_INVALID;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto &e : ctx->ev) (void)hipEventDestroy(e);
+  for (int k = 0; k < 2; k++) {
+    if (ctx->coef_ev[k]) (void)hipEventDestroy(ctx->coef_ev[k]);
+    if (ctx->coef_host[k]) (void)hipHostFree(ctx->coef_host[k]);
+  }
+  for (void *p : ctx->allocs) (void)hipFree(p);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
   return ODIS_OK;
 }
 
 extern "C" int odis_create(const odis_mesh_desc *m, const odis_params *params, int n_inst, int precision,
                            odis_ctx **out) {
-  odis_ctx *ctx = nullptr;
   if (!m || !params || !out || n_inst < 1) return fail(nullptr, ODIS_E_INVALID, "null argument or n_inst < 1");
   if (precision != ODIS_FP64 && precision != ODIS_FP32) return fail(nullptr, ODIS_E_INVALID, "bad precision");
   const int N = m->n_cells, E = m->n_faces;
@@ -418,13 +403,14 @@ extern "C" int odis_create(const odis_mesh_desc *m, const odis_params *params, i
       return fail(nullptr, ODIS_E_UNSUPPORTED,
                   "GENERAL forcing needs the Legendre tables of CalcLegendreFuncs, which the reference leaves "
                   "commented out");
+    if (p.tide < 0 || p.tide > ODIS_NONE) return fail(nullptr, ODIS_E_INVALID, "unknown potential");
     if (p.tide != params[0].tide || p.omega != params[0].omega || p.friction != params[0].friction ||
         p.quad_drag != params[0].quad_drag)
       return fail(nullptr, ODIS_E_UNSUPPORTED, "instances of one context must share tide type, omega and drag law");
     if (!(p.dt > 0) || !(p.h > 0) || !(p.radius > 0))
       return fail(nullptr, ODIS_E_INVALID, "dt, h and radius must be positive");
   }
-  // validate index ranges on the host before anything reaches a kernel
+  // validate every index on the host before anything reaches a kernel
   for (int e = 0; e < E; e++) {
     for (int k = 0; k < 2; k++)
       if (m->face_nodes[2 * e + k] < 0 || m->face_nodes[2 * e + k] >= N)
@@ -441,7 +427,7 @@ extern "C" int odis_create(const odis_mesh_desc *m, const odis_params *params, i
       if (f >= E || (f < 0 && j < 5)) return fail(nullptr, ODIS_E_INVALID, "cell_faces out of range");
     }
 
-  ctx = new odis_ctx();
+  odis_ctx *ctx = new odis_ctx();
   ctx->N = N;
   ctx->E = E;
   ctx->B = n_inst;
@@ -451,10 +437,9 @@ extern "C" int odis_create(const odis_mesh_desc *m, const odis_params *params, i
   ctx->quad_drag = params[0].quad_drag != 0;
   ctx->init_hist_v.assign(n_inst, std::vector<double>(size_t(3) * E, 0.0));
   ctx->init_hist_e.assign(n_inst, std::vector<double>(size_t(3) * N, 0.0));
-  ctx->host_step.assign(n_inst, 0);
 
   // -- locality renumbering: cells along a 3-D Hilbert curve, faces by the
-  //    ranks of their two cells
+  //    device ranks of their two cells
   std::vector<std::pair<uint64_t, int>> ck(N);
   for (int i = 0; i < N; i++) {
     const double la = m->cell_lat[i], lo = m->cell_lon[i];
@@ -538,9 +523,7 @@ extern "C" int odis_create(const odis_mesh_desc *m, const odis_params *params, i
 
   int rc = ODIS_OK;
   hipError_t he = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-  if (he != hipSuccess) {
-    rc = fail(ctx, ODIS_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
-  }
+  if (he != hipSuccess) rc = fail(ctx, ODIS_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
   if (!rc) {
     if (precision == ODIS_FP64)
       rc = build_ops(ctx, ctx->d, tri_c, tri_i, gc, gi, fcor, farea, fgs, dc, di);
@@ -551,8 +534,8 @@ extern "C" int odis_create(const odis_mesh_desc *m, const odis_params *params, i
   ctx->nb_cell = (N + kBlock - 1) / kBlock;
   // per-instance parameters
   std::vector<double> inst(size_t(8) * n_inst, 0.0), dconst(n_inst);
-  std::vector<TideDev> tides(n_inst);
-  for (int b = 0; b < n_inst && !rc; b++) {
+  ctx->tides.resize(n_inst);
+  for (int b = 0; b < n_inst; b++) {
     const odis_params &p = params[b];
     double *q = &inst[size_t(8) * b];
     q[0] = p.g;
@@ -563,42 +546,40 @@ extern "C" int odis_create(const odis_mesh_desc *m, const odis_params *params, i
     q[5] = p.h;
     const double area = 4 * kPi * std::pow(p.radius, 2.0);
     dconst[b] = (p.friction == ODIS_QUADRATIC ? p.alpha / p.h : p.alpha * 1000.0 * p.h) / area;
-    tides[b] = make_tide(p);
-    tides[b].nfields = K;
+    ctx->tides[b] = make_tide(p);
+    ctx->tides[b].nfields = K;
   }
   if (!rc) rc = upload(ctx, &ctx->inst, inst);
   if (!rc) rc = upload(ctx, &ctx->dconst, dconst);
-  if (!rc) rc = upload(ctx, &ctx->tide, tides);
-  if (!rc) rc = dev_alloc(ctx, &ctx->info, size_t(n_inst));
   if (!rc) rc = dev_alloc(ctx, &ctx->partials, size_t(n_inst) * ctx->nb_face);
   if (!rc) rc = dev_alloc(ctx, &ctx->series, size_t(n_inst) * ctx->cap);
+  if (!rc) rc = dev_alloc(ctx, &ctx->nonfinite, size_t(n_inst));
   if (!rc) rc = dev_alloc(ctx, &ctx->scratch_out, size_t(n_inst));
-  if (!rc) {
-    he = hipMemset(ctx->info, 0, sizeof(StepInfo) * n_inst);
-    if (he == hipSuccess) he = hipMemset(ctx->series, 0, sizeof(double) * n_inst * ctx->cap);
-    if (he != hipSuccess) rc = fail(ctx, ODIS_E_HIP, hipGetErrorString(he));
+  // coefficient tables: chunk steps x B x 8 doubles, <= 4 MiB per buffer
+  ctx->chunk = int(std::max<long long>(1, std::min<long long>(4096, (4LL << 20) / (8LL * 8 * n_inst))));
+  const size_t tab = size_t(ctx->chunk) * n_inst * 8;
+  for (int k = 0; k < 2 && !rc; k++) {
+    rc = dev_alloc(ctx, &ctx->coef_dev[k], tab);
+    if (!rc) {
+      he = hipHostMalloc(reinterpret_cast<void **>(&ctx->coef_host[k]), tab * sizeof(double), hipHostMallocDefault);
+      if (he == hipSuccess) he = hipEventCreateWithFlags(&ctx->coef_ev[k], hipEventDisableTiming);
+      if (he == hipSuccess) he = hipEventRecord(ctx->coef_ev[k], ctx->stream);
+      if (he != hipSuccess) rc = fail(ctx, ODIS_E_HIP, std::string("coef table: ") + hipGetErrorString(he));
+    }
   }
-  if (!rc) rc = launch_prologue(ctx);
   if (!rc) {
-    he = hipStreamSynchronize(ctx->stream);
+    he = hipMemset(ctx->series, 0, sizeof(double) * n_inst * ctx->cap);
+    if (he == hipSuccess) he = hipMemset(ctx->nonfinite, 0, sizeof(int) * n_inst);
+    if (he == hipSuccess) he = hipDeviceSynchronize();
     if (he != hipSuccess) rc = fail(ctx, ODIS_E_HIP, hipGetErrorString(he));
   }
   if (rc) {
-    g_last_error = ctx->err;
+    const std::string msg = ctx->err;
     odis_destroy(ctx);
+    g_last_error = msg;
     return rc;
   }
   *out = ctx;
-  return ODIS_OK;
-}
-
-extern "C" int odis_destroy(odis_ctx *ctx) {
-  if (!ctx) return ODIS_E_INVALID;
-  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  for (auto &e : ctx->ev) (void)hipEventDestroy(e);
-  for (void *p : ctx->allocs) (void)hipFree(p);
-  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-  delete ctx;
   return ODIS_OK;
 }
 
@@ -610,6 +591,12 @@ extern "C" const char *odis_last_error(const odis_ctx *ctx) {
 // ---------------------------------------------------------------------------
 // state in / out (reference numbering and Array2D history layout)
 // ---------------------------------------------------------------------------
+//
+// The AB3 history lives in a 3-slot ring: step n writes its tendency f_n to
+// slot n % 3 and reads f_{n-1}, f_{n-2} from slots (n+2) % 3 and (n+1) % 3.
+// The reference keeps ds(:,0..2) and shifts them (src/temporalOperators.cpp:
+// 36-65); after step m >= 1 it holds (f_m, f_m, f_{m-1}), except after an
+// Euler step 0 (f_0, ds1 as set, f_0).  set/get translate between the two.
 
 template <typename T>
 static int put_state(odis_ctx *ctx, Ops<T> &o, int b, const double *v, const double *dv, const double *eta,
@@ -627,7 +614,6 @@ static int put_state(odis_ctx *ctx, Ops<T> &o, int b, const double *v, const dou
     for (int r = 0; r < N; r++) h[r] = T(eta[ctx->cell_inv[r]]);
     CK(hipMemcpy(o.eta + bN, h.data(), N * sizeof(T), hipMemcpyHostToDevice));
   }
-  // ring slots consumed by the next step k (see odis_get_state for the inverse)
   auto put_hist = [&](T *ring, int M, const std::vector<int> &inv, const double *ds,
                       std::vector<double> &keep) -> int {
     std::vector<T> r1(M), r2(M);
@@ -638,7 +624,7 @@ static int put_state(odis_ctx *ctx, Ops<T> &o, int b, const double *v, const dou
     keep.assign(ds, ds + size_t(3) * M);
     const size_t base = size_t(b) * 3 * M;
     if (k == 1 && !load) {
-      CK(hipMemcpy(ring + base + 0 * size_t(M), r2.data(), M * sizeof(T), hipMemcpyHostToDevice));
+      CK(hipMemcpy(ring + base, r2.data(), M * sizeof(T), hipMemcpyHostToDevice));
     } else {
       const int s1 = int((k + 2) % 3), s2 = int((k + 1) % 3);
       CK(hipMemcpy(ring + base + size_t(s1) * M, r1.data(), M * sizeof(T), hipMemcpyHostToDevice));
@@ -658,14 +644,9 @@ extern "C" int odis_set_state(odis_ctx *ctx, int inst, const double *v, const do
   if (inst < 0 || inst >= ctx->B) return fail(ctx, ODIS_E_RANGE, "instance out of range");
   if (iter < 0) return fail(ctx, ODIS_E_INVALID, "iter < 0");
   CK(hipStreamSynchronize(ctx->stream));
-  int rc = ctx->precision == ODIS_FP64 ? put_state(ctx, ctx->d, inst, v, dvdt, eta, detadt, iter)
-                                       : put_state(ctx, ctx->f, inst, v, dvdt, eta, detadt, iter);
-  if (rc) return rc;
-  StepInfo si{};
-  si.step = iter;
-  CK(hipMemcpy(ctx->info + inst, &si, sizeof(StepInfo), hipMemcpyHostToDevice));
-  ctx->host_step[inst] = iter;
-  return launch_prologue(ctx);
+  ctx->step = iter;  // the step counter is shared by all instances of a context
+  return ctx->precision == ODIS_FP64 ? put_state(ctx, ctx->d, inst, v, dvdt, eta, detadt, iter)
+                                     : put_state(ctx, ctx->f, inst, v, dvdt, eta, detadt, iter);
 }
 
 template <typename T>
@@ -690,7 +671,7 @@ static int get_state_t(odis_ctx *ctx, Ops<T> &o, int b, double *v, double *dv, d
     CK(hipMemcpy(all.data(), ring + size_t(b) * 3 * M, all.size() * sizeof(T), hipMemcpyDeviceToHost));
     for (int r = 0; r < M; r++) {
       const int i = inv[r];
-      if (k == 0) {  // no step taken since set_state: history as set
+      if (k == 0) {  // no step since set_state: the history as set
         for (int q = 0; q < 3; q++) ds[3 * i + q] = keep[3 * i + q];
         continue;
       }
@@ -717,12 +698,9 @@ extern "C" int odis_get_state(odis_ctx *ctx, int inst, double *v, double *dvdt, 
   if (!ctx) return ODIS_E_INVALID;
   if (inst < 0 || inst >= ctx->B) return fail(ctx, ODIS_E_RANGE, "instance out of range");
   CK(hipStreamSynchronize(ctx->stream));
-  StepInfo si;
-  CK(hipMemcpy(&si, ctx->info + inst, sizeof(StepInfo), hipMemcpyDeviceToHost));
-  const long long k = si.step;
-  if (iter) *iter = k;
-  return ctx->precision == ODIS_FP64 ? get_state_t(ctx, ctx->d, inst, v, dvdt, eta, detadt, k)
-                                     : get_state_t(ctx, ctx->f, inst, v, dvdt, eta, detadt, k);
+  if (iter) *iter = ctx->step;
+  return ctx->precision == ODIS_FP64 ? get_state_t(ctx, ctx->d, inst, v, dvdt, eta, detadt, ctx->step)
+                                     : get_state_t(ctx, ctx->f, inst, v, dvdt, eta, detadt, ctx->step);
 }
 
 // ---------------------------------------------------------------------------
@@ -749,7 +727,6 @@ static FaceArgs<T> face_args(odis_ctx *ctx, Ops<T> &o) {
   a.v_stride = ctx->E;
   a.ring_stride = 3LL * ctx->E;
   a.eta_stride = ctx->N;
-  a.info = ctx->info;
   a.partials = ctx->partials;
   a.inst = ctx->inst;
   a.quad_drag = ctx->quad_drag;
@@ -770,14 +747,37 @@ static CellArgs<T> cell_args(odis_ctx *ctx, Ops<T> &o) {
   a.v_stride = ctx->E;
   a.eta_stride = ctx->N;
   a.ring_stride = 3LL * ctx->N;
-  a.info = ctx->info;
   a.partials = ctx->partials;
   a.series = ctx->series;
+  a.nonfinite = ctx->nonfinite;
   a.cap = ctx->cap;
   a.inst = ctx->inst;
-  a.tide = ctx->tide;
   a.dconst = ctx->dconst;
   return a;
+}
+
+// Fill and upload the forcing coefficients of steps [n0, n0 + count) into the
+// next table buffer; returns its device pointer.
+static int stage_coefficients(odis_ctx *ctx, long long n0, int count, const double **dev) {
+  const int k = ctx->coef_next;
+  ctx->coef_next ^= 1;
+  // the previous upload from this pinned buffer must have been consumed
+  CK(hipEventSynchronize(ctx->coef_ev[k]));
+  double *h = ctx->coef_host[k];
+  const int B = ctx->B;
+  for (int s = 0; s < count; s++)
+    for (int b = 0; b < B; b++) {
+      double *c = h + (size_t(s) * B + b) * 8;
+      for (int q = 0; q < 8; q++) c[q] = 0.0;
+      const double dt = ctx->params[b].dt;
+      // forcing at current_time + dt, current_time = dt * iter (src/timeIntegrator.cpp:218)
+      tide_coeffs(ctx->tides[b], dt * double(n0 + s) + dt, c);
+    }
+  CK(hipMemcpyAsync(ctx->coef_dev[k], h, sizeof(double) * size_t(count) * B * 8, hipMemcpyHostToDevice,
+                    ctx->stream));
+  CK(hipEventRecord(ctx->coef_ev[k], ctx->stream));
+  *dev = ctx->coef_dev[k];
+  return ODIS_OK;
 }
 
 template <typename T>
@@ -792,21 +792,29 @@ static int step_t(odis_ctx *ctx, Ops<T> &o, int n_steps) {
     ctx->ev.push_back(e);
   }
   int done = 0;
-  for (int s = 0; s < n_steps; s++) {
-    if (ctx->stop_flag) {
-      ctx->stop_flag = 0;
-      return fail(ctx, ODIS_E_STOPPED, "stop requested");
+  while (done < n_steps) {
+    const int count = std::min(ctx->chunk, n_steps - done);
+    const double *coef = nullptr;
+    int rc = stage_coefficients(ctx, ctx->step, count, &coef);
+    if (rc) return rc;
+    for (int s = 0; s < count; s++, done++) {
+      if (ctx->stop_flag) {
+        ctx->stop_flag = 0;
+        return fail(ctx, ODIS_E_STOPPED, "stop requested");
+      }
+      const long long n = ctx->step;
+      const double *cs = coef + size_t(s) * ctx->B * 8;
+      FaceArgs<T> fa = face_args(ctx, o);
+      CellArgs<T> ca = cell_args(ctx, o);
+      if (ctx->timing) CK(hipEventRecord(ctx->ev[3 * done], ctx->stream));
+      hipLaunchKernelGGL((k_face<T, false>), gf, blk, 0, ctx->stream, fa, ctx->energy_quad, n, cs);
+      if (ctx->timing) CK(hipEventRecord(ctx->ev[3 * done + 1], ctx->stream));
+      hipLaunchKernelGGL((k_cell<T>), gc, blk, 0, ctx->stream, ca, n);
+      if (ctx->timing) CK(hipEventRecord(ctx->ev[3 * done + 2], ctx->stream));
+      CK(hipGetLastError());
+      ctx->cur ^= 1;
+      ctx->step = n + 1;
     }
-    FaceArgs<T> fa = face_args(ctx, o);
-    CellArgs<T> ca = cell_args(ctx, o);
-    if (ctx->timing) CK(hipEventRecord(ctx->ev[3 * s], ctx->stream));
-    hipLaunchKernelGGL((k_face<T, false>), gf, blk, 0, ctx->stream, fa, ctx->energy_quad);
-    if (ctx->timing) CK(hipEventRecord(ctx->ev[3 * s + 1], ctx->stream));
-    hipLaunchKernelGGL((k_cell<T>), gc, blk, 0, ctx->stream, ca);
-    if (ctx->timing) CK(hipEventRecord(ctx->ev[3 * s + 2], ctx->stream));
-    CK(hipGetLastError());
-    ctx->cur ^= 1;
-    done++;
   }
   if (ctx->timing && done > 0) {
     CK(hipEventSynchronize(ctx->ev[3 * (done - 1) + 2]));
@@ -832,7 +840,6 @@ extern "C" int odis_step(odis_ctx *ctx, int n_steps, int flags) {
   if (n_steps < 0) return fail(ctx, ODIS_E_INVALID, "n_steps < 0");
   int rc = ctx->precision == ODIS_FP64 ? step_t(ctx, ctx->d, n_steps) : step_t(ctx, ctx->f, n_steps);
   if (rc) return rc;
-  for (auto &s : ctx->host_step) s += n_steps;
   if (flags & ODIS_SYNC) CK(hipStreamSynchronize(ctx->stream));
   return ODIS_OK;
 }
@@ -847,7 +854,8 @@ template <typename T>
 static int diss_t(odis_ctx *ctx, Ops<T> &o) {
   FaceArgs<T> fa = face_args(ctx, o);
   const dim3 gf(ctx->nb_face, ctx->B), blk(kBlock);
-  hipLaunchKernelGGL((k_face<T, true>), gf, blk, 0, ctx->stream, fa, ctx->energy_quad);
+  hipLaunchKernelGGL((k_face<T, true>), gf, blk, 0, ctx->stream, fa, ctx->energy_quad, ctx->step,
+                     static_cast<const double *>(nullptr));
   hipLaunchKernelGGL(k_diss_final, dim3(ctx->B), blk, 0, ctx->stream, ctx->partials, ctx->nb_face, ctx->dconst,
                      ctx->scratch_out);
   CK(hipGetLastError());
@@ -871,10 +879,8 @@ extern "C" int odis_get_dissipation_series(odis_ctx *ctx, int inst, long long fi
   if (!ctx || !out || count < 0) return ODIS_E_INVALID;
   if (inst < 0 || inst >= ctx->B) return fail(ctx, ODIS_E_RANGE, "instance out of range");
   CK(hipStreamSynchronize(ctx->stream));
-  StepInfo si;
-  CK(hipMemcpy(&si, ctx->info + inst, sizeof(StepInfo), hipMemcpyDeviceToHost));
-  // series[k] exists for k < si.step and k >= si.step - cap
-  if (first < 0 || first + count > si.step || first < si.step - ctx->cap)
+  // series[k] exists for max(0, step - cap) <= k < step
+  if (first < 0 || first + count > ctx->step || first < ctx->step - ctx->cap)
     return fail(ctx, ODIS_E_RANGE, "series range not available");
   std::vector<double> all(ctx->cap);
   CK(hipMemcpy(all.data(), ctx->series + size_t(inst) * ctx->cap, sizeof(double) * ctx->cap,
@@ -900,8 +906,11 @@ static int vel_t(odis_ctx *ctx, Ops<T> &o, int b, const double *normal, double *
   }
   std::vector<double> h(size_t(2) * E);
   e = hipMemcpy(dn, nd.data(), sizeof(double) * 2 * E, hipMemcpyHostToDevice);
-  hipLaunchKernelGGL((k_velocity_en<T>), dim3((E + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, E,
-                     o.tri_idx, o.tri_coef, o.v[ctx->cur] + size_t(b) * E, dn, duv);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL((k_velocity_en<T>), dim3((E + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, E,
+                       o.tri_idx, o.tri_coef, o.v[ctx->cur] + size_t(b) * E, dn, duv);
+    e = hipGetLastError();
+  }
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e == hipSuccess) e = hipMemcpy(h.data(), duv, sizeof(double) * 2 * E, hipMemcpyDeviceToHost);
   (void)hipFree(dn);

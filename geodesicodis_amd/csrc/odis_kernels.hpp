@@ -11,9 +11,14 @@
 //     v'    = v + dt (a f0 + b f1 + c f2) + dt drag        AB3 + Euler drag (src/temporalOperators.cpp:36-47)
 //   cell kernel (one lane per cell i, reads v'):
 //     f0    = h (Div v')_i ; eta' = eta + dt (a f0 + b f1 + c f2)      updateEta + AB3
-//     the last workgroup to finish (atomic ticket) sums the face kernel's
-//     dissipation partials in a fixed order, evaluates the forcing time
-//     coefficients of the next step and advances the step counter.
+//     workgroup 0 also sums the face kernel's dissipation partials of this
+//     step in a fixed order (deterministic FP64 result).
+//
+// The step index n and the forcing time coefficients c_k(t_n + dt) are
+// computed on the host (glibc cos/sin, as the reference's forcing does) and
+// passed per step: n as a kernel argument, c_k through a small table that is
+// uploaded once per chunk of steps.  No device-scope atomics or fences are
+// used inside a step: kernel boundaries order the two kernels.
 //
 // Layout: structure-of-arrays, stencil slot-major ([slot][face]) so each
 // slot load is one coalesced 64-lane access; AB3 history is a 3-slot ring
@@ -39,14 +44,6 @@ struct TideDev {
   double radius;
 };
 
-struct StepInfo {
-  long long step;         // next step index n (t = dt n)
-  unsigned int ticket;    // workgroups of the cell kernel that finished
-  int nonfinite;          // set when a dissipation sum is not finite
-  double coef[kMaxForcing];  // forcing time coefficients of step n
-  double last_diss;       // dissipation of the state at the start of step n-1
-};
-
 template <typename T>
 struct FaceArgs {
   int E, nb;                 // faces, workgroups per instance
@@ -64,9 +61,8 @@ struct FaceArgs {
   T *ring;                   // [B][3][E]
   const T *eta;              // [B][N]
   long long v_stride, ring_stride, eta_stride;
-  StepInfo *info;            // [B]
   double *partials;          // [B][nb]
-  const double *inst;        // [B][8] g, alpha, dt, init_load, quad_drag(0/1), h
+  const double *inst;        // [B][8] g, alpha, dt, init_load, quad_coef, h
   int quad_drag;
 };
 
@@ -79,12 +75,11 @@ struct CellArgs {
   T *eta;                    // [B][N]
   T *ring;                   // [B][3][N]
   long long v_stride, eta_stride, ring_stride;
-  StepInfo *info;            // [B]
   const double *partials;    // [B][nb_face]
   double *series;            // [B][cap]
+  int *nonfinite;            // [B]
   int cap;
   const double *inst;        // [B][8]
-  const TideDev *tide;       // [B]
   const double *dconst;      // [B] dissipation constant
 };
 
