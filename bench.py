@@ -51,7 +51,20 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU work of the baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (20 times), to calibrate "
+                         "the FETCH_SIZE / WRITE_SIZE counters, and exit")
     args = ap.parse_args()
+    if args.copy_calibration_mib:
+        import torch
+        n = args.copy_calibration_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(20):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 8, "launches": 20}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU; multi-GPU runs are not part of it "
               "(requested --gpus %d)" % args.gpus, file=sys.stderr)

@@ -3,6 +3,7 @@
 // Replaces the body of ab3Explicit (src/timeIntegrator.cpp:57-322) for one
 // or more ocean instances sharing one grid.  See odis_kernels.hpp for the
 // per-step dataflow.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -783,9 +784,9 @@ static int stage_coefficients(odis_ctx *ctx, long long n0, int count, const doub
 template <typename T>
 static int step_t(odis_ctx *ctx, Ops<T> &o, int n_steps) {
   const dim3 gf(ctx->nb_face, ctx->B), gc(ctx->nb_cell, ctx->B), blk(kBlock);
-  // timing mode: three events per step on the context stream, read once at
-  // the end so the host never stalls the queue inside the timed region
-  const size_t need = ctx->timing ? size_t(3) * n_steps : 0;
+  // timing mode: start/end events of both kernels of every step, read once
+  // at the end so the host never stalls the queue inside the timed region
+  const size_t need = ctx->timing ? size_t(4) * n_steps : 0;
   while (ctx->ev.size() < need) {
     hipEvent_t e;
     CK(hipEventCreate(&e));
@@ -806,28 +807,34 @@ static int step_t(odis_ctx *ctx, Ops<T> &o, int n_steps) {
       const double *cs = coef + size_t(s) * ctx->B * 8;
       FaceArgs<T> fa = face_args(ctx, o);
       CellArgs<T> ca = cell_args(ctx, o);
-      if (ctx->timing) CK(hipEventRecord(ctx->ev[3 * done], ctx->stream));
-      hipLaunchKernelGGL((k_face<T, false>), gf, blk, 0, ctx->stream, fa, ctx->energy_quad, n, cs);
-      if (ctx->timing) CK(hipEventRecord(ctx->ev[3 * done + 1], ctx->stream));
-      hipLaunchKernelGGL((k_cell<T>), gc, blk, 0, ctx->stream, ca, n);
-      if (ctx->timing) CK(hipEventRecord(ctx->ev[3 * done + 2], ctx->stream));
+      if (ctx->timing) {
+        // events stamped by the kernel dispatch itself (start / end of the
+        // kernel), not by separate marker packets
+        hipExtLaunchKernelGGL((k_face<T, false>), gf, blk, 0, ctx->stream, ctx->ev[4 * done],
+                              ctx->ev[4 * done + 1], 0, fa, ctx->energy_quad, n, cs);
+        hipExtLaunchKernelGGL((k_cell<T>), gc, blk, 0, ctx->stream, ctx->ev[4 * done + 2], ctx->ev[4 * done + 3],
+                              0, ca, n);
+      } else {
+        hipLaunchKernelGGL((k_face<T, false>), gf, blk, 0, ctx->stream, fa, ctx->energy_quad, n, cs);
+        hipLaunchKernelGGL((k_cell<T>), gc, blk, 0, ctx->stream, ca, n);
+      }
       CK(hipGetLastError());
       ctx->cur ^= 1;
       ctx->step = n + 1;
     }
   }
   if (ctx->timing && done > 0) {
-    CK(hipEventSynchronize(ctx->ev[3 * (done - 1) + 2]));
+    CK(hipEventSynchronize(ctx->ev[4 * (done - 1) + 3]));
     double tf = 0, tc = 0;
     for (int s = 0; s < done; s++) {
       float a = 0, b = 0;
-      CK(hipEventElapsedTime(&a, ctx->ev[3 * s], ctx->ev[3 * s + 1]));
-      CK(hipEventElapsedTime(&b, ctx->ev[3 * s + 1], ctx->ev[3 * s + 2]));
+      CK(hipEventElapsedTime(&a, ctx->ev[4 * s], ctx->ev[4 * s + 1]));
+      CK(hipEventElapsedTime(&b, ctx->ev[4 * s + 2], ctx->ev[4 * s + 3]));
       tf += a;
       tc += b;
     }
     float all = 0;
-    CK(hipEventElapsedTime(&all, ctx->ev[0], ctx->ev[3 * (done - 1) + 2]));
+    CK(hipEventElapsedTime(&all, ctx->ev[0], ctx->ev[4 * (done - 1) + 3]));
     ctx->t_face = tf / done;
     ctx->t_cell = tc / done;
     ctx->t_step = double(all) / done;

@@ -1,0 +1,204 @@
+"""Host-side product code (no GPU): the C ABI exports, the grid generator and
+reader, the TRiSK mesh assembly against the oracle, and input.in parsing."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import REFERENCE_INPUTS, ROOT, generated_grid, have_reference_grids
+
+MESH_ARRAYS = ["cv_area", "faces", "node_face_dir", "face_nodes", "face_len", "face_node_dist",
+               "face_centre_lat", "face_centre_lon", "face_centre_m", "face_normal",
+               "face_intercept", "face_area", "interp_count", "interp_friends", "interp_weights"]
+
+
+def declared_functions():
+    with open(os.path.join(ROOT, "include", "odis.h")) as fh:
+        text = fh.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(odis_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol(product):
+    import ctypes
+    lib = ctypes.CDLL(product.LIB_PATH)
+    names = declared_functions()
+    assert len(names) >= 30
+    for nm in names:
+        assert hasattr(lib, nm), nm
+    assert sorted(product.EXPORTS) == names
+
+
+def test_oracle_library_loads(oracle):
+    lib = oracle.lib()
+    for nm in ("orc_mesh_build", "orc_sim_step", "orc_analytical_obliq_west"):
+        assert hasattr(lib, nm)
+
+
+@pytest.mark.parametrize("level", [3, 4, 5])
+def test_mesh_matches_oracle_generated(product, oracle, level):
+    """The product's TRiSK assembly reproduces the oracle (bit for bit up to
+    one ulp of a libm result)."""
+    g, arrs = generated_grid(level)
+    pm = product.Mesh(g, 252.1e3)
+    om = oracle.Mesh(arrs, 252.1e3)
+    for nm in MESH_ARRAYS:
+        assert_same(pm.get(nm), om.get(nm), nm)
+
+
+@pytest.mark.skipif(not have_reference_grids(), reason="reference grid files not mounted")
+@pytest.mark.parametrize("level", [3, 4, 5, 6])
+def test_mesh_matches_oracle_reference_grids(product, oracle, level):
+    path = os.path.join(REFERENCE_INPUTS, "grid_l%d.txt" % level)
+    g = product.Grid.read(path)
+    arrs = oracle.read_reference_grid(path)
+    a = g.arrays()
+    for k in ("lat", "lon", "friends", "clat", "clon"):
+        assert np.array_equal(a[k], arrs[k]), k
+    pm = product.Mesh(g, 252.1e3)
+    om = oracle.Mesh(arrs, 252.1e3)
+    for nm in MESH_ARRAYS:
+        assert np.array_equal(pm.get(nm), om.get(nm)), nm
+
+
+def assert_same(a, b, nm):
+    """Integer arrays equal; float arrays equal up to 1 ulp of libm (the two
+    compilers may fuse sin/cos calls differently)."""
+    if a.dtype.kind == "i":
+        assert np.array_equal(a, b), nm
+    else:
+        scale = np.maximum(np.abs(b), np.finfo(float).tiny)
+        assert np.max(np.abs(a - b) / scale) <= 4e-16, nm
+
+
+def _xyz(lat, lon):
+    return np.stack([np.cos(lat) * np.cos(lon), np.cos(lat) * np.sin(lon), np.sin(lat)], -1)
+
+
+@pytest.mark.parametrize("level", [1, 2, 3, 4, 5])
+def test_generated_grid_structure(product, level):
+    g = product.Grid.generate(level, 64)
+    a = g.arrays()
+    n = 10 * 4 ** (level - 1) + 2
+    assert g.n == n
+    fr = a["friends"]
+    assert list(np.where(fr[:, 5] < 0)[0]) == list(range(12))
+    P = _xyz(a["lat"], a["lon"])
+    C = _xyz(a["clat"], a["clon"])
+    for i in range(n):
+        k = 5 if fr[i, 5] < 0 else 6
+        nb = fr[i, :k]
+        # neighbour relation is symmetric
+        for f in nb:
+            assert i in fr[f]
+        # clockwise seen from outside: consecutive neighbour cross products point inward
+        for j in range(k):
+            d1, d2 = P[nb[j]] - P[i], P[nb[(j + 1) % k]] - P[i]
+            assert np.dot(np.cross(d1, d2), P[i]) < 0
+            # vertex j is the circumcentre of (i, f_j, f_j+1)
+            dd = [np.arccos(np.clip(np.dot(C[i, j], P[q]), -1, 1)) for q in (i, nb[j], nb[(j + 1) % k])]
+            assert max(dd) - min(dd) < 1e-9
+    # every vertex is listed bit-identically by its three cells
+    m = product.Mesh(g, 1.0)
+    assert m.V == 2 * n - 4
+
+
+def test_generated_grid_statistics_match_reference_levels(product):
+    """The generated grids reproduce the shipped SCVT grids' sums of
+    d_e l_e / 4 pi R^2 (tests/golden/grid_invariants.json)."""
+    import json
+    with open(os.path.join(ROOT, "tests", "golden", "grid_invariants.json")) as fh:
+        fx = json.load(fh)
+    for level in (4, 5, 6):
+        g, _ = generated_grid(level)
+        m = product.Mesh(g, 1.0)
+        s = m.get("face_area").sum() / (4 * np.pi)
+        assert abs(s - fx["grid_l%d" % level]["sum_de_le_over_4pi"]) < 2e-5
+
+
+def test_grid_file_roundtrip(product, tmp_path):
+    g, _ = generated_grid(3)
+    p = tmp_path / "grid_l3.txt"
+    g.write(p)
+    g2 = product.Grid.read(p)
+    a, b = g.arrays(), g2.arrays()
+    assert np.array_equal(a["friends"], b["friends"])
+    for k in ("lat", "lon", "clat", "clon"):
+        assert np.max(np.abs(a[k] - b[k])) < 1e-15
+
+
+def test_grid_read_missing_file(product):
+    with pytest.raises(product.OdisError) as ei:
+        product.Grid.read("/nonexistent/grid_l9.txt")
+    assert ei.value.code == -8
+
+
+INPUT_ENCELADUS = """radius; \t 252.1e3; \t top radius \t;
+k2;  0.0;   k2 Love number;
+h2;  0.0;  h2 Love number;
+love reduction factor;  7.0;  overwritten by the FREE boundary condition;
+ANGULAR VELOCITY;  5.307e-5;  Spin rate/mean motion;
+surface gravity;  0.113;  gravity;
+eccentricity;  0.0047;  e;
+obliquity;  0.5;  degrees;
+ocean thickness;  38e3;  h_o;
+friction coefficient;  1e-8;  alpha;
+friction type;  LINEAR;  friction;
+potential;  ECC;  potential;
+surface type;  FREE;  top boundary condition;
+advection;  false;  advection switch;
+solver type;  AB3;  solver;
+geodesic grid level;  6;  recursion level;
+output time;  10;  data out opt;
+initial conditions;  NONE;  ic;
+simulation end time;  10;  endTime;
+time step;  30;  timeStep;
+unknown key;  123;  ignored;
+"""
+
+
+def test_config_reference_semantics(product, tmp_path):
+    (tmp_path / "input.in").write_text(INPUT_ENCELADUS)
+    p, info = product.read_config(tmp_path)
+    assert p.radius == 252.1e3 and p.g == 0.113 and p.h == 38e3
+    assert p.omega == 2 * np.pi / 118394.0           # period rounded to even seconds
+    assert p.dt == 118394.0 / 4000 and info["steps_per_orbit"] == 4000
+    assert p.love_reduct == 1.0                       # FREE: 1 + k2 - h2
+    assert p.obliquity == 0.5 * np.pi / 180.
+    assert p.tide == product.ECC and p.friction == product.LINEAR and p.surface == product.FREE
+    assert info["grid_level"] == 6 and info["output_time"] == 10 and info["end_time"] == 10
+
+
+@pytest.mark.parametrize("key,val", [("potential", "ECCX"), ("friction type", "CUBIC"),
+                                     ("surface type", "NOPE"), ("solver type", "RK9"),
+                                     ("initial conditions", "SOME")])
+def test_config_errors(product, tmp_path, key, val):
+    text = re.sub(r"(?m)^%s;[^;]*;" % re.escape(key), "%s; %s;" % (key, val), INPUT_ENCELADUS)
+    (tmp_path / "input.in").write_text(text)
+    with pytest.raises(product.OdisError):
+        product.read_config(tmp_path)
+
+
+def test_config_membrane_shell(product, tmp_path):
+    """LID_MEMBR: membraneNuBeta moves g and R to the ocean top and sets
+    loveReduct = nu_2 (src/boundaryConditions.cpp:81-108)."""
+    text = INPUT_ENCELADUS.replace("FREE;", "LID_MEMBR;") + "shell thickness; 20e3; h_s;\nsh degree; 4; l;\n"
+    (tmp_path / "input.in").write_text(text)
+    p, _ = product.read_config(tmp_path)
+    assert p.radius == 232.1e3 and 0.1 < p.g < 0.12
+    assert 0.9 < p.love_reduct < 1.0 and p.surface == product.LID_MEMBR
+
+
+def test_run_case_without_input(product, tmp_path):
+    with pytest.raises(product.OdisError):
+        product.run_case(tmp_path)
+    assert "Unable to open" in (tmp_path / "DATA" / "ERROR.txt").read_text()
+
+
+def test_params_layout_matches_header(product):
+    """odis_params / odis_mesh_desc field order and sizes as in include/odis.h."""
+    import ctypes
+    assert ctypes.sizeof(product.Params) == 11 * 8 + 6 * 4 + 0
+    assert [f for f, _ in product.Params._fields_][:3] == ["g", "h", "alpha"]
+    assert ctypes.sizeof(product.MeshDesc) == 8 + 11 * 8
