@@ -35,7 +35,7 @@ EXPORTS = [
     "odis_grid_arrays", "odis_grid_from_arrays", "odis_grid_free", "odis_mesh_build",
     "odis_mesh_sizes", "odis_mesh_describe", "odis_mesh_get", "odis_mesh_free",
     "odis_round_period", "odis_quantise_dt", "odis_config_read", "odis_host_last_error",
-    "odis_run_case",
+    "odis_run_case", "odis_interp_ll_set", "odis_interp_ll",
 ]
 
 
@@ -116,6 +116,8 @@ def lib():
                                  ctypes.c_int),
             "odis_host_last_error": ([], ctypes.c_char_p),
             "odis_run_case": ([ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
+            "odis_interp_ll_set": ([vp, ctypes.c_int, ctypes.c_int, ip, ip, dp], ctypes.c_int),
+            "odis_interp_ll": ([vp, ctypes.c_int, dp], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -329,6 +331,22 @@ class Context:
         uv = np.zeros(self.mesh.E * 2)
         self._check(lib().odis_get_velocity_en(self._h, int(inst), _dp(normal), _dp(uv)))
         return uv.reshape(-1, 2)
+
+    def set_interp_ll(self, row_ptr, col, val, n_cols=None):
+        """Attach the geodesic -> lat-lon CSR matrix (3N columns, reference
+        cell numbering, interleaved [eta, r deta/dlat, r deta/dlon])."""
+        rp = np.ascontiguousarray(row_ptr, np.int32)
+        c = np.ascontiguousarray(col, np.int32)
+        v = np.ascontiguousarray(val, np.float64)
+        self._ll_rows = len(rp) - 1
+        self._check(lib().odis_interp_ll_set(self._h, self._ll_rows,
+                                             int(n_cols if n_cols is not None else 3 * self.mesh.N),
+                                             _ip(rp), _ip(c), _dp(v)))
+
+    def interp_ll(self, inst=0):
+        out = np.zeros(self._ll_rows)
+        self._check(lib().odis_interp_ll(self._h, int(inst), _dp(out)))
+        return out
 
     def request_stop(self):
         self._check(lib().odis_request_stop(self._h))

@@ -152,6 +152,20 @@ __global__ __launch_bounds__(kBlock) void k_velocity_en(int E, const int *tri_id
   uv[2 * e + 1] = ny * vn + (-nx) * vt;
 }
 
+// CSR rows of the lat-lon interpolation: one wavefront per row, the eta
+// columns already mapped to device cell indices on the host.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_interp_ll(int n_rows, const int *rp, const int *col,
+                                                      const double *val, const T *eta, double *out) {
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (row >= n_rows) return;
+  double acc = 0.0;
+  for (int k = rp[row] + lane; k < rp[row + 1]; k += 64) acc += val[k] * double(eta[col[k]]);
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if (lane == 0) out[row] = acc;
+}
+
 }  // namespace odis
 
 // ===========================================================================
@@ -239,6 +253,11 @@ struct odis_ctx {
   int timing = 0;
   std::vector<hipEvent_t> ev;
   double t_face = 0, t_cell = 0, t_step = 0;
+  // lat-lon interpolation operator (eta columns only, device cell indices)
+  int ll_rows = 0;
+  int *ll_rp = nullptr, *ll_col = nullptr;
+  double *ll_val = nullptr, *ll_out = nullptr;
+  std::vector<void *> ll_allocs;
   std::string err;
 };
 
@@ -384,6 +403,7 @@ extern "C" int odis_destroy(odis_ctx *ctx) {
     if (ctx->coef_host[k]) (void)hipHostFree(ctx->coef_host[k]);
   }
   for (void *p : ctx->allocs) (void)hipFree(p);
+  for (void *p : ctx->ll_allocs) (void)hipFree(p);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return ODIS_OK;
@@ -955,5 +975,66 @@ extern "C" int odis_get_timing(odis_ctx *ctx, double *face_ms, double *cell_ms, 
   if (face_ms) *face_ms = ctx->t_face;
   if (cell_ms) *cell_ms = ctx->t_cell;
   if (step_ms) *step_ms = ctx->t_step;
+  return ODIS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// geodesic -> lat-lon interpolation (src/interpolation.cpp:208-287)
+// ---------------------------------------------------------------------------
+
+extern "C" int odis_interp_ll_set(odis_ctx *ctx, int n_rows, int n_cols, const int *row_ptr, const int *col,
+                                  const double *val) {
+  if (!ctx || !row_ptr || !col || !val || n_rows < 1) return ODIS_E_INVALID;
+  if (n_cols != 3 * ctx->N) return fail(ctx, ODIS_E_INVALID, "interpolation matrix must have 3N columns");
+  if (row_ptr[0] != 0) return fail(ctx, ODIS_E_INVALID, "row_ptr[0] must be 0");
+  for (int r = 0; r < n_rows; r++)
+    if (row_ptr[r + 1] < row_ptr[r]) return fail(ctx, ODIS_E_INVALID, "row_ptr not monotone");
+  const int nnz = row_ptr[n_rows];
+  std::vector<int> rp(n_rows + 1, 0), cols;
+  std::vector<double> vals;
+  cols.reserve(nnz / 3 + 1);
+  vals.reserve(nnz / 3 + 1);
+  for (int r = 0; r < n_rows; r++) {
+    for (int k = row_ptr[r]; k < row_ptr[r + 1]; k++) {
+      const int c = col[k];
+      if (c < 0 || c >= n_cols) return fail(ctx, ODIS_E_INVALID, "interpolation column out of range");
+      if (c % 3 != 0) continue;  // gradient entries of x are zero in the reference
+      cols.push_back(ctx->cell_perm[c / 3]);
+      vals.push_back(val[k]);
+    }
+    rp[r + 1] = int(cols.size());
+  }
+  CK(hipStreamSynchronize(ctx->stream));
+  for (void *p : ctx->ll_allocs) (void)hipFree(p);
+  ctx->ll_allocs.clear();
+  auto up = [&](void **dst, const void *src, size_t bytes) -> int {
+    CK(hipMalloc(dst, std::max<size_t>(bytes, 8)));
+    ctx->ll_allocs.push_back(*dst);
+    if (src) CK(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    return ODIS_OK;
+  };
+  int rc;
+  if ((rc = up(reinterpret_cast<void **>(&ctx->ll_rp), rp.data(), rp.size() * sizeof(int)))) return rc;
+  if ((rc = up(reinterpret_cast<void **>(&ctx->ll_col), cols.data(), cols.size() * sizeof(int)))) return rc;
+  if ((rc = up(reinterpret_cast<void **>(&ctx->ll_val), vals.data(), vals.size() * sizeof(double)))) return rc;
+  if ((rc = up(reinterpret_cast<void **>(&ctx->ll_out), nullptr, size_t(n_rows) * sizeof(double)))) return rc;
+  ctx->ll_rows = n_rows;
+  return ODIS_OK;
+}
+
+extern "C" int odis_interp_ll(odis_ctx *ctx, int inst, double *ll) {
+  if (!ctx || !ll) return ODIS_E_INVALID;
+  if (inst < 0 || inst >= ctx->B) return fail(ctx, ODIS_E_RANGE, "instance out of range");
+  if (!ctx->ll_rows) return fail(ctx, ODIS_E_INVALID, "no interpolation matrix set");
+  const dim3 grid((ctx->ll_rows + kBlock / 64 - 1) / (kBlock / 64)), blk(kBlock);
+  if (ctx->precision == ODIS_FP64)
+    hipLaunchKernelGGL((k_interp_ll<double>), grid, blk, 0, ctx->stream, ctx->ll_rows, ctx->ll_rp, ctx->ll_col,
+                       ctx->ll_val, ctx->d.eta + size_t(inst) * ctx->N, ctx->ll_out);
+  else
+    hipLaunchKernelGGL((k_interp_ll<float>), grid, blk, 0, ctx->stream, ctx->ll_rows, ctx->ll_rp, ctx->ll_col,
+                       ctx->ll_val, ctx->f.eta + size_t(inst) * ctx->N, ctx->ll_out);
+  CK(hipGetLastError());
+  CK(hipMemcpyAsync(ll, ctx->ll_out, sizeof(double) * ctx->ll_rows, hipMemcpyDeviceToHost, ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
   return ODIS_OK;
 }

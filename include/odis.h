@@ -139,6 +139,17 @@ int odis_series_capacity(const odis_ctx *ctx);
 /* Reconstructed (east, north) velocity per face, interpolateVelocity
  * (src/interpolation.cpp:26-62); uv[E][2]. */
 int odis_get_velocity_en(odis_ctx *ctx, int inst, const double *face_normal, double *uv);
+/* Geodesic -> lat-lon interpolation, interpolateGG2LLConservative
+ * (src/interpolation.cpp:208-287): ll = W x with x the interleaved
+ * [eta_i, r d(eta)/dlat_i, r d(eta)/dlon_i] of every cell i (3N columns in
+ * reference cell numbering).  As in the reference, the gradient entries of x
+ * are zero (its pressureGradient call is commented out, :236), so only the
+ * eta columns contribute.  W is a CSR matrix of n_rows = (180/d)(360/d) rows
+ * in the reference's row-major lat-lon order; it is copied to the device. */
+int odis_interp_ll_set(odis_ctx *ctx, int n_rows, int n_cols, const int *row_ptr,
+                       const int *col, const double *val);
+int odis_interp_ll(odis_ctx *ctx, int inst, double *ll);
+
 /* Async-signal-safe stop request (replaces CatchExit, src/timeIntegrator.cpp:34-37):
  * odis_step returns ODIS_E_STOPPED at the next step boundary. */
 int odis_request_stop(odis_ctx *ctx);

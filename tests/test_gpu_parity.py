@@ -227,3 +227,55 @@ def test_stop_request(product):
     assert ctx.get_state()["iter"] == 0
     ctx.step(3)
     assert ctx.get_state()["iter"] == 3
+
+
+def _synthetic_ll_matrix(arrs, d_deg, rng):
+    """A lat-lon interpolation matrix shaped like the reference's weight files
+    (rows (180/d)(360/d) row-major from the south-west corner, 3N columns
+    interleaved [eta, r d/dlat, r d/dlon]): inverse-distance weights to the
+    3 nearest cells, plus non-zero gradient entries (multiplied by zero)."""
+    lat, lon = arrs["lat"], arrs["lon"]
+    P = np.stack([np.cos(lat) * np.cos(lon), np.cos(lat) * np.sin(lon), np.sin(lat)], 1)
+    nla, nlo = 180 // d_deg, 360 // d_deg
+    rp, col, val = [0], [], []
+    for i in range(nla):
+        for j in range(nlo):
+            la = np.radians(-90 + d_deg * (i + 0.5))
+            lo = np.radians(d_deg * (j + 0.5))
+            q = np.array([np.cos(la) * np.cos(lo), np.cos(la) * np.sin(lo), np.sin(la)])
+            dist = np.arccos(np.clip(P @ q, -1, 1))
+            near = np.argsort(dist)[:3]
+            w = 1.0 / (dist[near] + 1e-3)
+            w /= w.sum()
+            for c, ww in zip(near, w):
+                col += [3 * c, 3 * c + 1, 3 * c + 2]
+                val += [ww, rng.standard_normal(), rng.standard_normal()]
+            rp.append(len(col))
+    return np.array(rp), np.array(col), np.array(val)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_interp_gg2ll(product, oracle, precision):
+    """ll = W [eta, 0, 0] (src/interpolation.cpp:236-281) on the GPU against a
+    CSR product in numpy."""
+    G = product
+    g, arrs = generated_grid(4)
+    p, _ = G.make_params()
+    mesh = G.Mesh(g, p.radius)
+    ctx = G.Context(mesh, p, precision=precision)
+    ctx.step(300)
+    eta = ctx.get_state()["eta"]
+    rng = np.random.default_rng(7)
+    for d in (6, 10):
+        rp, col, val = _synthetic_ll_matrix(arrs, d, rng)
+        ctx.set_interp_ll(rp, col, val)
+        ll = ctx.interp_ll(0)
+        import scipy.sparse as sp
+        W = sp.csr_matrix((val, col, rp), shape=(len(rp) - 1, 3 * mesh.N))
+        x = np.zeros(3 * mesh.N)
+        x[0::3] = eta
+        ref = W @ x
+        assert ll.shape == ((180 // d) * (360 // d),)
+        assert rel_l2(ll, ref) < (1e-14 if precision == 0 else 1e-6)
+    with pytest.raises(G.OdisError):
+        ctx.set_interp_ll(rp, col, val, n_cols=3 * mesh.N + 3)

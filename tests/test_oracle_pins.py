@@ -413,3 +413,40 @@ def test_membrane_love_numbers_cross_check(oracle):
     assert np.all((ratio > 0.8) & (ratio < 1.35)), ratio
     assert np.all(np.diff(beta[2:19]) > 0)
     assert 0.8 < nu[2] < 1.1 and abs(fx["upsilon2_hs20km"] - 1.1176) < 1e-4
+
+
+def _h5_dataset(path, name):
+    import re
+    import subprocess
+    out = subprocess.run(["/opt/conda/bin/h5dump", "-d", name, "-y", "-w", "0", path],
+                         capture_output=True, text=True, check=True).stdout
+    body = out.split("DATA {", 1)[1].split("}", 1)[0]
+    return np.array([float(x) for x in re.split(r"[,\s]+", body.strip()) if x])
+
+
+@pytest.mark.skipif(not (have_reference_grids() and os.path.exists("/opt/conda/bin/h5dump")),
+                    reason="reference weight files or h5dump not available")
+@pytest.mark.parametrize("res", [6, 10])
+def test_reference_ll_weights_layout(res):
+    """The shipped level-4 lat-lon weights (input_files/grid_l4_<d>x<d>_weights.h5,
+    loaded by the commented ReadWeightingFile, src/mesh.cpp:4238-4358) have
+    (180/d)(360/d) CSR rows and 3N columns, and the eta columns (3i) carry the
+    interpolation weights (rows sum to ~1) while the gradient columns do not:
+    the interleaved [eta, r deta/dlat, r deta/dlon] layout of
+    src/interpolation.cpp:246-249 that odis_interp_ll_set assumes."""
+    path = os.path.join(REFERENCE_INPUTS, "grid_l4_%dx%d_weights.h5" % (res, res))
+    col = _h5_dataset(path, "/column index").astype(int)
+    rp = _h5_dataset(path, "/row index").astype(int)
+    w = _h5_dataset(path, "/weights")
+    n_rows = (180 // res) * (360 // res)
+    assert len(rp) == n_rows + 1 and rp[-1] == len(w) == len(col)
+    assert col.min() >= 0 and col.max() < 3 * 642
+    W = sp.csr_matrix((w, col, rp), shape=(n_rows, 3 * 642))
+    x = np.zeros(3 * 642)
+    x[0::3] = 1.0
+    s0 = W @ x
+    assert np.median(s0) == pytest.approx(1.0, abs=5e-3)
+    for k in (1, 2):
+        x = np.zeros(3 * 642)
+        x[k::3] = 1.0
+        assert np.median(np.abs(W @ x)) < 0.1
