@@ -35,8 +35,10 @@ EXPORTS = [
     "odis_grid_arrays", "odis_grid_from_arrays", "odis_grid_free", "odis_mesh_build",
     "odis_mesh_sizes", "odis_mesh_describe", "odis_mesh_get", "odis_mesh_free",
     "odis_round_period", "odis_quantise_dt", "odis_config_read", "odis_host_last_error",
-    "odis_run_case", "odis_interp_ll_set", "odis_interp_ll",
+    "odis_run_case", "odis_interp_ll_set", "odis_interp_ll", "odis_sh_set_basis",
+    "odis_sh_project", "odis_sh_synthesize", "odis_surface_beta",
 ]
+SH_OFF, SH_FP64, SH_SPLIT_BF16 = 0, 1, 2
 
 
 class OdisError(RuntimeError):
@@ -118,6 +120,11 @@ def lib():
             "odis_run_case": ([ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
             "odis_interp_ll_set": ([vp, ctypes.c_int, ctypes.c_int, ip, ip, dp], ctypes.c_int),
             "odis_interp_ll": ([vp, ctypes.c_int, dp], ctypes.c_int),
+            "odis_sh_set_basis": ([vp, ctypes.c_int, dp, ctypes.c_int], ctypes.c_int),
+            "odis_sh_project": ([vp, dp, dp], ctypes.c_int),
+            "odis_sh_synthesize": ([vp, dp, dp], ctypes.c_int),
+            "odis_surface_beta": ([ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_double, dp], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -348,6 +355,28 @@ class Context:
         self._check(lib().odis_interp_ll(self._h, int(inst), _dp(out)))
         return out
 
+    def set_sh(self, l_max, beta=None, mode=SH_FP64):
+        """Self-gravity / shell coupling: beta is (n_inst, l_max + 1) or
+        (l_max + 1,) per-degree factors; mode SH_OFF / SH_FP64 / SH_SPLIT_BF16."""
+        b = None
+        if beta is not None:
+            b = np.ascontiguousarray(np.broadcast_to(np.asarray(beta, np.float64),
+                                                     (self.B, l_max + 1)))
+        self.sh_l_max = l_max
+        self._check(lib().odis_sh_set_basis(self._h, int(l_max), _dp(b), int(mode)))
+
+    def sh_project(self, field):
+        f = np.ascontiguousarray(field, np.float64)
+        c = np.zeros((self.sh_l_max + 1) ** 2)
+        self._check(lib().odis_sh_project(self._h, _dp(f), _dp(c)))
+        return c
+
+    def sh_synthesize(self, coeffs):
+        c = np.ascontiguousarray(coeffs, np.float64)
+        f = np.zeros(self.mesh.N)
+        self._check(lib().odis_sh_synthesize(self._h, _dp(c), _dp(f)))
+        return f
+
     def request_stop(self):
         self._check(lib().odis_request_stop(self._h))
 
@@ -366,6 +395,15 @@ class Context:
 
     def __del__(self):
         self.close()
+
+
+def surface_beta(surface, l_max, top_radius, g, shell_thickness, ocean_thickness):
+    """Per-degree pressure factors beta_l of a surface type (membraneNuBeta for
+    LID_MEMBR, 1 - loading factor for FREE_LOADING, else 1)."""
+    b = np.zeros(l_max + 1)
+    _host_check(lib().odis_surface_beta(int(surface), int(l_max), float(top_radius), float(g),
+                                        float(shell_thickness), float(ocean_thickness), _dp(b)))
+    return b
 
 
 def run_case(case_dir, grid_dir=None, precision=FP64):

@@ -230,3 +230,11 @@ extern "C" int odis_config_read(const char *case_dir, odis_params *p, int *steps
     return host_fail(ODIS_E_INVALID, e.what());
   }
 }
+
+extern "C" int odis_surface_beta(int surface, int l_max, double radius, double g, double shell, double ocean,
+                                 double *beta) {
+  if (!beta || l_max < 0 || l_max > 1000) return host_fail(ODIS_E_INVALID, "bad argument");
+  const std::vector<double> b = odis::surface_beta(surface, l_max, radius, g, shell, ocean);
+  for (int l = 0; l <= l_max; l++) beta[l] = b[l];
+  return ODIS_OK;
+}

@@ -166,6 +166,115 @@ __global__ __launch_bounds__(kBlock) void k_interp_ll(int n_rows, const int *rp,
   if (lane == 0) out[row] = acc;
 }
 
+// ---------------------------------------------------------------------------
+// spherical-harmonic projection (self-gravity / ice-shell coupling)
+// ---------------------------------------------------------------------------
+// Basis Yt[k][i] (k = harmonic, i = device cell) stored basis-major so that
+// the analysis b_k = sum_i Yt[k][i] x_i streams row k (one workgroup per k)
+// and the synthesis y_i = sum_k c_k Yt[k][i] streams the rows with one lane
+// per cell.  The split-BF16 form packs hi = bf16(Y) in the upper and
+// lo = bf16(Y - hi) in the lower half of a 32-bit word.
+
+__device__ __forceinline__ float bf16_hi(unsigned int u) { return __uint_as_float(u & 0xFFFF0000u); }
+__device__ __forceinline__ float bf16_lo(unsigned int u) { return __uint_as_float(u << 16); }
+
+template <typename T, bool SPLIT>
+__global__ __launch_bounds__(kBlock) void k_sh_analysis(int K, int N, const double *Yt, const unsigned int *Yp,
+                                                        const T *x, long long x_stride, double *b_out) {
+  __shared__ double sh[4];
+  const int k = blockIdx.x, b = blockIdx.y;
+  const T *xb = x + b * x_stride;
+  double acc = 0.0;
+  if (SPLIT) {
+    float facc = 0.f;
+    const unsigned int *row = Yp + size_t(k) * N;
+    for (int i = threadIdx.x; i < N; i += kBlock) {
+      const unsigned int u = row[i];
+      const float xi = float(xb[i]);
+      facc = fmaf(bf16_hi(u), xi, facc);
+      facc = fmaf(bf16_lo(u), xi, facc);
+    }
+    acc = double(facc);
+  } else {
+    const double *row = Yt + size_t(k) * N;
+    for (int i = threadIdx.x; i < N; i += kBlock) acc += row[i] * double(xb[i]);
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) sh[wave] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) b_out[size_t(b) * K + k] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+// c_k = D_k sum_j Minv[k][j] b_j : one wavefront per k
+__global__ __launch_bounds__(kBlock) void k_sh_coef(int K, const double *Minv, const double *D, const double *bvec,
+                                                    double *c) {
+  const int lane = threadIdx.x & 63;
+  const int k = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const int b = blockIdx.y;
+  if (k >= K) return;
+  const double *bb = bvec + size_t(b) * K;
+  const double *row = Minv + size_t(k) * K;
+  double acc = 0.0;
+  for (int j = lane; j < K; j += 64) acc += row[j] * bb[j];
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if (lane == 0) c[size_t(b) * K + k] = (D ? D[size_t(b) * K + k] : 1.0) * acc;
+}
+
+// y_i = base_i + sum_k c_k Yt[k][i]
+template <typename T, bool SPLIT>
+__global__ __launch_bounds__(kBlock) void k_sh_synth(int K, int N, const double *Yt, const unsigned int *Yp,
+                                                     const double *c, const T *base, long long stride, T *out) {
+  extern __shared__ double cs[];
+  const int b = blockIdx.y;
+  for (int k = threadIdx.x; k < K; k += kBlock) cs[k] = c[size_t(b) * K + k];
+  __syncthreads();
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= N) return;
+  double acc = 0.0;
+  if (SPLIT) {
+    float facc = 0.f;
+    for (int k = 0; k < K; ++k) {
+      const unsigned int u = Yp[size_t(k) * N + i];
+      const float ck = float(cs[k]);
+      facc = fmaf(bf16_hi(u), ck, facc);
+      facc = fmaf(bf16_lo(u), ck, facc);
+    }
+    acc = double(facc);
+  } else {
+    for (int k = 0; k < K; ++k) acc += cs[k] * Yt[size_t(k) * N + i];
+  }
+  const T add = base ? base[b * stride + i] : T(0);
+  out[b * stride + i] = T(double(add) + acc);
+}
+
+// Gram matrix M = Yt Yt^T (K x K), 32 x 32 output tiles, FP64.
+__global__ __launch_bounds__(kBlock) void k_gram(int K, int N, const double *Yt, double *M) {
+  __shared__ double A[32][33], Bm[32][33];
+  const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
+  if (c0 > r0 + 31) return;  // upper triangle filled by symmetry on the host
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  double acc[4] = {0, 0, 0, 0};
+  for (int i0 = 0; i0 < N; i0 += 32) {
+    for (int q = 0; q < 4; q++) {
+      const int rr = ty + 8 * q;
+      const int i = i0 + tx;
+      A[rr][tx] = (r0 + rr < K && i < N) ? Yt[size_t(r0 + rr) * N + i] : 0.0;
+      Bm[rr][tx] = (c0 + rr < K && i < N) ? Yt[size_t(c0 + rr) * N + i] : 0.0;
+    }
+    __syncthreads();
+    for (int j = 0; j < 32; j++) {
+      const double bv = Bm[tx][j];
+      for (int q = 0; q < 4; q++) acc[q] += A[ty + 8 * q][j] * bv;
+    }
+    __syncthreads();
+  }
+  for (int q = 0; q < 4; q++) {
+    const int r = r0 + ty + 8 * q, cc = c0 + tx;
+    if (r < K && cc < K) M[size_t(r) * K + cc] = acc[q];
+  }
+}
+
 }  // namespace odis
 
 // ===========================================================================
@@ -253,6 +362,13 @@ struct odis_ctx {
   int timing = 0;
   std::vector<hipEvent_t> ev;
   double t_face = 0, t_cell = 0, t_step = 0;
+  // spherical-harmonic coupling
+  int sh_mode = ODIS_SH_OFF, sh_L = 0, sh_K = 0;
+  double *sh_Yt = nullptr, *sh_Minv = nullptr, *sh_D = nullptr, *sh_b = nullptr, *sh_c = nullptr;
+  unsigned int *sh_Yp = nullptr;
+  void *eta_eff = nullptr;  // [B][N] of the stencil precision
+  std::vector<void *> sh_allocs;
+  std::vector<double> cell_lat, cell_lon;  // reference order
   // lat-lon interpolation operator (eta columns only, device cell indices)
   int ll_rows = 0;
   int *ll_rp = nullptr, *ll_col = nullptr;
@@ -404,6 +520,7 @@ extern "C" int odis_destroy(odis_ctx *ctx) {
   }
   for (void *p : ctx->allocs) (void)hipFree(p);
   for (void *p : ctx->ll_allocs) (void)hipFree(p);
+  for (void *p : ctx->sh_allocs) (void)hipFree(p);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return ODIS_OK;
@@ -458,6 +575,8 @@ extern "C" int odis_create(const odis_mesh_desc *m, const odis_params *params, i
   ctx->quad_drag = params[0].quad_drag != 0;
   ctx->init_hist_v.assign(n_inst, std::vector<double>(size_t(3) * E, 0.0));
   ctx->init_hist_e.assign(n_inst, std::vector<double>(size_t(3) * N, 0.0));
+  ctx->cell_lat.assign(m->cell_lat, m->cell_lat + N);
+  ctx->cell_lon.assign(m->cell_lon, m->cell_lon + N);
 
   // -- locality renumbering: cells along a 3-D Hilbert curve, faces by the
   //    device ranks of their two cells
@@ -744,7 +863,7 @@ static FaceArgs<T> face_args(odis_ctx *ctx, Ops<T> &o) {
   a.v_cur = o.v[ctx->cur];
   a.v_nxt = o.v[ctx->cur ^ 1];
   a.ring = o.ring_v;
-  a.eta = o.eta;
+  a.eta = ctx->sh_mode != ODIS_SH_OFF ? static_cast<const T *>(ctx->eta_eff) : o.eta;
   a.v_stride = ctx->E;
   a.ring_stride = 3LL * ctx->E;
   a.eta_stride = ctx->N;
@@ -801,6 +920,30 @@ static int stage_coefficients(odis_ctx *ctx, long long n0, int count, const doub
   return ODIS_OK;
 }
 
+// eta_eff = eta + Yt^T D Minv Yt eta for every instance (three kernels)
+template <typename T>
+static int sh_effective_height(odis_ctx *ctx, Ops<T> &o) {
+  const int K = ctx->sh_K, N = ctx->N;
+  const dim3 blk(kBlock);
+  if (ctx->sh_mode == ODIS_SH_SPLIT_BF16)
+    hipLaunchKernelGGL((k_sh_analysis<T, true>), dim3(K, ctx->B), blk, 0, ctx->stream, K, N, ctx->sh_Yt,
+                       ctx->sh_Yp, o.eta, (long long)N, ctx->sh_b);
+  else
+    hipLaunchKernelGGL((k_sh_analysis<T, false>), dim3(K, ctx->B), blk, 0, ctx->stream, K, N, ctx->sh_Yt,
+                       ctx->sh_Yp, o.eta, (long long)N, ctx->sh_b);
+  hipLaunchKernelGGL(k_sh_coef, dim3((K + 3) / 4, ctx->B), blk, 0, ctx->stream, K, ctx->sh_Minv, ctx->sh_D,
+                     ctx->sh_b, ctx->sh_c);
+  const size_t lds = sizeof(double) * K;
+  if (ctx->sh_mode == ODIS_SH_SPLIT_BF16)
+    hipLaunchKernelGGL((k_sh_synth<T, true>), dim3(ctx->nb_cell, ctx->B), blk, lds, ctx->stream, K, N, ctx->sh_Yt,
+                       ctx->sh_Yp, ctx->sh_c, o.eta, (long long)N, static_cast<T *>(ctx->eta_eff));
+  else
+    hipLaunchKernelGGL((k_sh_synth<T, false>), dim3(ctx->nb_cell, ctx->B), blk, lds, ctx->stream, K, N,
+                       ctx->sh_Yt, ctx->sh_Yp, ctx->sh_c, o.eta, (long long)N, static_cast<T *>(ctx->eta_eff));
+  CK(hipGetLastError());
+  return ODIS_OK;
+}
+
 template <typename T>
 static int step_t(odis_ctx *ctx, Ops<T> &o, int n_steps) {
   const dim3 gf(ctx->nb_face, ctx->B), gc(ctx->nb_cell, ctx->B), blk(kBlock);
@@ -827,6 +970,10 @@ static int step_t(odis_ctx *ctx, Ops<T> &o, int n_steps) {
       const double *cs = coef + size_t(s) * ctx->B * 8;
       FaceArgs<T> fa = face_args(ctx, o);
       CellArgs<T> ca = cell_args(ctx, o);
+      if (ctx->sh_mode != ODIS_SH_OFF) {
+        const int rc2 = sh_effective_height(ctx, o);
+        if (rc2) return rc2;
+      }
       if (ctx->timing) {
         // events stamped by the kernel dispatch itself (start / end of the
         // kernel), not by separate marker packets
@@ -1036,5 +1183,191 @@ extern "C" int odis_interp_ll(odis_ctx *ctx, int inst, double *ll) {
   CK(hipGetLastError());
   CK(hipMemcpyAsync(ll, ctx->ll_out, sizeof(double) * ctx->ll_rows, hipMemcpyDeviceToHost, ctx->stream));
   CK(hipStreamSynchronize(ctx->stream));
+  return ODIS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// spherical-harmonic basis set-up and stand-alone projection
+// ---------------------------------------------------------------------------
+
+static uint16_t to_bf16(float f) {
+  // round to nearest even
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  const uint32_t lsb = (u >> 16) & 1u;
+  u += 0x7FFFu + lsb;
+  return uint16_t(u >> 16);
+}
+static float from_bf16(uint16_t h) {
+  const uint32_t u = uint32_t(h) << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+// In-place inverse of a symmetric positive definite matrix (Cholesky).
+static bool spd_inverse(std::vector<double> &A, int n) {
+  std::vector<double> L(size_t(n) * n, 0.0);
+  for (int j = 0; j < n; j++) {
+    double d = A[size_t(j) * n + j];
+    for (int k = 0; k < j; k++) d -= L[size_t(j) * n + k] * L[size_t(j) * n + k];
+    if (!(d > 0)) return false;
+    const double ljj = std::sqrt(d);
+    L[size_t(j) * n + j] = ljj;
+    for (int i = j + 1; i < n; i++) {
+      double s = A[size_t(i) * n + j];
+      for (int k = 0; k < j; k++) s -= L[size_t(i) * n + k] * L[size_t(j) * n + k];
+      L[size_t(i) * n + j] = s / ljj;
+    }
+  }
+  // inverse of L (lower), then A^-1 = L^-T L^-1
+  std::vector<double> Li(size_t(n) * n, 0.0);
+  for (int j = 0; j < n; j++) {
+    Li[size_t(j) * n + j] = 1.0 / L[size_t(j) * n + j];
+    for (int i = j + 1; i < n; i++) {
+      double s = 0.0;
+      for (int k = j; k < i; k++) s -= L[size_t(i) * n + k] * Li[size_t(k) * n + j];
+      Li[size_t(i) * n + j] = s / L[size_t(i) * n + i];
+    }
+  }
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j <= i; j++) {
+      double s = 0.0;
+      for (int k = i; k < n; k++) s += Li[size_t(k) * n + i] * Li[size_t(k) * n + j];
+      A[size_t(i) * n + j] = s;
+      A[size_t(j) * n + i] = s;
+    }
+  return true;
+}
+
+extern "C" int odis_sh_set_basis(odis_ctx *ctx, int l_max, const double *beta, int mode) {
+  if (!ctx) return ODIS_E_INVALID;
+  if (mode != ODIS_SH_OFF && mode != ODIS_SH_FP64 && mode != ODIS_SH_SPLIT_BF16)
+    return fail(ctx, ODIS_E_INVALID, "bad sh mode");
+  CK(hipStreamSynchronize(ctx->stream));
+  for (void *p : ctx->sh_allocs) (void)hipFree(p);
+  ctx->sh_allocs.clear();
+  ctx->sh_mode = ODIS_SH_OFF;
+  ctx->sh_L = ctx->sh_K = 0;
+  if (mode == ODIS_SH_OFF) return ODIS_OK;
+  if (l_max < 0 || l_max > 100) return fail(ctx, ODIS_E_INVALID, "l_max out of range");
+  const int K = (l_max + 1) * (l_max + 1), N = ctx->N;
+  if (K > N) return fail(ctx, ODIS_E_INVALID, "(l_max+1)^2 exceeds the number of cells");
+  if (size_t(K) * sizeof(double) > 64 * 1024) return fail(ctx, ODIS_E_INVALID, "l_max too large for LDS");
+  // basis on the host in reference order, then columns in device order
+  std::vector<double> Yref;
+  odis::sh_basis(l_max, ctx->cell_lat, ctx->cell_lon, Yref);
+  std::vector<double> Y(size_t(K) * N);
+  for (int k = 0; k < K; k++)
+    for (int r = 0; r < N; r++) Y[size_t(k) * N + r] = Yref[size_t(k) * N + ctx->cell_inv[r]];
+  std::vector<double>().swap(Yref);
+  auto alloc = [&](void **p, size_t bytes) -> int {
+    CK(hipMalloc(p, std::max<size_t>(bytes, 8)));
+    ctx->sh_allocs.push_back(*p);
+    return ODIS_OK;
+  };
+  int rc;
+  if ((rc = alloc(reinterpret_cast<void **>(&ctx->sh_Yt), Y.size() * sizeof(double)))) return rc;
+  CK(hipMemcpy(ctx->sh_Yt, Y.data(), Y.size() * sizeof(double), hipMemcpyHostToDevice));
+  // Gram matrix on the GPU, inverse on the host
+  double *dM = nullptr;
+  if ((rc = alloc(reinterpret_cast<void **>(&dM), size_t(K) * K * sizeof(double)))) return rc;
+  const int tiles = (K + 31) / 32;
+  hipLaunchKernelGGL(k_gram, dim3(tiles, tiles), dim3(kBlock), 0, ctx->stream, K, N, ctx->sh_Yt, dM);
+  CK(hipGetLastError());
+  std::vector<double> M(size_t(K) * K);
+  CK(hipStreamSynchronize(ctx->stream));
+  CK(hipMemcpy(M.data(), dM, M.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int r = 0; r < K; r++)
+    for (int c = r + 1; c < K; c++) M[size_t(r) * K + c] = M[size_t(c) * K + r];
+  if (!spd_inverse(M, K)) return fail(ctx, ODIS_E_INVALID, "basis Gram matrix is singular");
+  CK(hipMemcpy(dM, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice));
+  ctx->sh_Minv = dM;
+  if (mode == ODIS_SH_SPLIT_BF16) {
+    std::vector<unsigned int> P(Y.size());
+    for (size_t q = 0; q < Y.size(); q++) {
+      const float f = float(Y[q]);
+      const uint16_t hi = to_bf16(f);
+      const uint16_t lo = to_bf16(float(Y[q] - double(from_bf16(hi))));
+      P[q] = (unsigned(hi) << 16) | unsigned(lo);
+    }
+    if ((rc = alloc(reinterpret_cast<void **>(&ctx->sh_Yp), P.size() * sizeof(unsigned int)))) return rc;
+    CK(hipMemcpy(ctx->sh_Yp, P.data(), P.size() * sizeof(unsigned int), hipMemcpyHostToDevice));
+  }
+  // per-instance degree factors (beta_l - 1 for 2 <= l <= L, 0 otherwise)
+  std::vector<double> D(size_t(ctx->B) * K, 0.0);
+  if (beta)
+    for (int b = 0; b < ctx->B; b++)
+      for (int l = 2; l <= l_max; l++)
+        for (int m = -l; m <= l; m++) D[size_t(b) * K + l * l + l + m] = beta[size_t(b) * (l_max + 1) + l] - 1.0;
+  if ((rc = alloc(reinterpret_cast<void **>(&ctx->sh_D), D.size() * sizeof(double)))) return rc;
+  CK(hipMemcpy(ctx->sh_D, D.data(), D.size() * sizeof(double), hipMemcpyHostToDevice));
+  if ((rc = alloc(reinterpret_cast<void **>(&ctx->sh_b), size_t(ctx->B) * K * sizeof(double)))) return rc;
+  if ((rc = alloc(reinterpret_cast<void **>(&ctx->sh_c), size_t(ctx->B) * K * sizeof(double)))) return rc;
+  const size_t tsz = ctx->precision == ODIS_FP64 ? sizeof(double) : sizeof(float);
+  if ((rc = alloc(&ctx->eta_eff, size_t(ctx->B) * N * tsz))) return rc;
+  ctx->sh_L = l_max;
+  ctx->sh_K = K;
+  ctx->sh_mode = mode;
+  return ODIS_OK;
+}
+
+// field (reference order) -> LSQ coefficients with the context's basis
+extern "C" int odis_sh_project(odis_ctx *ctx, const double *field, double *coeffs) {
+  if (!ctx || !field || !coeffs) return ODIS_E_INVALID;
+  if (ctx->sh_mode == ODIS_SH_OFF) return fail(ctx, ODIS_E_INVALID, "no spherical-harmonic basis set");
+  const int K = ctx->sh_K, N = ctx->N;
+  std::vector<double> x(N);
+  for (int r = 0; r < N; r++) x[r] = field[ctx->cell_inv[r]];
+  double *dx = nullptr, *db = nullptr, *dc = nullptr;
+  CK(hipMalloc(&dx, sizeof(double) * N));
+  hipError_t e = hipMalloc(&db, sizeof(double) * K);
+  if (e == hipSuccess) e = hipMalloc(&dc, sizeof(double) * K);
+  if (e == hipSuccess) e = hipMemcpy(dx, x.data(), sizeof(double) * N, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    if (ctx->sh_mode == ODIS_SH_SPLIT_BF16)
+      hipLaunchKernelGGL((k_sh_analysis<double, true>), dim3(K, 1), dim3(kBlock), 0, ctx->stream, K, N, ctx->sh_Yt,
+                         ctx->sh_Yp, dx, (long long)N, db);
+    else
+      hipLaunchKernelGGL((k_sh_analysis<double, false>), dim3(K, 1), dim3(kBlock), 0, ctx->stream, K, N,
+                         ctx->sh_Yt, ctx->sh_Yp, dx, (long long)N, db);
+    hipLaunchKernelGGL(k_sh_coef, dim3((K + 3) / 4, 1), dim3(kBlock), 0, ctx->stream, K, ctx->sh_Minv,
+                       static_cast<const double *>(nullptr), db, dc);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) e = hipMemcpy(coeffs, dc, sizeof(double) * K, hipMemcpyDeviceToHost);
+  (void)hipFree(dx);
+  (void)hipFree(db);
+  (void)hipFree(dc);
+  if (e != hipSuccess) return fail(ctx, ODIS_E_HIP, hipGetErrorString(e));
+  return ODIS_OK;
+}
+
+extern "C" int odis_sh_synthesize(odis_ctx *ctx, const double *coeffs, double *field) {
+  if (!ctx || !field || !coeffs) return ODIS_E_INVALID;
+  if (ctx->sh_mode == ODIS_SH_OFF) return fail(ctx, ODIS_E_INVALID, "no spherical-harmonic basis set");
+  const int K = ctx->sh_K, N = ctx->N;
+  double *dy = nullptr, *dc = nullptr;
+  CK(hipMalloc(&dy, sizeof(double) * N));
+  hipError_t e = hipMalloc(&dc, sizeof(double) * K);
+  if (e == hipSuccess) e = hipMemcpy(dc, coeffs, sizeof(double) * K, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    const size_t lds = sizeof(double) * K;
+    if (ctx->sh_mode == ODIS_SH_SPLIT_BF16)
+      hipLaunchKernelGGL((k_sh_synth<double, true>), dim3(ctx->nb_cell, 1), dim3(kBlock), lds, ctx->stream, K, N,
+                         ctx->sh_Yt, ctx->sh_Yp, dc, static_cast<const double *>(nullptr), (long long)N, dy);
+    else
+      hipLaunchKernelGGL((k_sh_synth<double, false>), dim3(ctx->nb_cell, 1), dim3(kBlock), lds, ctx->stream, K,
+                         N, ctx->sh_Yt, ctx->sh_Yp, dc, static_cast<const double *>(nullptr), (long long)N, dy);
+    e = hipGetLastError();
+  }
+  std::vector<double> y(N);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) e = hipMemcpy(y.data(), dy, sizeof(double) * N, hipMemcpyDeviceToHost);
+  (void)hipFree(dy);
+  (void)hipFree(dc);
+  if (e != hipSuccess) return fail(ctx, ODIS_E_HIP, hipGetErrorString(e));
+  for (int r = 0; r < N; r++) field[ctx->cell_inv[r]] = y[r];
   return ODIS_OK;
 }

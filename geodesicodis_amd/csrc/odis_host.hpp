@@ -103,4 +103,9 @@ void quantise_dt(double period, double target, double *dt, int *n_steps);
 void membrane_nu_beta(double radius, double hs, double ho, int l_max, std::vector<double> &nu,
                       std::vector<double> &beta, double *radius_ocean, double *grav_ocean);
 
+// Real 4pi-normalised spherical harmonics, k = l*l + l + m (odis_sh.cpp).
+void sh_basis_row(int L, double lat, double lon, double *Y);
+void sh_basis(int L, const std::vector<double> &lat, const std::vector<double> &lon, std::vector<double> &Yt);
+std::vector<double> surface_beta(int surface, int L, double radius, double g, double shell, double ocean);
+
 }  // namespace odis

@@ -29,10 +29,10 @@
 #include <hip/hip_runtime.h>
 
 #include "odis.h"
+#include "odis_host.hpp"
 
 namespace odis {
 
-constexpr double kPi = 3.1415926535897932384626433832795028841971693993751058;
 constexpr int kMaxForcing = 7;
 constexpr int kBlock = 256;
 

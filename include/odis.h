@@ -150,6 +150,26 @@ int odis_interp_ll_set(odis_ctx *ctx, int n_rows, int n_cols, const int *row_ptr
                        const int *col, const double *val);
 int odis_interp_ll(odis_ctx *ctx, int inst, double *ll);
 
+/* Self-gravity / ice-shell coupling through a spherical-harmonic projection
+ * of eta (the reference's commented design: CalcLegendreFuncs,
+ * src/mesh.cpp:2154-2259, and pressureGradientSH, src/spatialOperators.cpp:
+ * 387-460).  Each step the pressure gradient acts on
+ *     eta_eff = eta + sum_{l=2..L} sum_m (beta_l - 1) c_lm Y_lm,
+ * c = least-squares coefficients of eta on the nodes up to degree L
+ * (as SHExpandLSQ, src/extractSHCoeffGG.f95:29), Y_lm real 4pi-normalised
+ * harmonics, index k = l*l + l + m.  beta holds n_inst rows of (l_max + 1)
+ * per-degree factors (e.g. membraneNuBeta's beta_l for LID_MEMBR).
+ * mode: ODIS_SH_OFF, ODIS_SH_FP64 (FP64 basis, FP64 sums) or
+ * ODIS_SH_SPLIT_BF16 (basis stored as bf16 hi + bf16 lo, FP32 accumulation). */
+#define ODIS_SH_OFF 0
+#define ODIS_SH_FP64 1
+#define ODIS_SH_SPLIT_BF16 2
+int odis_sh_set_basis(odis_ctx *ctx, int l_max, const double *beta, int mode);
+/* Coefficients (K = (l_max+1)^2) of a cell field in reference numbering, and
+ * the synthesis of a field from coefficients, with the context's basis. */
+int odis_sh_project(odis_ctx *ctx, const double *field, double *coeffs);
+int odis_sh_synthesize(odis_ctx *ctx, const double *coeffs, double *field);
+
 /* Async-signal-safe stop request (replaces CatchExit, src/timeIntegrator.cpp:34-37):
  * odis_step returns ODIS_E_STOPPED at the next step boundary. */
 int odis_request_stop(odis_ctx *ctx);
@@ -200,6 +220,12 @@ int odis_quantise_dt(double period, double target_dt, double *dt, int *n_steps);
  * optional outputs may be null. */
 int odis_config_read(const char *case_dir, odis_params *params, int *steps_per_orbit,
                      double *end_time_orbits, int *grid_level, int *output_time);
+/* Per-degree pressure factors beta_l, l = 0..l_max, of a surface type:
+ * LID_MEMBR -> membraneNuBeta (src/membraneConstants.cpp:8-144) with the
+ * top radius, shell and ocean thicknesses; FREE_LOADING -> 1 - loading factor
+ * (src/boundaryConditions.cpp:29-77) with gravity g; otherwise 1. */
+int odis_surface_beta(int surface, int l_max, double top_radius, double g, double shell_thickness,
+                      double ocean_thickness, double *beta);
 /* Last error of a host entry point (grid, mesh, config). */
 const char *odis_host_last_error(void);
 

@@ -66,6 +66,7 @@ def lib():
         L.orc_sim_forcing.argtypes = [vp, ctypes.c_double, dp]
         L.orc_sim_interp_velocity.argtypes = [vp, dp, dp]
         L.orc_sim_momentum.argtypes = [vp, dp, dp, dp]
+        L.orc_sim_set_projector.argtypes = [vp, ctypes.c_int, dp, dp]
         L.orc_round_period.argtypes = [ctypes.c_double, dp, dp]
         L.orc_quantise_dt.argtypes = [ctypes.c_double, ctypes.c_double, dp, ip]
         L.orc_analytical_obliq_west.argtypes = [ctypes.POINTER(OrcParams), ctypes.c_double,
@@ -204,6 +205,13 @@ class Sim:
         lib().orc_sim_interp_velocity(self.h, _dp(v), _dp(uv))
         return uv.reshape(-1, 2)
 
+    def set_projector(self, Pb, Pc):
+        """Pressure term on eta + Pb @ (Pc @ eta); Pb (N, K), Pc (K, N)."""
+        Pb = np.ascontiguousarray(Pb, dtype=np.float64)
+        Pc = np.ascontiguousarray(Pc, dtype=np.float64)
+        self._proj = (Pb, Pc)
+        lib().orc_sim_set_projector(self.h, Pb.shape[1], _dp(Pb), _dp(Pc))
+
     def momentum(self, v, eta):
         v = np.ascontiguousarray(v, dtype=np.float64)
         eta = np.ascontiguousarray(eta, dtype=np.float64)
@@ -273,3 +281,21 @@ def read_reference_grid(path):
         for j, (x, y) in enumerate(cl):
             clat[i, j], clon[i, j] = x, y
     return dict(lat=lat, lon=lon, friends=friends, clat=clat, clon=clon)
+
+
+def real_sh_basis(L, lat, lon):
+    """Real 4pi-normalised spherical harmonics on points, index l*l + l + m,
+    from scipy's associated Legendre functions (independent of the product's
+    recurrence): m >= 0 -> P_lm cos(m lon), m < 0 -> P_l|m| sin(|m| lon),
+    no Condon-Shortley phase."""
+    from scipy.special import gammaln, lpmv
+    x = np.sin(lat)
+    Y = np.zeros((len(lat), (L + 1) ** 2))
+    for l in range(L + 1):
+        for m in range(l + 1):
+            norm = np.sqrt((2.0 - (m == 0)) * (2 * l + 1) * np.exp(gammaln(l - m + 1) - gammaln(l + m + 1)))
+            p = norm * (-1.0) ** m * lpmv(m, l, x)
+            Y[:, l * l + l + m] = p * np.cos(m * lon)
+            if m > 0:
+                Y[:, l * l + l - m] = p * np.sin(m * lon)
+    return Y

@@ -634,6 +634,9 @@ typedef struct {
     double *v, *dv, *eta, *de;  /* v[E], dv[E*3], eta[N], de[N*3] */
     double *tmpE, *tmpE2, *U, *uv, *tmpN;
     int iter;
+    /* optional pressure-height correction eta_eff = eta + Pb (Pc eta) */
+    int pk;
+    double *Pb, *Pc, *pcoef, *eta_eff;
 } sim_t;
 
 static void csr_alloc(csr_t *A, int rows, int nnz)
@@ -689,6 +692,7 @@ void orc_sim_free(void *p)
     for (int i = 0; i < 3; i++) { free(cs[i]->rp); free(cs[i]->col); free(cs[i]->val); }
     free(S->v); free(S->dv); free(S->eta); free(S->de);
     free(S->tmpE); free(S->tmpE2); free(S->U); free(S->uv); free(S->tmpN);
+    free(S->Pb); free(S->Pc); free(S->pcoef); free(S->eta_eff);
     free(S);
 }
 
@@ -851,11 +855,52 @@ double orc_sim_dissipation(void *p)
     return energy(S, S->uv);
 }
 
+/* Self-gravity / shell coupling of the pressure term (the reference's
+ * commented pressureGradientSH design, src/spatialOperators.cpp:387-460):
+ * eta_eff = eta + Pb (Pc eta) with Pb [N x K] the beta-weighted synthesis
+ * basis and Pc [K x N] the least-squares analysis operator, both supplied by
+ * the caller.  Plain row-by-row dense products. */
+void orc_sim_set_projector(void *p, int K, const double *Pb, const double *Pc)
+{
+    sim_t *S = p;
+    int N = S->M->N;
+    free(S->Pb); free(S->Pc); free(S->pcoef); free(S->eta_eff);
+    S->pk = K;
+    S->Pb = NULL; S->Pc = NULL; S->pcoef = NULL; S->eta_eff = NULL;
+    if (K <= 0) return;
+    S->Pb = xcalloc((size_t)N * K, sizeof(double));
+    S->Pc = xcalloc((size_t)K * N, sizeof(double));
+    S->pcoef = xcalloc(K, sizeof(double));
+    S->eta_eff = xcalloc(N, sizeof(double));
+    memcpy(S->Pb, Pb, sizeof(double) * N * K);
+    memcpy(S->Pc, Pc, sizeof(double) * K * N);
+}
+
+static const double *effective_height(sim_t *S, const double *eta)
+{
+    if (S->pk <= 0) return eta;
+    int N = S->M->N, K = S->pk;
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < K; k++) {
+        double t = 0.0;
+        for (int i = 0; i < N; i++) t += S->Pc[(size_t)k * N + i] * eta[i];
+        S->pcoef[k] = t;
+    }
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; i++) {
+        double t = 0.0;
+        for (int k = 0; k < K; k++) t += S->Pb[(size_t)i * K + k] * S->pcoef[k];
+        S->eta_eff[i] = eta[i] + t;
+    }
+    return S->eta_eff;
+}
+
 /* updateMomentum linear branch, src/updateMomentum.cpp:42 */
 void orc_sim_momentum(void *p, const double *v, const double *eta, double *dvdt)
 {
     sim_t *S = p;
     int E = S->M->E;
+    eta = effective_height(S, eta);
     spmv(&S->G, -S->p.g, eta, S->tmpE2);
     spmv(&S->C, 1.0, v, dvdt);
     for (int e = 0; e < E; e++) dvdt[e] = S->tmpE2[e] + dvdt[e];

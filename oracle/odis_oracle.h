@@ -75,6 +75,8 @@ double orc_sim_dissipation(void *sim);
 void  orc_sim_forcing(void *sim, double time, double *potential);
 void  orc_sim_interp_velocity(void *sim, const double *v, double *uv);
 void  orc_sim_momentum(void *sim, const double *v, const double *eta, double *dvdt);
+/* pressure term acts on eta + Pb (Pc eta); Pb [N][K], Pc [K][N]; K = 0 clears */
+void  orc_sim_set_projector(void *sim, int K, const double *Pb, const double *Pc);
 
 /* ---- scalar helpers ------------------------------------------------------ */
 void  orc_round_period(double ang_vel, double *period, double *omega);

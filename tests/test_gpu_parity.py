@@ -279,3 +279,72 @@ def test_interp_gg2ll(product, oracle, precision):
         assert rel_l2(ll, ref) < (1e-14 if precision == 0 else 1e-6)
     with pytest.raises(G.OdisError):
         ctx.set_interp_ll(rp, col, val, n_cols=3 * mesh.N + 3)
+
+
+# ---------------------------------------------------------------------------
+# spherical-harmonic self-gravity / ice-shell coupling
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("mode,tol", [(1, 1e-11), (2, 2e-6)])
+def test_sh_synthesis_and_projection(product, oracle, mode, tol):
+    """Synthesis of unit coefficients reproduces scipy's harmonics on the
+    nodes; the GPU least-squares projection recovers band-limited random
+    coefficients (FP64: rounding level; split-BF16: ~1e-6)."""
+    G, O = product, oracle
+    g, arrs = generated_grid(5)
+    p, _ = G.make_params()
+    mesh = G.Mesh(g, p.radius)
+    ctx = G.Context(mesh, p)
+    L = 12
+    ctx.set_sh(L, None, mode=mode)
+    Y = O.real_sh_basis(L, arrs["lat"], arrs["lon"])
+    K = (L + 1) ** 2
+    for k in (0, 5, 40, K - 1):
+        e = np.zeros(K)
+        e[k] = 1.0
+        assert np.max(np.abs(ctx.sh_synthesize(e) - Y[:, k])) < (1e-12 if mode == 1 else 1e-5)
+    c_true = np.random.default_rng(11).standard_normal(K)
+    c = ctx.sh_project(Y @ c_true)
+    assert rel_l2(c, c_true) < tol
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_self_gravity_shell_step_parity(product, oracle, precision):
+    """Ice shell under LID_MEMBR factors up to degree 8: the pressure term of
+    the GPU step acts on eta + sum (beta_l - 1) c_lm Y_lm; the oracle gets the
+    same term from numpy (pinv of scipy's harmonics).  FP64 to 1e-12 per
+    step; FP32 stencils + FP64 basis reported and bounded."""
+    G, O = product, oracle
+    g, arrs = generated_grid(5)
+    L = 8
+    beta = G.surface_beta(G.LID_MEMBR, L, 252.1e3, 0.113, 20e3, 38e3)
+    assert beta[2] > 10 and np.all(np.diff(beta[2:]) > 0)
+    p, _ = G.make_params(time_step=10.0, radius=232.1e3, g=0.1166, surface=G.LID_MEMBR,
+                         shell_thickness=20e3, love_reduct=0.99)
+    mesh = G.Mesh(g, p.radius)
+    ctx = G.Context(mesh, p, precision=precision)
+    ctx.set_sh(L, beta, mode=G.SH_FP64)
+    Y = O.real_sh_basis(L, arrs["lat"], arrs["lon"])
+    dfac = np.zeros((L + 1) ** 2)
+    for l in range(2, L + 1):
+        dfac[l * l:(l + 1) ** 2] = beta[l] - 1.0
+    om = O.Mesh(arrs, p.radius)
+    sim = O.Sim(om, O.make_params(g=p.g, h=p.h, alpha=p.alpha, omega=p.omega, radius=p.radius,
+                                  love_reduct=p.love_reduct, ecc=p.ecc, dt=p.dt, tide=p.tide,
+                                  surface=p.surface, shell_thickness=p.shell_thickness))
+    sim.set_projector(Y * dfac[None, :], np.linalg.pinv(Y))
+    for n in (1, 2, 200):
+        ctx.step(n)
+        sim.step(n)
+        if precision == 0:
+            _compare(ctx, sim, TOL_SHORT)
+    a, b = ctx.get_state(), sim.get_state()
+    err = rel_l2(a["eta"], b["eta"])
+    if precision == 1:
+        print("fp32 + sh: eta rel err %.2e" % err)
+        assert err < 1e-4
+    # the coupling matters: without it the trajectory differs at O(1)
+    sim.set_projector(np.zeros((mesh.N, 1)), np.zeros((1, mesh.N)))
+    sim.step(100)
+    ctx.step(100)
+    assert rel_l2(ctx.get_state()["eta"], sim.get_state()["eta"]) > 1e-3

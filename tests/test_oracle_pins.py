@@ -450,3 +450,43 @@ def test_reference_ll_weights_layout(res):
         x = np.zeros(3 * 642)
         x[k::3] = 1.0
         assert np.median(np.abs(W @ x)) < 0.1
+
+
+# ---------------------------------------------------------------------------
+# spherical-harmonic basis of the self-gravity / shell coupling
+# ---------------------------------------------------------------------------
+
+def test_real_sh_basis_orthonormal_and_closed_forms(oracle):
+    """(1/4 pi) int Y_k Y_k' = delta on a Gauss-Legendre x uniform-longitude
+    rule, and the l <= 2 functions equal their closed forms."""
+    L = 8
+    xg, wg = np.polynomial.legendre.leggauss(L + 2)
+    nlon = 2 * L + 3
+    lat = np.repeat(np.arcsin(xg), nlon)
+    lon = np.tile(2 * np.pi * np.arange(nlon) / nlon, len(xg))
+    w = np.repeat(wg, nlon) * 2 * np.pi / nlon
+    Y = oracle.real_sh_basis(L, lat, lon)
+    G = (Y * w[:, None]).T @ Y / (4 * np.pi)
+    assert np.max(np.abs(G - np.eye((L + 1) ** 2))) < 1e-12
+    s, c = np.sin(lat), np.cos(lat)
+    closed = {0: np.ones_like(lat), 2: np.sqrt(3) * s, 3: np.sqrt(3) * c * np.cos(lon),
+              1: np.sqrt(3) * c * np.sin(lon), 6: np.sqrt(5) * (1.5 * s * s - 0.5),
+              7: np.sqrt(15) * s * c * np.cos(lon), 8: np.sqrt(15) / 2 * c * c * np.cos(2 * lon),
+              4: np.sqrt(15) / 2 * c * c * np.sin(2 * lon)}
+    for k, f in closed.items():
+        assert np.max(np.abs(Y[:, k] - f)) < 1e-13, k
+
+
+def test_sh_basis_is_laplacian_eigenfunction(oracle):
+    """The discrete Div G of Y_lm approaches -l(l+1) Y_lm (unit sphere)."""
+    errs = []
+    for level in (4, 5, 6):
+        m, arrs = _omesh(oracle, level, 1.0)
+        rp, col, val = m.grad_csr()
+        Gm = sp.csr_matrix((val, col, rp), shape=(m.E, m.N))
+        rp, col, val = m.div_csr()
+        D = sp.csr_matrix((val, col, rp), shape=(m.N, m.E))
+        Y = oracle.real_sh_basis(3, arrs["lat"], arrs["lon"])
+        k = 3 * 3 + 3 + 2            # l = 3, m = 2
+        errs.append(rel_l2(-(D @ (Gm @ Y[:, k])), -12.0 * Y[:, k]))
+    assert errs[-1] < 1e-2 and errs == sorted(errs, reverse=True), errs
