@@ -26,7 +26,7 @@ ERRORS = {0: "OK", -1: "ODIS_E_INVALID", -2: "ODIS_E_HIP", -3: "ODIS_E_NONFINITE
           -4: "ODIS_E_UNSUPPORTED", -5: "ODIS_E_NOMEM", -6: "ODIS_E_RANGE", -7: "ODIS_E_STOPPED",
           -8: "ODIS_E_IO"}
 
-# Symbols declared in include/odis.h (kept in sync by tests/test_capi.py).
+# Symbols declared in include/odis.h (kept in sync by tests/test_host_and_abi.py).
 EXPORTS = [
     "odis_create", "odis_destroy", "odis_last_error", "odis_set_state", "odis_get_state",
     "odis_step", "odis_sync", "odis_get_dissipation", "odis_get_dissipation_series",
@@ -36,7 +36,7 @@ EXPORTS = [
     "odis_mesh_sizes", "odis_mesh_describe", "odis_mesh_get", "odis_mesh_free",
     "odis_round_period", "odis_quantise_dt", "odis_config_read", "odis_host_last_error",
     "odis_run_case", "odis_interp_ll_set", "odis_interp_ll", "odis_sh_set_basis",
-    "odis_sh_project", "odis_sh_synthesize", "odis_surface_beta",
+    "odis_sh_project", "odis_sh_synthesize", "odis_surface_beta", "odis_use_graphs",
 ]
 SH_OFF, SH_FP64, SH_SPLIT_BF16 = 0, 1, 2
 
@@ -99,6 +99,7 @@ def lib():
             "odis_get_velocity_en": ([vp, ctypes.c_int, dp, dp], ctypes.c_int),
             "odis_request_stop": ([vp], ctypes.c_int),
             "odis_enable_timing": ([vp, ctypes.c_int], ctypes.c_int),
+            "odis_use_graphs": ([vp, ctypes.c_int], ctypes.c_int),
             "odis_get_timing": ([vp, dp, dp, dp], ctypes.c_int),
             "odis_grid_read": ([ctypes.c_char_p, vpp], ctypes.c_int),
             "odis_grid_generate": ([ctypes.c_int, ctypes.c_int, vpp], ctypes.c_int),
@@ -380,8 +381,12 @@ class Context:
     def request_stop(self):
         self._check(lib().odis_request_stop(self._h))
 
-    def enable_timing(self, on=True):
-        self._check(lib().odis_enable_timing(self._h, 1 if on else 0))
+    def enable_timing(self, stride=1):
+        """Sample every stride-th chunk of steps with HIP events (0 = off)."""
+        self._check(lib().odis_enable_timing(self._h, int(stride)))
+
+    def use_graphs(self, on=True):
+        self._check(lib().odis_use_graphs(self._h, 1 if on else 0))
 
     def timing(self):
         a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()

@@ -36,9 +36,12 @@ __device__ __forceinline__ double block_sum(double x, double *sh) {
 
 // Face kernel.  DISS_ONLY: evaluate the dissipation partials of the current
 // state without stepping (output cadence / final state).
+// The step index and forcing coefficients come from a host-filled table:
+// table[0] = n0, then per step s and instance b 8 coefficients at
+// table[8 + (s * B + b) * 8]; step s of the table is n = n0 + s.  Kernels
+// captured in a graph therefore stay valid for every chunk of steps.
 template <typename T, bool DISS_ONLY>
-__global__ __launch_bounds__(kBlock) void k_face(FaceArgs<T> a, int energy_quad, long long n,
-                                                 const double *coef) {
+__global__ __launch_bounds__(kBlock) void k_face(FaceArgs<T> a, int energy_quad, const double *table, int s) {
   __shared__ double sh[4];
   const int b = blockIdx.y;
   const int tile = xcd_tile(blockIdx.x, a.nb);
@@ -56,6 +59,8 @@ __global__ __launch_bounds__(kBlock) void k_face(FaceArgs<T> a, int energy_quad,
     const T en = energy_quad ? sqrt(sp) * sp : sp;
     local = double(a.farea[e]) * double(en);
     if (!DISS_ONLY) {
+      const long long n = (long long)table[0] + s;
+      const double *coef = table + 8 + size_t(s) * gridDim.y * 8;
       const T g = T(ip[0]), alpha = T(ip[1]), dt = T(ip[2]), qd = T(ip[4]);
       const bool ab3 = (n >= 2) || (ip[3] != 0.0);
       const T *eta = a.eta + b * a.eta_stride;
@@ -76,8 +81,8 @@ __global__ __launch_bounds__(kBlock) void k_face(FaceArgs<T> a, int energy_quad,
       a.v_nxt[b * a.v_stride + e] = (ve + upd) + dt * drag;
     }
   }
-  const double s = block_sum<T>(local, sh);
-  if (threadIdx.x == 0) a.partials[b * a.nb + tile] = s;
+  const double part = block_sum<T>(local, sh);
+  if (threadIdx.x == 0) a.partials[b * a.nb + tile] = part;
 }
 
 // Sum the face kernel's partials of instance b in a fixed order.
@@ -92,8 +97,9 @@ __device__ __forceinline__ double sum_partials(const double *p, int count, doubl
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cell(CellArgs<T> a, long long n) {
+__global__ __launch_bounds__(kBlock) void k_cell(CellArgs<T> a, const double *table, int s_off) {
   __shared__ double sh[4];
+  const long long n = (long long)table[0] + s_off;
   const int b = blockIdx.y;
   const int tile = xcd_tile(blockIdx.x, a.nb);
   const int N = a.N;
@@ -353,7 +359,10 @@ struct odis_ctx {
   double *coef_dev[2] = {nullptr, nullptr};
   double *coef_host[2] = {nullptr, nullptr};
   hipEvent_t coef_ev[2] = {nullptr, nullptr};
-  int coef_next = 0;
+  int coef_next = 0, coef_last = 0;
+  // graphs of graph_steps steps, one per (coefficient buffer, v parity)
+  int graph_steps = 0, use_graphs = 1;
+  hipGraphExec_t graphs[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   std::vector<TideDev> tides;
   std::vector<odis_params> params;
   std::vector<std::vector<double>> init_hist_v, init_hist_e;  // history as last set ([M][3])
@@ -514,6 +523,9 @@ extern "C" int odis_destroy(odis_ctx *ctx) {
   if (!ctx) return ODIS_E_INVALID;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto &e : ctx->ev) (void)hipEventDestroy(e);
+  for (int k = 0; k < 2; k++)
+    for (int c = 0; c < 2; c++)
+      if (ctx->graphs[k][c]) (void)hipGraphExecDestroy(ctx->graphs[k][c]);
   for (int k = 0; k < 2; k++) {
     if (ctx->coef_ev[k]) (void)hipEventDestroy(ctx->coef_ev[k]);
     if (ctx->coef_host[k]) (void)hipHostFree(ctx->coef_host[k]);
@@ -696,8 +708,10 @@ extern "C" int odis_create(const odis_mesh_desc *m, const odis_params *params, i
   if (!rc) rc = dev_alloc(ctx, &ctx->nonfinite, size_t(n_inst));
   if (!rc) rc = dev_alloc(ctx, &ctx->scratch_out, size_t(n_inst));
   // coefficient tables: chunk steps x B x 8 doubles, <= 4 MiB per buffer
-  ctx->chunk = int(std::max<long long>(1, std::min<long long>(4096, (4LL << 20) / (8LL * 8 * n_inst))));
-  const size_t tab = size_t(ctx->chunk) * n_inst * 8;
+  ctx->chunk = int(std::max<long long>(2, std::min<long long>(4096, (4LL << 20) / (8LL * 8 * n_inst))));
+  ctx->chunk &= ~1;
+  ctx->graph_steps = std::min(64, ctx->chunk);
+  const size_t tab = 8 + size_t(ctx->chunk) * n_inst * 8;
   for (int k = 0; k < 2 && !rc; k++) {
     rc = dev_alloc(ctx, &ctx->coef_dev[k], tab);
     if (!rc) {
@@ -905,18 +919,21 @@ static int stage_coefficients(odis_ctx *ctx, long long n0, int count, const doub
   CK(hipEventSynchronize(ctx->coef_ev[k]));
   double *h = ctx->coef_host[k];
   const int B = ctx->B;
+  for (int q = 0; q < 8; q++) h[q] = 0.0;
+  h[0] = double(n0);  // exact for n0 < 2^53
   for (int s = 0; s < count; s++)
     for (int b = 0; b < B; b++) {
-      double *c = h + (size_t(s) * B + b) * 8;
+      double *c = h + 8 + (size_t(s) * B + b) * 8;
       for (int q = 0; q < 8; q++) c[q] = 0.0;
       const double dt = ctx->params[b].dt;
       // forcing at current_time + dt, current_time = dt * iter (src/timeIntegrator.cpp:218)
       tide_coeffs(ctx->tides[b], dt * double(n0 + s) + dt, c);
     }
-  CK(hipMemcpyAsync(ctx->coef_dev[k], h, sizeof(double) * size_t(count) * B * 8, hipMemcpyHostToDevice,
+  CK(hipMemcpyAsync(ctx->coef_dev[k], h, sizeof(double) * (8 + size_t(count) * B * 8), hipMemcpyHostToDevice,
                     ctx->stream));
   CK(hipEventRecord(ctx->coef_ev[k], ctx->stream));
   *dev = ctx->coef_dev[k];
+  ctx->coef_last = k;
   return ODIS_OK;
 }
 
@@ -944,67 +961,122 @@ static int sh_effective_height(odis_ctx *ctx, Ops<T> &o) {
   return ODIS_OK;
 }
 
+// Enqueue the kernels of one step (s = index within the staged table).
+template <typename T>
+static int launch_step(odis_ctx *ctx, Ops<T> &o, const double *table, int s, hipEvent_t *ev) {
+  const dim3 gf(ctx->nb_face, ctx->B), gc(ctx->nb_cell, ctx->B), blk(kBlock);
+  if (ctx->sh_mode != ODIS_SH_OFF) {
+    const int rc = sh_effective_height(ctx, o);
+    if (rc) return rc;
+  }
+  FaceArgs<T> fa = face_args(ctx, o);
+  CellArgs<T> ca = cell_args(ctx, o);
+  if (ev) {
+    // events stamped by the kernel dispatch itself (start / end of the kernel)
+    hipExtLaunchKernelGGL((k_face<T, false>), gf, blk, 0, ctx->stream, ev[0], ev[1], 0, fa, ctx->energy_quad,
+                          table, s);
+    hipExtLaunchKernelGGL((k_cell<T>), gc, blk, 0, ctx->stream, ev[2], ev[3], 0, ca, table, s);
+  } else {
+    hipLaunchKernelGGL((k_face<T, false>), gf, blk, 0, ctx->stream, fa, ctx->energy_quad, table, s);
+    hipLaunchKernelGGL((k_cell<T>), gc, blk, 0, ctx->stream, ca, table, s);
+  }
+  CK(hipGetLastError());
+  ctx->cur ^= 1;
+  return ODIS_OK;
+}
+
+// Capture graph_steps steps reading coefficient buffer k, starting at the
+// current v parity (graph_steps is even, so the parity is restored).
+template <typename T>
+static int get_graph(odis_ctx *ctx, Ops<T> &o, int k, hipGraphExec_t *out) {
+  hipGraphExec_t &g = ctx->graphs[k][ctx->cur];
+  if (!g) {
+    hipGraph_t graph = nullptr;
+    CK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int rc = ODIS_OK;
+    for (int s = 0; s < ctx->graph_steps && !rc; s++) rc = launch_step(ctx, o, ctx->coef_dev[k], s, nullptr);
+    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(ctx, ODIS_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      g = nullptr;
+      return fail(ctx, ODIS_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
+  }
+  *out = g;
+  return ODIS_OK;
+}
+
+static void drop_graphs(odis_ctx *ctx) {
+  for (int k = 0; k < 2; k++)
+    for (int c = 0; c < 2; c++)
+      if (ctx->graphs[k][c]) {
+        (void)hipGraphExecDestroy(ctx->graphs[k][c]);
+        ctx->graphs[k][c] = nullptr;
+      }
+}
+
 template <typename T>
 static int step_t(odis_ctx *ctx, Ops<T> &o, int n_steps) {
-  const dim3 gf(ctx->nb_face, ctx->B), gc(ctx->nb_cell, ctx->B), blk(kBlock);
-  // timing mode: start/end events of both kernels of every step, read once
-  // at the end so the host never stalls the queue inside the timed region
-  const size_t need = ctx->timing ? size_t(4) * n_steps : 0;
-  while (ctx->ev.size() < need) {
-    hipEvent_t e;
-    CK(hipEventCreate(&e));
-    ctx->ev.push_back(e);
-  }
-  int done = 0;
+  // Timing mode (ctx->timing = T > 0): every T-th chunk is launched kernel by
+  // kernel with dispatch-stamped start/end events on every step; the other
+  // chunks run as graphs.  Events are read once, after the last chunk.
+  std::vector<int> sampled;  // event base index per sampled step
+  size_t ev_used = 0;
+  int done = 0, chunk_no = 0;
   while (done < n_steps) {
-    const int count = std::min(ctx->chunk, n_steps - done);
-    const double *coef = nullptr;
-    int rc = stage_coefficients(ctx, ctx->step, count, &coef);
-    if (rc) return rc;
-    for (int s = 0; s < count; s++, done++) {
-      if (ctx->stop_flag) {
-        ctx->stop_flag = 0;
-        return fail(ctx, ODIS_E_STOPPED, "stop requested");
-      }
-      const long long n = ctx->step;
-      const double *cs = coef + size_t(s) * ctx->B * 8;
-      FaceArgs<T> fa = face_args(ctx, o);
-      CellArgs<T> ca = cell_args(ctx, o);
-      if (ctx->sh_mode != ODIS_SH_OFF) {
-        const int rc2 = sh_effective_height(ctx, o);
-        if (rc2) return rc2;
-      }
-      if (ctx->timing) {
-        // events stamped by the kernel dispatch itself (start / end of the
-        // kernel), not by separate marker packets
-        hipExtLaunchKernelGGL((k_face<T, false>), gf, blk, 0, ctx->stream, ctx->ev[4 * done],
-                              ctx->ev[4 * done + 1], 0, fa, ctx->energy_quad, n, cs);
-        hipExtLaunchKernelGGL((k_cell<T>), gc, blk, 0, ctx->stream, ctx->ev[4 * done + 2], ctx->ev[4 * done + 3],
-                              0, ca, n);
-      } else {
-        hipLaunchKernelGGL((k_face<T, false>), gf, blk, 0, ctx->stream, fa, ctx->energy_quad, n, cs);
-        hipLaunchKernelGGL((k_cell<T>), gc, blk, 0, ctx->stream, ca, n);
-      }
-      CK(hipGetLastError());
-      ctx->cur ^= 1;
-      ctx->step = n + 1;
+    if (ctx->stop_flag) {
+      ctx->stop_flag = 0;
+      return fail(ctx, ODIS_E_STOPPED, "stop requested");
     }
+    const int left = n_steps - done;
+    const bool sample = ctx->timing > 0 && (chunk_no % ctx->timing) == 0;
+    const bool graph = ctx->use_graphs && left >= ctx->graph_steps && !sample;
+    const int count = graph ? ctx->graph_steps : std::min(ctx->chunk, left);
+    const double *table = nullptr;
+    int rc = stage_coefficients(ctx, ctx->step, count, &table);
+    if (rc) return rc;
+    if (graph) {
+      hipGraphExec_t g = nullptr;
+      if ((rc = get_graph(ctx, o, ctx->coef_last, &g))) return rc;
+      CK(hipGraphLaunch(g, ctx->stream));
+    } else {
+      for (int s = 0; s < count; s++) {
+        hipEvent_t *ev = nullptr;
+        if (sample) {
+          while (ctx->ev.size() < ev_used + 4) {
+            hipEvent_t e;
+            CK(hipEventCreate(&e));
+            ctx->ev.push_back(e);
+          }
+          ev = &ctx->ev[ev_used];
+          sampled.push_back(int(ev_used));
+          ev_used += 4;
+        }
+        if ((rc = launch_step(ctx, o, table, s, ev))) return rc;
+      }
+    }
+    ctx->step += count;
+    done += count;
+    chunk_no++;
   }
-  if (ctx->timing && done > 0) {
-    CK(hipEventSynchronize(ctx->ev[4 * (done - 1) + 3]));
-    double tf = 0, tc = 0;
-    for (int s = 0; s < done; s++) {
-      float a = 0, b = 0;
-      CK(hipEventElapsedTime(&a, ctx->ev[4 * s], ctx->ev[4 * s + 1]));
-      CK(hipEventElapsedTime(&b, ctx->ev[4 * s + 2], ctx->ev[4 * s + 3]));
+  if (!sampled.empty()) {
+    CK(hipEventSynchronize(ctx->ev[sampled.back() + 3]));
+    double tf = 0, tc = 0, ts = 0;
+    for (int base : sampled) {
+      float a = 0, b = 0, c = 0;
+      CK(hipEventElapsedTime(&a, ctx->ev[base], ctx->ev[base + 1]));
+      CK(hipEventElapsedTime(&b, ctx->ev[base + 2], ctx->ev[base + 3]));
+      CK(hipEventElapsedTime(&c, ctx->ev[base], ctx->ev[base + 3]));
       tf += a;
       tc += b;
+      ts += c;
     }
-    float all = 0;
-    CK(hipEventElapsedTime(&all, ctx->ev[0], ctx->ev[4 * (done - 1) + 3]));
-    ctx->t_face = tf / done;
-    ctx->t_cell = tc / done;
-    ctx->t_step = double(all) / done;
+    ctx->t_face = tf / sampled.size();
+    ctx->t_cell = tc / sampled.size();
+    ctx->t_step = ts / sampled.size();
   }
   return ODIS_OK;
 }
@@ -1028,8 +1100,8 @@ template <typename T>
 static int diss_t(odis_ctx *ctx, Ops<T> &o) {
   FaceArgs<T> fa = face_args(ctx, o);
   const dim3 gf(ctx->nb_face, ctx->B), blk(kBlock);
-  hipLaunchKernelGGL((k_face<T, true>), gf, blk, 0, ctx->stream, fa, ctx->energy_quad, ctx->step,
-                     static_cast<const double *>(nullptr));
+  hipLaunchKernelGGL((k_face<T, true>), gf, blk, 0, ctx->stream, fa, ctx->energy_quad,
+                     static_cast<const double *>(nullptr), 0);
   hipLaunchKernelGGL(k_diss_final, dim3(ctx->B), blk, 0, ctx->stream, ctx->partials, ctx->nb_face, ctx->dconst,
                      ctx->scratch_out);
   CK(hipGetLastError());
@@ -1111,9 +1183,15 @@ extern "C" int odis_request_stop(odis_ctx *ctx) {
   return ODIS_OK;
 }
 
-extern "C" int odis_enable_timing(odis_ctx *ctx, int on) {
+extern "C" int odis_enable_timing(odis_ctx *ctx, int stride) {
+  if (!ctx || stride < 0) return ODIS_E_INVALID;
+  ctx->timing = stride;
+  return ODIS_OK;
+}
+
+extern "C" int odis_use_graphs(odis_ctx *ctx, int on) {
   if (!ctx) return ODIS_E_INVALID;
-  ctx->timing = on;
+  ctx->use_graphs = on ? 1 : 0;
   return ODIS_OK;
 }
 
@@ -1245,6 +1323,7 @@ extern "C" int odis_sh_set_basis(odis_ctx *ctx, int l_max, const double *beta, i
   if (mode != ODIS_SH_OFF && mode != ODIS_SH_FP64 && mode != ODIS_SH_SPLIT_BF16)
     return fail(ctx, ODIS_E_INVALID, "bad sh mode");
   CK(hipStreamSynchronize(ctx->stream));
+  drop_graphs(ctx);
   for (void *p : ctx->sh_allocs) (void)hipFree(p);
   ctx->sh_allocs.clear();
   ctx->sh_mode = ODIS_SH_OFF;

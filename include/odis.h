@@ -174,10 +174,15 @@ int odis_sh_synthesize(odis_ctx *ctx, const double *coeffs, double *field);
  * odis_step returns ODIS_E_STOPPED at the next step boundary. */
 int odis_request_stop(odis_ctx *ctx);
 
-/* Timing helper for benchmarks: average device time (ms) of the last
- * odis_step call's kernels of the given kind (0 = face kernel, 1 = cell
- * kernel, 2 = whole step), measured with HIP events on the context stream. */
-int odis_enable_timing(odis_ctx *ctx, int on);
+/* odis_step runs chunks of 64 steps as captured HIP graphs (on by default);
+ * off launches every kernel from the host (same results bit for bit). */
+int odis_use_graphs(odis_ctx *ctx, int on);
+/* Timing for benchmarks: with stride T > 0, every T-th chunk of the next
+ * odis_step calls is launched kernel by kernel with dispatch-stamped
+ * start/stop HIP events on the context stream; odis_get_timing returns the
+ * average face-kernel, cell-kernel and face-start-to-cell-end times (ms) of
+ * the sampled steps of the last odis_step call.  T = 0 disables. */
+int odis_enable_timing(odis_ctx *ctx, int stride);
 int odis_get_timing(odis_ctx *ctx, double *face_ms, double *cell_ms, double *step_ms);
 
 /* ---- host: grid and mesh (src/mesh.cpp:32-197, 436-1154, 2808-3287) ---- */

@@ -285,7 +285,7 @@ def test_interp_gg2ll(product, oracle, precision):
 # spherical-harmonic self-gravity / ice-shell coupling
 # ---------------------------------------------------------------------------
 
-@pytest.mark.parametrize("mode,tol", [(1, 1e-11), (2, 2e-6)])
+@pytest.mark.parametrize("mode,tol", [(1, 1e-11), (2, 2e-5)])
 def test_sh_synthesis_and_projection(product, oracle, mode, tol):
     """Synthesis of unit coefficients reproduces scipy's harmonics on the
     nodes; the GPU least-squares projection recovers band-limited random
@@ -302,7 +302,9 @@ def test_sh_synthesis_and_projection(product, oracle, mode, tol):
     for k in (0, 5, 40, K - 1):
         e = np.zeros(K)
         e[k] = 1.0
-        assert np.max(np.abs(ctx.sh_synthesize(e) - Y[:, k])) < (1e-12 if mode == 1 else 1e-5)
+        # bf16 hi + lo keeps ~16 significant bits: |Y| <= sqrt(2l+1)-ish
+        bound = 1e-12 if mode == 1 else 2 ** -16 * np.max(np.abs(Y[:, k]))
+        assert np.max(np.abs(ctx.sh_synthesize(e) - Y[:, k])) < bound
     c_true = np.random.default_rng(11).standard_normal(K)
     c = ctx.sh_project(Y @ c_true)
     assert rel_l2(c, c_true) < tol
@@ -348,3 +350,37 @@ def test_self_gravity_shell_step_parity(product, oracle, precision):
     sim.step(100)
     ctx.step(100)
     assert rel_l2(ctx.get_state()["eta"], sim.get_state()["eta"]) > 1e-3
+
+
+def test_graph_replays_match_direct_launches(product, oracle):
+    """Chunks of 64 steps replayed from captured HIP graphs give the same
+    state, bit for bit, as kernel-by-kernel launches; replays are separated
+    by eager work (state reads, dissipation) and synchronisations, and each
+    replay is checked, with the SH coupling inside the graph too."""
+    G = product
+    g, arrs = generated_grid(5)
+    p, _ = G.make_params()
+    mesh = G.Mesh(g, p.radius)
+    a = G.Context(mesh, p)
+    b = G.Context(mesh, p)
+    b.use_graphs(False)
+    for ctx in (a, b):
+        ctx.set_sh(6, np.linspace(1.0, 1.3, 7), mode=G.SH_FP64)
+    for n in (64, 64, 10, 128, 1, 64):
+        a.step(n)
+        b.step(n)
+        sa, sb = a.get_state(), b.get_state()
+        assert sa["iter"] == sb["iter"]
+        for k in ("v", "eta", "dvdt", "detadt"):
+            assert np.array_equal(sa[k], sb[k]), (n, k)
+        assert a.dissipation()[0] == b.dissipation()[0]
+    sim = oracle.Sim(oracle.Mesh(arrs, p.radius),
+                     oracle.make_params(g=p.g, h=p.h, alpha=p.alpha, omega=p.omega, radius=p.radius,
+                                        ecc=p.ecc, dt=p.dt, tide=p.tide))
+    Y = oracle.real_sh_basis(6, arrs["lat"], arrs["lon"])
+    dfac = np.zeros(49)
+    for l in range(2, 7):
+        dfac[l * l:(l + 1) ** 2] = np.linspace(1.0, 1.3, 7)[l] - 1.0
+    sim.set_projector(Y * dfac[None, :], np.linalg.pinv(Y))
+    sim.step(331)
+    _compare(a, sim, TOL_SHORT)
