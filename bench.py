@@ -12,7 +12,8 @@ omega_max = 0.078 s^-1 on this grid, so 8 s steps diverge.  A "step" is one
 pass of the loop body over all cells and faces.
 
 Prints one JSON line: cell-steps/s of the whole job, the roofline of the
-dominant kernel (face kernel, HIP-event timed inside the timed region) and a
+dominant kernel (the face kernel, or the fused step kernel with --fused;
+HIP-event timed on every 16th step inside the timed region) and a
 CPU baseline (the FP64 C oracle, OpenMP, on a bounded sample of the same
 workload).
 """
@@ -41,6 +42,31 @@ def cell_bytes(E, N, p):
     return N * per_cell + E * p                   # + new v read once
 
 
+def fused_bytes(E, N, nfields, p):
+    """Algorithmic bytes of one fused step-kernel launch: the face kernel's
+    set plus the continuity operands and state of the cells (eta^{n-1} read,
+    eta^n written, two history reads and one write) and the owner byte per
+    face; v is read once for both updates."""
+    per_cell = 6 * 4 + 6 * p + p + 2 * p + p
+    return face_bytes(E, N, nfields, p) + N * per_cell + E * 1
+
+
+def measured_traffic(kname, tname, n_cells):
+    """HBM bytes per launch of the kernel from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE, copy-calibrated), when one exists for
+    this kernel, type and grid; else None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rows = json.load(f)["kernels"]
+    except (OSError, ValueError, KeyError):
+        return None
+    for r in rows:
+        if r.get("kernel") == "%s<%s>" % (kname, tname) and r.get("cells") == n_cells:
+            return r.get("hbm_bytes_per_launch")
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -51,6 +77,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU work of the baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--fused", action="store_true",
+                    help="one fused step kernel per step instead of the face + cell kernel pair")
+    ap.add_argument("--graphs", action="store_true", help="replay 64-step chunks as HIP graphs")
+    ap.add_argument("--timing-stride", type=int, default=16,
+                    help="dispatch-stamped HIP events on the kernels of every T-th step (0: none, "
+                         "roofline achieved then null)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (20 times), to calibrate "
                          "the FETCH_SIZE / WRITE_SIZE counters, and exit")
@@ -79,11 +111,13 @@ def main():
     prec = G.FP64 if args.precision == "fp64" else G.FP32
     pbytes = 8 if prec == G.FP64 else 4
     ctx = G.Context(mesh, p, precision=prec)
+    ctx.use_fused(args.fused)
+    ctx.use_graphs(args.graphs)
     print("bench: grid %d cells / %d faces ready in %.1fs" % (mesh.N, mesh.E, time.time() - t0),
           file=sys.stderr, flush=True)
 
     ctx.step(args.warmup, sync=True)
-    ctx.enable_timing(8)   # every 8th 64-step chunk kernel by kernel with HIP events
+    ctx.enable_timing(args.timing_stride)  # dispatch-stamped HIP events on the kernels
     ctx.sync()
     t1 = time.perf_counter()
     ctx.step(args.steps, sync=True)
@@ -96,9 +130,16 @@ def main():
     value = mesh.N * args.steps / elapsed
 
     nfields = 2
-    fb = face_bytes(mesh.E, mesh.N, nfields, pbytes)
     cb = cell_bytes(mesh.E, mesh.N, pbytes)
-    achieved = fb / (tm["face_ms"] * 1e-3) / 1e9
+    if not args.fused:
+        fb = face_bytes(mesh.E, mesh.N, nfields, pbytes)
+        kname, step_bytes = "k_face", fb + cb
+    else:
+        fb = fused_bytes(mesh.E, mesh.N, nfields, pbytes)
+        kname, step_bytes = "k_fused", fb
+    achieved = fb / (tm["face_ms"] * 1e-3) / 1e9 if tm["face_ms"] > 0 else None
+    tname = "double" if prec == G.FP64 else "float"
+    traffic = measured_traffic(kname, tname, mesh.N)
     out = {
         "metric": "cell-steps/s at level 7 (163842 cells)",
         "value": value,
@@ -118,8 +159,11 @@ def main():
             "cells": mesh.N, "faces": mesh.E, "instances": 1,
             "grid": "icosahedral bisection + 64 Lloyd sweeps, reference level %d" % args.level,
             "dt_s": p.dt, "steps_per_orbit": steps_per_orbit,
-            "kernel_ms": {"face": tm["face_ms"], "cell": tm["cell_ms"], "step_events": tm["step_ms"]},
-            "bytes_per_step_alg": fb + cb,
+            "launch": "fused step kernel, k_cell once per 4096-step chunk" if args.fused else
+                      "face kernel + cell kernel per step",
+            "graphs": bool(args.graphs),
+            "kernel_ms": {"momentum": tm["face_ms"], "cell": tm["cell_ms"], "step_events": tm["step_ms"]},
+            "bytes_per_step_alg": step_bytes,
             "dissipation_W_m2": diss,
         },
         "roofline": {
@@ -127,9 +171,9 @@ def main():
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": "k_face<%s>" % ("double" if prec == G.FP64 else "float"),
+            "frac": achieved / HBM_PEAK_GBS if achieved else None,
+            "traffic": traffic,
+            "kernel": "%s<%s>" % (kname, tname),
             "alg_bytes_per_launch": fb,
         },
     }

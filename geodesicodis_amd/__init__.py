@@ -36,7 +36,7 @@ EXPORTS = [
     "odis_mesh_sizes", "odis_mesh_describe", "odis_mesh_get", "odis_mesh_free",
     "odis_round_period", "odis_quantise_dt", "odis_config_read", "odis_host_last_error",
     "odis_run_case", "odis_interp_ll_set", "odis_interp_ll", "odis_sh_set_basis",
-    "odis_sh_project", "odis_sh_synthesize", "odis_surface_beta", "odis_use_graphs",
+    "odis_sh_project", "odis_sh_synthesize", "odis_surface_beta", "odis_use_fused", "odis_use_graphs",
 ]
 SH_OFF, SH_FP64, SH_SPLIT_BF16 = 0, 1, 2
 
@@ -100,6 +100,7 @@ def lib():
             "odis_request_stop": ([vp], ctypes.c_int),
             "odis_enable_timing": ([vp, ctypes.c_int], ctypes.c_int),
             "odis_use_graphs": ([vp, ctypes.c_int], ctypes.c_int),
+            "odis_use_fused": ([vp, ctypes.c_int], ctypes.c_int),
             "odis_get_timing": ([vp, dp, dp, dp], ctypes.c_int),
             "odis_grid_read": ([ctypes.c_char_p, vpp], ctypes.c_int),
             "odis_grid_generate": ([ctypes.c_int, ctypes.c_int, vpp], ctypes.c_int),
@@ -387,6 +388,10 @@ class Context:
 
     def use_graphs(self, on=True):
         self._check(lib().odis_use_graphs(self._h, 1 if on else 0))
+
+    def use_fused(self, on=True):
+        """One kernel per step (default) or a face and a cell kernel per step."""
+        self._check(lib().odis_use_fused(self._h, 1 if on else 0))
 
     def timing(self):
         a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()

@@ -82,7 +82,8 @@ __global__ __launch_bounds__(kBlock) void k_face(FaceArgs<T> a, int energy_quad,
     }
   }
   const double part = block_sum<T>(local, sh);
-  if (threadIdx.x == 0) a.partials[b * a.nb + tile] = part;
+  const long long par = DISS_ONLY ? 0 : (((long long)table[0] + s) & 1);
+  if (threadIdx.x == 0) a.partials[(par * gridDim.y + b) * a.nb + tile] = part;
 }
 
 // Sum the face kernel's partials of instance b in a fixed order.
@@ -127,7 +128,7 @@ __global__ __launch_bounds__(kBlock) void k_cell(CellArgs<T> a, const double *ta
   // the first-dispatched workgroup of each instance reduces the face
   // kernel's dissipation partials of this step (complete: previous kernel)
   if (blockIdx.x != 0) return;
-  const double tot = sum_partials(a.partials + size_t(b) * a.nb_face, a.nb_face, sh);
+  const double tot = sum_partials(a.partials + ((n & 1) * gridDim.y + b) * size_t(a.nb_face), a.nb_face, sh);
   if (threadIdx.x == 0) {
     const double d = tot * a.dconst[b];
     a.series[size_t(b) * a.cap + size_t(n % a.cap)] = d;
@@ -156,6 +157,120 @@ __global__ __launch_bounds__(kBlock) void k_velocity_en(int E, const int *tri_id
   const double nx = normal[2 * e], ny = normal[2 * e + 1], vn = double(v[e]);
   uv[2 * e] = nx * vn + ny * vt;
   uv[2 * e + 1] = ny * vn + (-nx) * vt;
+}
+
+// ---------------------------------------------------------------------------
+// fused step: one kernel per step
+// ---------------------------------------------------------------------------
+// Step n's kernel first completes the continuity update of step n-1 for the
+// two cells of its face (eta^n = eta^{n-1} + dt AB3(h Div v^n)), with the
+// same arithmetic as k_cell; the face that owns a cell (one per cell) writes
+// eta^n and the history slot, the others only use the value.  Then it does
+// the momentum update of step n with eta^n.  The last step of a chunk is
+// completed by k_cell.  eta is double-buffered (read n-1, write n).
+template <typename T>
+struct FusedArgs {
+  FaceArgs<T> f;                // eta = eta^{n-1} buffer when PENDING, else eta^n
+  int N;
+  const int *div_idx;           // [6][N]
+  const T *div_coef;            // [6][N]
+  T *eta_nxt;                   // [B][N] eta^n (owner writes)
+  T *ring_e;                    // [B][3][N]
+  long long ring_e_stride;
+  const unsigned char *owner;   // [E] bit 0: owns cell g_idx[e], bit 1: owns g_idx[E + e]
+  double *series;
+  int *nonfinite;
+  int cap;
+  const double *dconst;
+};
+
+template <typename T>
+__device__ __forceinline__ T complete_continuity(const FusedArgs<T> &a, int c, const T *v, const T *eta, T *ringe,
+                                                 long long m, bool ab3, T h, T dt, bool own, int b) {
+  const int N = a.N;
+  T acc = T(0);
+#pragma unroll
+  for (int j = 0; j < 6; ++j) acc += a.div_coef[j * N + c] * v[a.div_idx[j * N + c]];
+  const T f0 = h * acc;
+  const int s0 = int(m % 3), s1 = int((m + 2) % 3), s2 = int((m + 1) % 3);
+  T upd;
+  if (ab3)
+    upd = (T(23. / 12.) * f0 + T(-16. / 12.) * ringe[s1 * N + c] + T(5. / 12.) * ringe[s2 * N + c]) * dt;
+  else
+    upd = f0 * dt;
+  const T val = eta[c] + upd;
+  if (own) {
+    a.eta_nxt[b * a.f.eta_stride + c] = val;
+    ringe[s0 * N + c] = f0;
+  }
+  return val;
+}
+
+template <typename T, bool PENDING>
+__global__ __launch_bounds__(kBlock) void k_fused(FusedArgs<T> a, int energy_quad, const double *table, int s) {
+  __shared__ double sh[4];
+  __shared__ double sh2[4];
+  const FaceArgs<T> &F = a.f;
+  const int b = blockIdx.y;
+  const int tile = xcd_tile(blockIdx.x, F.nb);
+  const int E = F.E;
+  const int e = tile * kBlock + threadIdx.x;
+  const double *ip = F.inst + b * 8;
+  const long long n = (long long)table[0] + s;
+  const double *coef = table + 8 + size_t(s) * gridDim.y * 8;
+  const bool load = ip[3] != 0.0;
+  double local = 0.0;
+  if (e < E) {
+    const T *v = F.v_cur + b * F.v_stride;
+    T vt = T(0);
+#pragma unroll
+    for (int j = 0; j < 10; ++j) vt += F.tri_coef[j * E + e] * v[F.tri_idx[j * E + e]];
+    const T ve = v[e];
+    const T sp = ve * ve + vt * vt;
+    const T en = energy_quad ? sqrt(sp) * sp : sp;
+    local = double(F.farea[e]) * double(en);
+    const T g = T(ip[0]), alpha = T(ip[1]), dt = T(ip[2]), qd = T(ip[4]), h = T(ip[5]);
+    const T *eta = F.eta + b * F.eta_stride;
+    const int c0 = F.g_idx[e], c1 = F.g_idx[E + e];
+    T eta0, eta1;
+    if (PENDING) {
+      const long long m = n - 1;
+      const bool ab3m = (m >= 2) || load;
+      T *ringe = a.ring_e + b * a.ring_e_stride;
+      const unsigned int own = a.owner[e];
+      eta0 = complete_continuity(a, c0, v, eta, ringe, m, ab3m, h, dt, (own & 1u) != 0, b);
+      eta1 = complete_continuity(a, c1, v, eta, ringe, m, ab3m, h, dt, (own & 2u) != 0, b);
+    } else {
+      eta0 = eta[c0];
+      eta1 = eta[c1];
+    }
+    const bool ab3 = (n >= 2) || load;
+    const T ge = F.g_coef[e] * eta0 + F.g_coef[E + e] * eta1;
+    T f0 = -g * ge + F.fcor[e] * vt;
+    if (F.quad_drag) f0 -= qd * sqrt(sp) * ve;
+    T fu = T(0);
+    for (int k = 0; k < F.nfields; ++k) fu += T(coef[b * 8 + k]) * F.fgs[k * E + e];
+    const T drag = -alpha * ve + fu;
+    T *ring = F.ring + b * F.ring_stride;
+    const int s0 = int(n % 3), s1 = int((n + 2) % 3), s2 = int((n + 1) % 3);
+    T upd;
+    if (ab3)
+      upd = (T(23. / 12.) * f0 + T(-16. / 12.) * ring[s1 * E + e] + T(5. / 12.) * ring[s2 * E + e]) * dt;
+    else
+      upd = f0 * dt;
+    ring[s0 * E + e] = f0;
+    F.v_nxt[b * F.v_stride + e] = (ve + upd) + dt * drag;
+  }
+  const double part = block_sum<T>(local, sh);
+  if (threadIdx.x == 0) F.partials[((n & 1) * gridDim.y + b) * F.nb + tile] = part;
+  // dissipation of state n-1: partials of the previous kernel
+  if (!PENDING || blockIdx.x != 0) return;
+  const double tot = sum_partials(F.partials + (((n - 1) & 1) * gridDim.y + b) * size_t(F.nb), F.nb, sh2);
+  if (threadIdx.x == 0) {
+    const double d = tot * a.dconst[b];
+    a.series[size_t(b) * a.cap + size_t((n - 1) % a.cap)] = d;
+    if (!isfinite(d)) a.nonfinite[b] = 1;
+  }
 }
 
 // CSR rows of the lat-lon interpolation: one wavefront per row, the eta
@@ -303,7 +418,9 @@ struct Ops {
   int *div_idx = nullptr;
   T *div_coef = nullptr;
   T *v[2] = {nullptr, nullptr};
-  T *ring_v = nullptr, *eta = nullptr, *ring_e = nullptr;
+  T *ring_v = nullptr, *ring_e = nullptr;
+  T *eta_buf[2] = {nullptr, nullptr};  // canonical eta is eta_buf[ctx->ecur]
+  unsigned char *owner = nullptr;
 };
 
 uint64_t hilbert3(uint32_t X0, uint32_t X1, uint32_t X2, int bits) {
@@ -344,9 +461,10 @@ struct odis_ctx {
   std::vector<int> cell_perm, cell_inv, face_perm, face_inv;  // ref -> dev, dev -> ref
   Ops<double> d;
   Ops<float> f;
-  int cur = 0;
+  int cur = 0, ecur = 0;        // v is v[cur], eta is eta_buf[ecur] between calls
+  int fused = 0;               // one kernel per step (SH coupling off)
   long long step = 0;          // next step index n, shared by all instances
-  double *partials = nullptr;  // [B][nb_face]
+  double *partials = nullptr;  // [2][B][nb_face], by step parity
   double *series = nullptr;    // [B][cap]
   int *nonfinite = nullptr;    // [B]
   int cap = 1 << 16;
@@ -360,9 +478,9 @@ struct odis_ctx {
   double *coef_host[2] = {nullptr, nullptr};
   hipEvent_t coef_ev[2] = {nullptr, nullptr};
   int coef_next = 0, coef_last = 0;
-  // graphs of graph_steps steps, one per (coefficient buffer, v parity)
-  int graph_steps = 0, use_graphs = 1;
-  hipGraphExec_t graphs[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  // graphs of graph_steps steps, one per (coefficient buffer, v parity x 2 + eta parity)
+  int graph_steps = 0, use_graphs = 0;
+  hipGraphExec_t graphs[2][4] = {};
   std::vector<TideDev> tides;
   std::vector<odis_params> params;
   std::vector<std::vector<double>> init_hist_v, init_hist_e;  // history as last set ([M][3])
@@ -494,8 +612,10 @@ template <typename T>
 static int build_ops(odis_ctx *ctx, Ops<T> &o, const std::vector<double> &tri_c, const std::vector<int> &tri_i,
                      const std::vector<double> &gc, const std::vector<int> &gi, const std::vector<double> &fcor,
                      const std::vector<double> &farea, const std::vector<double> &fgs,
-                     const std::vector<double> &dc, const std::vector<int> &di) {
+                     const std::vector<double> &dc, const std::vector<int> &di,
+                     const std::vector<unsigned char> &owner) {
   int rc;
+  if ((rc = upload(ctx, &o.owner, owner))) return rc;
   if ((rc = upload(ctx, &o.tri_idx, tri_i))) return rc;
   if ((rc = upload(ctx, &o.tri_coef, cast_vec<T>(tri_c)))) return rc;
   if ((rc = upload(ctx, &o.g_idx, gi))) return rc;
@@ -509,12 +629,14 @@ static int build_ops(odis_ctx *ctx, Ops<T> &o, const std::vector<double> &tri_c,
   if ((rc = dev_alloc(ctx, &o.v[0], B * E))) return rc;
   if ((rc = dev_alloc(ctx, &o.v[1], B * E))) return rc;
   if ((rc = dev_alloc(ctx, &o.ring_v, B * 3 * E))) return rc;
-  if ((rc = dev_alloc(ctx, &o.eta, B * N))) return rc;
+  if ((rc = dev_alloc(ctx, &o.eta_buf[0], B * N))) return rc;
+  if ((rc = dev_alloc(ctx, &o.eta_buf[1], B * N))) return rc;
   if ((rc = dev_alloc(ctx, &o.ring_e, B * 3 * N))) return rc;
   CK(hipMemset(o.v[0], 0, B * E * sizeof(T)));
   CK(hipMemset(o.v[1], 0, B * E * sizeof(T)));
   CK(hipMemset(o.ring_v, 0, B * 3 * E * sizeof(T)));
-  CK(hipMemset(o.eta, 0, B * N * sizeof(T)));
+  CK(hipMemset(o.eta_buf[0], 0, B * N * sizeof(T)));
+  CK(hipMemset(o.eta_buf[1], 0, B * N * sizeof(T)));
   CK(hipMemset(o.ring_e, 0, B * 3 * N * sizeof(T)));
   return ODIS_OK;
 }
@@ -524,7 +646,7 @@ extern "C" int odis_destroy(odis_ctx *ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto &e : ctx->ev) (void)hipEventDestroy(e);
   for (int k = 0; k < 2; k++)
-    for (int c = 0; c < 2; c++)
+    for (int c = 0; c < 4; c++)
       if (ctx->graphs[k][c]) (void)hipGraphExecDestroy(ctx->graphs[k][c]);
   for (int k = 0; k < 2; k++) {
     if (ctx->coef_ev[k]) (void)hipEventDestroy(ctx->coef_ev[k]);
@@ -673,14 +795,29 @@ extern "C" int odis_create(const odis_mesh_desc *m, const odis_params *params, i
     }
   }
 
+  // one owner face per cell (its lowest device face index) writes the cell's
+  // continuity update in the fused step kernel
+  std::vector<unsigned char> owner(E, 0);
+  for (int r = 0; r < N; r++) {
+    int f = di[r];
+    for (int j = 1; j < 6; j++) f = std::min(f, di[size_t(j) * N + r]);
+    if (gi[f] == r) {
+      owner[f] |= 1u;
+    } else if (gi[size_t(E) + f] == r) {
+      owner[f] |= 2u;
+    } else {
+      delete ctx;
+      return fail(nullptr, ODIS_E_INVALID, "cell_faces disagrees with face_nodes");
+    }
+  }
   int rc = ODIS_OK;
   hipError_t he = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (he != hipSuccess) rc = fail(ctx, ODIS_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
   if (!rc) {
     if (precision == ODIS_FP64)
-      rc = build_ops(ctx, ctx->d, tri_c, tri_i, gc, gi, fcor, farea, fgs, dc, di);
+      rc = build_ops(ctx, ctx->d, tri_c, tri_i, gc, gi, fcor, farea, fgs, dc, di, owner);
     else
-      rc = build_ops(ctx, ctx->f, tri_c, tri_i, gc, gi, fcor, farea, fgs, dc, di);
+      rc = build_ops(ctx, ctx->f, tri_c, tri_i, gc, gi, fcor, farea, fgs, dc, di, owner);
   }
   ctx->nb_face = (E + kBlock - 1) / kBlock;
   ctx->nb_cell = (N + kBlock - 1) / kBlock;
@@ -703,7 +840,7 @@ extern "C" int odis_create(const odis_mesh_desc *m, const odis_params *params, i
   }
   if (!rc) rc = upload(ctx, &ctx->inst, inst);
   if (!rc) rc = upload(ctx, &ctx->dconst, dconst);
-  if (!rc) rc = dev_alloc(ctx, &ctx->partials, size_t(n_inst) * ctx->nb_face);
+  if (!rc) rc = dev_alloc(ctx, &ctx->partials, size_t(2) * n_inst * ctx->nb_face);
   if (!rc) rc = dev_alloc(ctx, &ctx->series, size_t(n_inst) * ctx->cap);
   if (!rc) rc = dev_alloc(ctx, &ctx->nonfinite, size_t(n_inst));
   if (!rc) rc = dev_alloc(ctx, &ctx->scratch_out, size_t(n_inst));
@@ -766,7 +903,7 @@ static int put_state(odis_ctx *ctx, Ops<T> &o, int b, const double *v, const dou
   if (eta) {
     std::vector<T> h(N);
     for (int r = 0; r < N; r++) h[r] = T(eta[ctx->cell_inv[r]]);
-    CK(hipMemcpy(o.eta + bN, h.data(), N * sizeof(T), hipMemcpyHostToDevice));
+    CK(hipMemcpy(o.eta_buf[ctx->ecur] + bN, h.data(), N * sizeof(T), hipMemcpyHostToDevice));
   }
   auto put_hist = [&](T *ring, int M, const std::vector<int> &inv, const double *ds,
                       std::vector<double> &keep) -> int {
@@ -816,7 +953,7 @@ static int get_state_t(odis_ctx *ctx, Ops<T> &o, int b, double *v, double *dv, d
   }
   if (eta) {
     std::vector<T> h(N);
-    CK(hipMemcpy(h.data(), o.eta + bN, N * sizeof(T), hipMemcpyDeviceToHost));
+    CK(hipMemcpy(h.data(), o.eta_buf[ctx->ecur] + bN, N * sizeof(T), hipMemcpyDeviceToHost));
     for (int r = 0; r < N; r++) eta[ctx->cell_inv[r]] = double(h[r]);
   }
   auto get_hist = [&](T *ring, int M, const std::vector<int> &inv, double *ds,
@@ -877,7 +1014,7 @@ static FaceArgs<T> face_args(odis_ctx *ctx, Ops<T> &o) {
   a.v_cur = o.v[ctx->cur];
   a.v_nxt = o.v[ctx->cur ^ 1];
   a.ring = o.ring_v;
-  a.eta = ctx->sh_mode != ODIS_SH_OFF ? static_cast<const T *>(ctx->eta_eff) : o.eta;
+  a.eta = ctx->sh_mode != ODIS_SH_OFF ? static_cast<const T *>(ctx->eta_eff) : o.eta_buf[ctx->ecur];
   a.v_stride = ctx->E;
   a.ring_stride = 3LL * ctx->E;
   a.eta_stride = ctx->N;
@@ -896,7 +1033,7 @@ static CellArgs<T> cell_args(odis_ctx *ctx, Ops<T> &o) {
   a.div_idx = o.div_idx;
   a.div_coef = o.div_coef;
   a.v_new = o.v[ctx->cur ^ 1];
-  a.eta = o.eta;
+  a.eta = o.eta_buf[ctx->ecur];
   a.ring = o.ring_e;
   a.v_stride = ctx->E;
   a.eta_stride = ctx->N;
@@ -944,66 +1081,164 @@ static int sh_effective_height(odis_ctx *ctx, Ops<T> &o) {
   const dim3 blk(kBlock);
   if (ctx->sh_mode == ODIS_SH_SPLIT_BF16)
     hipLaunchKernelGGL((k_sh_analysis<T, true>), dim3(K, ctx->B), blk, 0, ctx->stream, K, N, ctx->sh_Yt,
-                       ctx->sh_Yp, o.eta, (long long)N, ctx->sh_b);
+                       ctx->sh_Yp, o.eta_buf[ctx->ecur], (long long)N, ctx->sh_b);
   else
     hipLaunchKernelGGL((k_sh_analysis<T, false>), dim3(K, ctx->B), blk, 0, ctx->stream, K, N, ctx->sh_Yt,
-                       ctx->sh_Yp, o.eta, (long long)N, ctx->sh_b);
+                       ctx->sh_Yp, o.eta_buf[ctx->ecur], (long long)N, ctx->sh_b);
   hipLaunchKernelGGL(k_sh_coef, dim3((K + 3) / 4, ctx->B), blk, 0, ctx->stream, K, ctx->sh_Minv, ctx->sh_D,
                      ctx->sh_b, ctx->sh_c);
   const size_t lds = sizeof(double) * K;
   if (ctx->sh_mode == ODIS_SH_SPLIT_BF16)
     hipLaunchKernelGGL((k_sh_synth<T, true>), dim3(ctx->nb_cell, ctx->B), blk, lds, ctx->stream, K, N, ctx->sh_Yt,
-                       ctx->sh_Yp, ctx->sh_c, o.eta, (long long)N, static_cast<T *>(ctx->eta_eff));
+                       ctx->sh_Yp, ctx->sh_c, o.eta_buf[ctx->ecur], (long long)N, static_cast<T *>(ctx->eta_eff));
   else
     hipLaunchKernelGGL((k_sh_synth<T, false>), dim3(ctx->nb_cell, ctx->B), blk, lds, ctx->stream, K, N,
-                       ctx->sh_Yt, ctx->sh_Yp, ctx->sh_c, o.eta, (long long)N, static_cast<T *>(ctx->eta_eff));
+                       ctx->sh_Yt, ctx->sh_Yp, ctx->sh_c, o.eta_buf[ctx->ecur], (long long)N,
+                       static_cast<T *>(ctx->eta_eff));
   CK(hipGetLastError());
   return ODIS_OK;
 }
 
-// Enqueue the kernels of one step (s = index within the staged table).
 template <typename T>
-static int launch_step(odis_ctx *ctx, Ops<T> &o, const double *table, int s, hipEvent_t *ev) {
+static FusedArgs<T> fused_args(odis_ctx *ctx, Ops<T> &o) {
+  FusedArgs<T> a;
+  a.f = face_args(ctx, o);
+  a.N = ctx->N;
+  a.div_idx = o.div_idx;
+  a.div_coef = o.div_coef;
+  a.eta_nxt = o.eta_buf[ctx->ecur ^ 1];
+  a.ring_e = o.ring_e;
+  a.ring_e_stride = 3LL * ctx->N;
+  a.owner = o.owner;
+  a.series = ctx->series;
+  a.nonfinite = ctx->nonfinite;
+  a.cap = ctx->cap;
+  a.dconst = ctx->dconst;
+  return a;
+}
+
+// Event pairs (start, end) recorded for the kernels of sampled chunks.
+struct ChunkEvents {
+  std::vector<int> mom;   // momentum kernel of each step (k_face or k_fused)
+  std::vector<int> cont;  // continuity kernels (k_cell)
+};
+
+static int reserve_events(odis_ctx *ctx, size_t n) {
+  while (ctx->ev.size() < n) {
+    hipEvent_t e;
+    CK(hipEventCreate(&e));
+    ctx->ev.push_back(e);
+  }
+  return ODIS_OK;
+}
+
+// Enqueue the steps [n0, n0 + count) of a staged coefficient table
+// (n0 = ctx->step).  With the fused kernel a step is one launch plus one
+// k_cell at the end of the chunk; with the SH coupling it is the projection
+// kernels, k_face and k_cell.  evs: dispatch-stamped events on the kernels of
+// every ctx->timing-th step (null: none).
+template <typename T>
+static int launch_chunk(odis_ctx *ctx, Ops<T> &o, const double *table, int count, ChunkEvents *evs,
+                        size_t *ev_used) {
   const dim3 gf(ctx->nb_face, ctx->B), gc(ctx->nb_cell, ctx->B), blk(kBlock);
-  if (ctx->sh_mode != ODIS_SH_OFF) {
-    const int rc = sh_effective_height(ctx, o);
+  const long long n0 = ctx->step;
+  if (evs) {
+    const size_t sampled = size_t(count / ctx->timing + 2);
+    const int rc = reserve_events(ctx, *ev_used + 4 * sampled + 2);
     if (rc) return rc;
   }
-  FaceArgs<T> fa = face_args(ctx, o);
-  CellArgs<T> ca = cell_args(ctx, o);
-  if (ev) {
-    // events stamped by the kernel dispatch itself (start / end of the kernel)
-    hipExtLaunchKernelGGL((k_face<T, false>), gf, blk, 0, ctx->stream, ev[0], ev[1], 0, fa, ctx->energy_quad,
-                          table, s);
-    hipExtLaunchKernelGGL((k_cell<T>), gc, blk, 0, ctx->stream, ev[2], ev[3], 0, ca, table, s);
-  } else {
-    hipLaunchKernelGGL((k_face<T, false>), gf, blk, 0, ctx->stream, fa, ctx->energy_quad, table, s);
-    hipLaunchKernelGGL((k_cell<T>), gc, blk, 0, ctx->stream, ca, table, s);
+  bool any = false;
+  std::vector<int> dummy_rec;
+  auto take = [&](std::vector<int> &rec, int s) -> hipEvent_t * {
+    if (!evs || (s >= 0 && (n0 + s) % ctx->timing != 0) || (s < 0 && !any)) return nullptr;
+    any = true;
+    rec.push_back(int(*ev_used));
+    hipEvent_t *p = &ctx->ev[*ev_used];
+    *ev_used += 2;
+    return p;
+  };
+  if (ctx->fused && ctx->sh_mode == ODIS_SH_OFF) {
+    for (int s = 0; s < count; s++) {
+      FusedArgs<T> a = fused_args(ctx, o);
+      hipEvent_t *ev = take(evs ? evs->mom : dummy_rec, s);
+      if (s == 0) {
+        if (ev)
+          hipExtLaunchKernelGGL((k_fused<T, false>), gf, blk, 0, ctx->stream, ev[0], ev[1], 0, a,
+                                ctx->energy_quad, table, s);
+        else
+          hipLaunchKernelGGL((k_fused<T, false>), gf, blk, 0, ctx->stream, a, ctx->energy_quad, table, s);
+      } else {
+        if (ev)
+          hipExtLaunchKernelGGL((k_fused<T, true>), gf, blk, 0, ctx->stream, ev[0], ev[1], 0, a,
+                                ctx->energy_quad, table, s);
+        else
+          hipLaunchKernelGGL((k_fused<T, true>), gf, blk, 0, ctx->stream, a, ctx->energy_quad, table, s);
+        ctx->ecur ^= 1;
+      }
+      ctx->cur ^= 1;
+    }
+    // continuity of the chunk's last step, in place on eta_buf[ecur]
+    CellArgs<T> ca = cell_args(ctx, o);
+    ca.v_new = o.v[ctx->cur];
+    hipEvent_t *ev = take(evs ? evs->cont : dummy_rec, -1);
+    if (ev)
+      hipExtLaunchKernelGGL((k_cell<T>), gc, blk, 0, ctx->stream, ev[0], ev[1], 0, ca, table, count - 1);
+    else
+      hipLaunchKernelGGL((k_cell<T>), gc, blk, 0, ctx->stream, ca, table, count - 1);
+    CK(hipGetLastError());
+    return ODIS_OK;
   }
-  CK(hipGetLastError());
-  ctx->cur ^= 1;
+  for (int s = 0; s < count; s++) {
+    if (ctx->sh_mode != ODIS_SH_OFF) {
+      const int rc = sh_effective_height(ctx, o);
+      if (rc) return rc;
+    }
+    FaceArgs<T> fa = face_args(ctx, o);
+    CellArgs<T> ca = cell_args(ctx, o);
+    hipEvent_t *ev = take(evs ? evs->mom : dummy_rec, s);
+    const bool sampled = ev != nullptr;
+    if (ev)
+      hipExtLaunchKernelGGL((k_face<T, false>), gf, blk, 0, ctx->stream, ev[0], ev[1], 0, fa, ctx->energy_quad,
+                            table, s);
+    else
+      hipLaunchKernelGGL((k_face<T, false>), gf, blk, 0, ctx->stream, fa, ctx->energy_quad, table, s);
+    ev = sampled ? take(evs->cont, s) : nullptr;
+    if (ev)
+      hipExtLaunchKernelGGL((k_cell<T>), gc, blk, 0, ctx->stream, ev[0], ev[1], 0, ca, table, s);
+    else
+      hipLaunchKernelGGL((k_cell<T>), gc, blk, 0, ctx->stream, ca, table, s);
+    CK(hipGetLastError());
+    ctx->cur ^= 1;
+  }
   return ODIS_OK;
 }
 
-// Capture graph_steps steps reading coefficient buffer k, starting at the
-// current v parity (graph_steps is even, so the parity is restored).
+// Graph of graph_steps steps reading coefficient buffer k, one per entry
+// parity of (v, eta); replaying it leaves the parities as the captured
+// launches did.
 template <typename T>
 static int get_graph(odis_ctx *ctx, Ops<T> &o, int k, hipGraphExec_t *out) {
-  hipGraphExec_t &g = ctx->graphs[k][ctx->cur];
-  if (!g) {
-    hipGraph_t graph = nullptr;
-    CK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    int rc = ODIS_OK;
-    for (int s = 0; s < ctx->graph_steps && !rc; s++) rc = launch_step(ctx, o, ctx->coef_dev[k], s, nullptr);
-    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
-    if (rc) return rc;
-    if (e != hipSuccess) return fail(ctx, ODIS_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-    e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (e != hipSuccess) {
-      g = nullptr;
-      return fail(ctx, ODIS_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
-    }
+  const int key = ctx->cur * 2 + ctx->ecur;
+  hipGraphExec_t &g = ctx->graphs[k][key];
+  if (g) {
+    *out = g;
+    return ODIS_OK;
+  }
+  const int cur0 = ctx->cur, ecur0 = ctx->ecur;
+  hipGraph_t graph = nullptr;
+  CK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  int rc = launch_chunk(ctx, o, ctx->coef_dev[k], ctx->graph_steps, nullptr, nullptr);
+  hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+  // replays advance the parities (advance_parities); restore them here
+  ctx->cur = cur0;
+  ctx->ecur = ecur0;
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(ctx, ODIS_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    g = nullptr;
+    return fail(ctx, ODIS_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
   }
   *out = g;
   return ODIS_OK;
@@ -1011,30 +1246,39 @@ static int get_graph(odis_ctx *ctx, Ops<T> &o, int k, hipGraphExec_t *out) {
 
 static void drop_graphs(odis_ctx *ctx) {
   for (int k = 0; k < 2; k++)
-    for (int c = 0; c < 2; c++)
+    for (int c = 0; c < 4; c++)
       if (ctx->graphs[k][c]) {
         (void)hipGraphExecDestroy(ctx->graphs[k][c]);
         ctx->graphs[k][c] = nullptr;
       }
 }
 
+// parities after launching count steps from (cur, ecur)
+static void advance_parities(odis_ctx *ctx, int count) {
+  if (count & 1) ctx->cur ^= 1;
+  if (ctx->fused && ctx->sh_mode == ODIS_SH_OFF && ((count - 1) & 1)) ctx->ecur ^= 1;
+}
+
 template <typename T>
 static int step_t(odis_ctx *ctx, Ops<T> &o, int n_steps) {
-  // Timing mode (ctx->timing = T > 0): every T-th chunk is launched kernel by
-  // kernel with dispatch-stamped start/end events on every step; the other
-  // chunks run as graphs.  Events are read once, after the last chunk.
-  std::vector<int> sampled;  // event base index per sampled step
+  // Timing mode (ctx->timing = T > 0): the kernels of every T-th step get
+  // dispatch-stamped start/end events (chunks with a sampled step are not
+  // replayed from graphs).  Events are read once, after the last chunk.
+  std::vector<ChunkEvents> sampled;
   size_t ev_used = 0;
-  int done = 0, chunk_no = 0;
+  int done = 0;
   while (done < n_steps) {
     if (ctx->stop_flag) {
       ctx->stop_flag = 0;
       return fail(ctx, ODIS_E_STOPPED, "stop requested");
     }
     const int left = n_steps - done;
-    const bool sample = ctx->timing > 0 && (chunk_no % ctx->timing) == 0;
-    const bool graph = ctx->use_graphs && left >= ctx->graph_steps && !sample;
-    const int count = graph ? ctx->graph_steps : std::min(ctx->chunk, left);
+    const bool graph_fits = ctx->use_graphs && left >= ctx->graph_steps;
+    int count = graph_fits ? ctx->graph_steps : std::min(ctx->chunk, left);
+    // does [step, step + count) hold a multiple of the timing stride?
+    const bool sample = ctx->timing > 0 && ((ctx->step + ctx->timing - 1) / ctx->timing) * ctx->timing <
+                                               ctx->step + count;
+    const bool graph = graph_fits && !sample;
     const double *table = nullptr;
     int rc = stage_coefficients(ctx, ctx->step, count, &table);
     if (rc) return rc;
@@ -1042,41 +1286,49 @@ static int step_t(odis_ctx *ctx, Ops<T> &o, int n_steps) {
       hipGraphExec_t g = nullptr;
       if ((rc = get_graph(ctx, o, ctx->coef_last, &g))) return rc;
       CK(hipGraphLaunch(g, ctx->stream));
+      advance_parities(ctx, count);
     } else {
-      for (int s = 0; s < count; s++) {
-        hipEvent_t *ev = nullptr;
-        if (sample) {
-          while (ctx->ev.size() < ev_used + 4) {
-            hipEvent_t e;
-            CK(hipEventCreate(&e));
-            ctx->ev.push_back(e);
-          }
-          ev = &ctx->ev[ev_used];
-          sampled.push_back(int(ev_used));
-          ev_used += 4;
-        }
-        if ((rc = launch_step(ctx, o, table, s, ev))) return rc;
+      ChunkEvents *evs = nullptr;
+      if (sample) {
+        sampled.emplace_back();
+        evs = &sampled.back();
       }
+      if ((rc = launch_chunk(ctx, o, table, count, evs, &ev_used))) return rc;
     }
     ctx->step += count;
     done += count;
-    chunk_no++;
   }
   if (!sampled.empty()) {
-    CK(hipEventSynchronize(ctx->ev[sampled.back() + 3]));
-    double tf = 0, tc = 0, ts = 0;
-    for (int base : sampled) {
-      float a = 0, b = 0, c = 0;
-      CK(hipEventElapsedTime(&a, ctx->ev[base], ctx->ev[base + 1]));
-      CK(hipEventElapsedTime(&b, ctx->ev[base + 2], ctx->ev[base + 3]));
-      CK(hipEventElapsedTime(&c, ctx->ev[base], ctx->ev[base + 3]));
-      tf += a;
-      tc += b;
-      ts += c;
+    CK(hipStreamSynchronize(ctx->stream));
+    double tm = 0, tc = 0, ts = 0;
+    long long nm = 0, nc = 0, ns = 0;
+    for (const ChunkEvents &c : sampled) {
+      for (size_t j = 0; j < c.mom.size(); j++) {
+        float a = 0;
+        CK(hipEventElapsedTime(&a, ctx->ev[c.mom[j]], ctx->ev[c.mom[j] + 1]));
+        tm += a;
+        nm++;
+      }
+      for (int base : c.cont) {
+        float a = 0;
+        CK(hipEventElapsedTime(&a, ctx->ev[base], ctx->ev[base + 1]));
+        tc += a;
+        nc++;
+      }
+      // step time: momentum start to continuity end of a sampled split
+      // step (momentum kernel alone when fused)
+      for (size_t j = 0; j < c.mom.size(); j++) {
+        const bool pair = !ctx->fused || ctx->sh_mode != ODIS_SH_OFF;
+        const int end = pair && j < c.cont.size() ? c.cont[j] : c.mom[j];
+        float a = 0;
+        CK(hipEventElapsedTime(&a, ctx->ev[c.mom[j]], ctx->ev[end + 1]));
+        ts += a;
+        ns++;
+      }
     }
-    ctx->t_face = tf / sampled.size();
-    ctx->t_cell = tc / sampled.size();
-    ctx->t_step = ts / sampled.size();
+    ctx->t_face = nm ? tm / nm : 0.0;
+    ctx->t_cell = nc ? tc / nc : 0.0;
+    ctx->t_step = ns ? ts / ns : 0.0;
   }
   return ODIS_OK;
 }
@@ -1195,6 +1447,17 @@ extern "C" int odis_use_graphs(odis_ctx *ctx, int on) {
   return ODIS_OK;
 }
 
+extern "C" int odis_use_fused(odis_ctx *ctx, int on) {
+  if (!ctx) return ODIS_E_INVALID;
+  const int f = on ? 1 : 0;
+  if (f != ctx->fused) {
+    CK(hipStreamSynchronize(ctx->stream));
+    drop_graphs(ctx);
+    ctx->fused = f;
+  }
+  return ODIS_OK;
+}
+
 extern "C" int odis_get_timing(odis_ctx *ctx, double *face_ms, double *cell_ms, double *step_ms) {
   if (!ctx) return ODIS_E_INVALID;
   if (face_ms) *face_ms = ctx->t_face;
@@ -1254,10 +1517,10 @@ extern "C" int odis_interp_ll(odis_ctx *ctx, int inst, double *ll) {
   const dim3 grid((ctx->ll_rows + kBlock / 64 - 1) / (kBlock / 64)), blk(kBlock);
   if (ctx->precision == ODIS_FP64)
     hipLaunchKernelGGL((k_interp_ll<double>), grid, blk, 0, ctx->stream, ctx->ll_rows, ctx->ll_rp, ctx->ll_col,
-                       ctx->ll_val, ctx->d.eta + size_t(inst) * ctx->N, ctx->ll_out);
+                       ctx->ll_val, ctx->d.eta_buf[ctx->ecur] + size_t(inst) * ctx->N, ctx->ll_out);
   else
     hipLaunchKernelGGL((k_interp_ll<float>), grid, blk, 0, ctx->stream, ctx->ll_rows, ctx->ll_rp, ctx->ll_col,
-                       ctx->ll_val, ctx->f.eta + size_t(inst) * ctx->N, ctx->ll_out);
+                       ctx->ll_val, ctx->f.eta_buf[ctx->ecur] + size_t(inst) * ctx->N, ctx->ll_out);
   CK(hipGetLastError());
   CK(hipMemcpyAsync(ll, ctx->ll_out, sizeof(double) * ctx->ll_rows, hipMemcpyDeviceToHost, ctx->stream));
   CK(hipStreamSynchronize(ctx->stream));

@@ -174,14 +174,23 @@ int odis_sh_synthesize(odis_ctx *ctx, const double *coeffs, double *field);
  * odis_step returns ODIS_E_STOPPED at the next step boundary. */
 int odis_request_stop(odis_ctx *ctx);
 
-/* odis_step runs chunks of 64 steps as captured HIP graphs (on by default);
- * off launches every kernel from the host (same results bit for bit). */
+/* Launch controls (no counterpart in the reference; results are bit for bit
+ * the same in every combination):
+ *  - odis_use_graphs: run chunks of 64 steps as captured HIP graphs (off by
+ *    default; direct launches measured faster on MI355X);
+ *  - odis_use_fused: one kernel per step that completes the previous step's
+ *    continuity update before the momentum update (off by default: measured
+ *    slower than the face + cell kernel pair; ignored while an SH basis is
+ *    set). */
 int odis_use_graphs(odis_ctx *ctx, int on);
-/* Timing for benchmarks: with stride T > 0, every T-th chunk of the next
- * odis_step calls is launched kernel by kernel with dispatch-stamped
+int odis_use_fused(odis_ctx *ctx, int on);
+/* Timing for benchmarks: with stride T > 0, the kernels of every step n of
+ * the next odis_step calls with n % T == 0 are launched with dispatch-stamped
  * start/stop HIP events on the context stream; odis_get_timing returns the
- * average face-kernel, cell-kernel and face-start-to-cell-end times (ms) of
- * the sampled steps of the last odis_step call.  T = 0 disables. */
+ * average duration (ms) of the momentum kernel (the face kernel, or the fused
+ * step kernel), of the continuity kernel (the cell kernel) and of a whole
+ * sampled step, over the sampled steps of the last odis_step call.  T = 0
+ * disables. */
 int odis_enable_timing(odis_ctx *ctx, int stride);
 int odis_get_timing(odis_ctx *ctx, double *face_ms, double *cell_ms, double *step_ms);
 

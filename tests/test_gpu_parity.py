@@ -363,6 +363,7 @@ def test_graph_replays_match_direct_launches(product, oracle):
     mesh = G.Mesh(g, p.radius)
     a = G.Context(mesh, p)
     b = G.Context(mesh, p)
+    a.use_graphs(True)
     b.use_graphs(False)
     for ctx in (a, b):
         ctx.set_sh(6, np.linspace(1.0, 1.3, 7), mode=G.SH_FP64)
@@ -384,3 +385,61 @@ def test_graph_replays_match_direct_launches(product, oracle):
     sim.set_projector(Y * dfac[None, :], np.linalg.pinv(Y))
     sim.step(331)
     _compare(a, sim, TOL_SHORT)
+
+
+def _assert_same_state(a, b, insts, where):
+    for i in insts:
+        sa, sb = a.get_state(i), b.get_state(i)
+        assert sa["iter"] == sb["iter"]
+        for k in ("v", "eta", "dvdt", "detadt"):
+            assert np.array_equal(sa[k], sb[k]), (where, i, k)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_fused_step_matches_two_kernel_step(product, oracle, precision):
+    """The one-kernel step (continuity of step n-1 completed inside the
+    momentum kernel of step n) gives the same state, dissipation and
+    dissipation series bit for bit as the face + cell kernel pair: two
+    instances with different thickness and drag, linear and quadratic drag,
+    chunk boundaries (chunks hold up to 4096 steps), graph replays and a
+    restart from a loaded history.  In FP64 it also matches the oracle."""
+    G = product
+    g, _ = generated_grid(4)
+    for kw in (dict(), dict(friction=G.QUADRATIC, alpha=2e-3, quad_drag=1, ecc=0.05)):
+        ps = []
+        for h, fa in ((38e3, 1.0), (20e3, 3.0)):
+            p, _ = G.make_params(tide=G.ECC, h=h, **kw)
+            p.alpha *= fa
+            ps.append(p)
+        mesh = G.Mesh(g, ps[0].radius)
+        fused = G.Context(mesh, ps, precision=precision)
+        split = G.Context(mesh, ps, precision=precision)
+        fused.use_fused(True)
+        split.use_fused(False)
+        for n in (1, 1, 2, 3, 64, 4100, 7):
+            fused.step(n)
+            split.step(n)
+            _assert_same_state(fused, split, (0, 1), (kw, n))
+            assert np.array_equal(fused.dissipation(), split.dissipation())
+        last = fused.get_state(0)["iter"]
+        for b in (0, 1):
+            sa = fused.dissipation_series(b, last - 4000, 4000)
+            sb = split.dissipation_series(b, last - 4000, 4000)
+            assert np.array_equal(sa, sb)
+        fused.use_graphs(True)
+        for n in (64, 64, 65, 128):
+            fused.step(n)
+            split.step(n)
+            _assert_same_state(fused, split, (0, 1), (kw, "graph", n))
+        st = split.get_state(1)
+        for c in (fused, split):
+            c.set_state(0, st["v"], st["dvdt"], st["eta"], st["detadt"], st["iter"])
+            c.step(37)
+        _assert_same_state(fused, split, (0, 1), (kw, "load"))
+    if precision == 0:
+        ctx, sim, _, _, _ = _pair(product, oracle, 4, dict(tide=product.ECC))
+        ctx.use_fused(True)
+        for n in (1, 2, 300):
+            ctx.step(n)
+            sim.step(n)
+            _compare(ctx, sim, TOL_SHORT)
