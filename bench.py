@@ -12,8 +12,8 @@ omega_max = 0.078 s^-1 on this grid, so 8 s steps diverge.  A "step" is one
 pass of the loop body over all cells and faces.
 
 Prints one JSON line: cell-steps/s of the whole job, the roofline of the
-dominant kernel (the face kernel, or the fused step kernel with --fused;
-HIP-event timed on every 16th step inside the timed region) and a
+dominant kernel (the one-kernel step, or the longer of the face and cell
+kernels; HIP-event timed on every 16th step inside the timed region) and a
 CPU baseline (the FP64 C oracle, OpenMP, on a bounded sample of the same
 workload).
 """
@@ -29,26 +29,40 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def face_bytes(E, N, nfields, p):
+def face_bytes(E, N, nfields, p, compact):
     """Algorithmic bytes of one face-kernel launch: every operand and state
-    array touched once (p = bytes per value, int32 indices)."""
-    per_face = 10 * 4 + 10 * p + 2 * 4 + 2 * p + p + p + nfields * p  # operands
-    per_face += p + 2 * p + p + p  # v read, 2 history reads, history write, v write
-    return E * per_face + N * p     # + eta read once
+    array touched once (p = bytes per value, int32 indices).
+    Per face, both forms: 2 gradient indices and coefficients, Coriolis,
+    area; v read, 2 history reads, history write, v write; eta once per cell.
+    Tabulated: 10 TRiSK indices and coefficients and nfields forcing
+    gradients per face.
+    Compact: d_e and the 2 TRiSK partial sums per face, nfields forcing
+    fields per cell."""
+    common = E * (2 * 4 + 2 * p + p + p + p + 2 * p + p + p) + N * p
+    if compact:
+        return common + E * (p + 2 * p) + N * nfields * p
+    return common + E * (10 * 4 + 10 * p + nfields * p)
 
 
-def cell_bytes(E, N, p):
-    per_cell = 6 * 4 + 6 * p + p + p + 2 * p + p  # operands, eta r/w, history r2/w1
-    return N * per_cell + E * p                   # + new v read once
+def cell_bytes(E, N, p, compact):
+    """Algorithmic bytes of one cell-kernel launch.
+    Tabulated: 6 face indices and divergence coefficients, eta read and
+    write, 2 history reads and 1 write per cell; the new v once per face.
+    Compact: face cycle (6 indices), 6 kite fractions, cycle info, area, the
+    same eta and history traffic per cell; l_e and the new v read and the 2
+    TRiSK partial sums written once per face."""
+    if compact:
+        return N * (6 * 4 + 6 * p + 4 + p + 2 * p + 3 * p) + E * (p + p + 2 * p)
+    return N * (6 * 4 + 6 * p + 2 * p + 3 * p) + E * p
 
 
-def fused_bytes(E, N, nfields, p):
-    """Algorithmic bytes of one fused step-kernel launch: the face kernel's
-    set plus the continuity operands and state of the cells (eta^{n-1} read,
-    eta^n written, two history reads and one write) and the owner byte per
-    face; v is read once for both updates."""
-    per_cell = 6 * 4 + 6 * p + p + 2 * p + p
-    return face_bytes(E, N, nfields, p) + N * per_cell + E * 1
+def step_kernel_bytes(E, N, nfields, p):
+    """Algorithmic bytes of one launch of the one-kernel step: the compact
+    face update (plus l_e per face for the patch store), then per cell the
+    patch-local face cycle (6 uint16), 6 kite fractions, cycle info, area,
+    eta written and the history (2 reads, 1 write), and per face the 2 TRiSK
+    partial sums written (eta^n is counted once, in the face update)."""
+    return face_bytes(E, N, nfields, p, True) + E * p + N * (6 * 2 + 6 * p + 4 + p + p + 3 * p) + E * 2 * p
 
 
 def measured_traffic(kname, tname, n_cells):
@@ -77,9 +91,18 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU work of the baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--fused", action="store_true",
-                    help="one fused step kernel per step instead of the face + cell kernel pair")
+    ap.add_argument("--tabulated", action="store_true",
+                    help="tabulated TRiSK operator (10 indices and coefficients per face) instead of the "
+                         "cell-local form")
+    ap.add_argument("--face-cell", action="store_true",
+                    help="a face and a cell kernel per step instead of the one-kernel step")
     ap.add_argument("--graphs", action="store_true", help="replay 64-step chunks as HIP graphs")
+    ap.add_argument("--patch-cells", type=int, default=0,
+                    help="cells per patch of the one-kernel step (64, 128, 256; 0: library default)")
+    ap.add_argument("--step-lanes", type=int, default=0,
+                    help="lanes per cell of the one-kernel step (2 or 4; 0: library default)")
+    ap.add_argument("--step-blocks", type=int, default=0,
+                    help="workgroups of the one-kernel step (0: library default, one per patch)")
     ap.add_argument("--timing-stride", type=int, default=16,
                     help="dispatch-stamped HIP events on the kernels of every T-th step (0: none, "
                          "roofline achieved then null)")
@@ -111,7 +134,15 @@ def main():
     prec = G.FP64 if args.precision == "fp64" else G.FP32
     pbytes = 8 if prec == G.FP64 else 4
     ctx = G.Context(mesh, p, precision=prec)
-    ctx.use_fused(args.fused)
+    ctx.use_compact(not args.tabulated)
+    one = not args.tabulated and not args.face_cell
+    ctx.use_one_kernel(one)
+    if one and args.patch_cells:
+        ctx.set_option(G.OPT_PATCH_CELLS, args.patch_cells)
+    if one and args.step_lanes:
+        ctx.set_option(G.OPT_STEP_LANES, args.step_lanes)
+    if one and args.step_blocks:
+        ctx.set_option(G.OPT_STEP_BLOCKS, args.step_blocks)
     ctx.use_graphs(args.graphs)
     print("bench: grid %d cells / %d faces ready in %.1fs" % (mesh.N, mesh.E, time.time() - t0),
           file=sys.stderr, flush=True)
@@ -130,15 +161,19 @@ def main():
     value = mesh.N * args.steps / elapsed
 
     nfields = 2
-    cb = cell_bytes(mesh.E, mesh.N, pbytes)
-    if not args.fused:
-        fb = face_bytes(mesh.E, mesh.N, nfields, pbytes)
-        kname, step_bytes = "k_face", fb + cb
+    compact = not args.tabulated
+    fb = face_bytes(mesh.E, mesh.N, nfields, pbytes, compact)
+    cb = cell_bytes(mesh.E, mesh.N, pbytes, compact)
+    tname = ("double" if prec == G.FP64 else "float") + (",compact" if compact else ",tabulated")
+    if one:
+        sb = step_kernel_bytes(mesh.E, mesh.N, nfields, pbytes)
+        fb, cb = sb, 0
+        kname, kbytes, kms = "k_step", sb, tm["face_ms"]
+    elif tm["cell_ms"] > tm["face_ms"]:  # dominant kernel: the longer of the two per step
+        kname, kbytes, kms = "k_cell", cb, tm["cell_ms"]
     else:
-        fb = fused_bytes(mesh.E, mesh.N, nfields, pbytes)
-        kname, step_bytes = "k_fused", fb
-    achieved = fb / (tm["face_ms"] * 1e-3) / 1e9 if tm["face_ms"] > 0 else None
-    tname = "double" if prec == G.FP64 else "float"
+        kname, kbytes, kms = "k_face", fb, tm["face_ms"]
+    achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else None
     traffic = measured_traffic(kname, tname, mesh.N)
     out = {
         "metric": "cell-steps/s at level 7 (163842 cells)",
@@ -159,11 +194,15 @@ def main():
             "cells": mesh.N, "faces": mesh.E, "instances": 1,
             "grid": "icosahedral bisection + 64 Lloyd sweeps, reference level %d" % args.level,
             "dt_s": p.dt, "steps_per_orbit": steps_per_orbit,
-            "launch": "fused step kernel, k_cell once per 4096-step chunk" if args.fused else
-                      "face kernel + cell kernel per step",
+            "trisk": "cell-local (kite fractions)" if compact else "tabulated",
+            "launch": ("one kernel per step (patches of <= %d cells, %d lanes per cell, %d workgroups)"
+                       % (ctx.option(G.OPT_PATCH_CELLS), ctx.option(G.OPT_STEP_LANES),
+                          ctx.option(G.OPT_STEP_BLOCKS))) if one
+                      else "face + cell kernel per step",
             "graphs": bool(args.graphs),
-            "kernel_ms": {"momentum": tm["face_ms"], "cell": tm["cell_ms"], "step_events": tm["step_ms"]},
-            "bytes_per_step_alg": step_bytes,
+            "kernel_ms": {"face": tm["face_ms"], "cell": tm["cell_ms"], "step_events": tm["step_ms"]},
+            "alg_bytes": {"face": fb, "cell": cb, "step": fb + cb},
+            "hbm_gbs_step_events": (fb + cb) / (tm["step_ms"] * 1e-3) / 1e9 if tm["step_ms"] > 0 else None,
             "dissipation_W_m2": diss,
         },
         "roofline": {
@@ -174,7 +213,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS if achieved else None,
             "traffic": traffic,
             "kernel": "%s<%s>" % (kname, tname),
-            "alg_bytes_per_launch": fb,
+            "alg_bytes_per_launch": kbytes,
         },
     }
     ctx.close()

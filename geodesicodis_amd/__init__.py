@@ -36,9 +36,11 @@ EXPORTS = [
     "odis_mesh_sizes", "odis_mesh_describe", "odis_mesh_get", "odis_mesh_free",
     "odis_round_period", "odis_quantise_dt", "odis_config_read", "odis_host_last_error",
     "odis_run_case", "odis_interp_ll_set", "odis_interp_ll", "odis_sh_set_basis",
-    "odis_sh_project", "odis_sh_synthesize", "odis_surface_beta", "odis_use_fused", "odis_use_graphs",
+    "odis_sh_project", "odis_sh_synthesize", "odis_surface_beta", "odis_set_option",
+    "odis_get_option", "odis_mesh_compact_check",
 ]
 SH_OFF, SH_FP64, SH_SPLIT_BF16 = 0, 1, 2
+OPT_GRAPHS, OPT_COMPACT, OPT_ONE_KERNEL, OPT_PATCH_CELLS, OPT_STEP_BLOCKS, OPT_STEP_LANES = 1, 2, 3, 4, 5, 6
 
 
 class OdisError(RuntimeError):
@@ -66,7 +68,8 @@ class MeshDesc(ctypes.Structure):
                 ("trisk_coef", ctypes.c_void_p), ("face_lat", ctypes.c_void_p),
                 ("face_area", ctypes.c_void_p), ("cell_faces", ctypes.c_void_p),
                 ("div_coef", ctypes.c_void_p), ("cell_lat", ctypes.c_void_p),
-                ("cell_lon", ctypes.c_void_p)]
+                ("cell_lon", ctypes.c_void_p), ("face_len", ctypes.c_void_p),
+                ("face_dist", ctypes.c_void_p), ("cell_area", ctypes.c_void_p)]
 
 
 _lib = None
@@ -99,8 +102,9 @@ def lib():
             "odis_get_velocity_en": ([vp, ctypes.c_int, dp, dp], ctypes.c_int),
             "odis_request_stop": ([vp], ctypes.c_int),
             "odis_enable_timing": ([vp, ctypes.c_int], ctypes.c_int),
-            "odis_use_graphs": ([vp, ctypes.c_int], ctypes.c_int),
-            "odis_use_fused": ([vp, ctypes.c_int], ctypes.c_int),
+            "odis_set_option": ([vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+            "odis_get_option": ([vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+            "odis_mesh_compact_check": ([vp, vp, vp, vp], ctypes.c_int),
             "odis_get_timing": ([vp, dp, dp, dp], ctypes.c_int),
             "odis_grid_read": ([ctypes.c_char_p, vpp], ctypes.c_int),
             "odis_grid_generate": ([ctypes.c_int, ctypes.c_int, vpp], ctypes.c_int),
@@ -227,6 +231,15 @@ class Mesh:
         d = MeshDesc()
         lib().odis_mesh_describe(self._h, ctypes.byref(d))
         return d
+
+    def compact_check(self, desc=None):
+        """(ok, reason, coef_err, div_err, area_err) of the cell-local TRiSK form
+        (odis_mesh_compact_check)."""
+        d = desc if desc is not None else self.describe()
+        errs = [ctypes.c_double() for _ in range(3)]
+        rc = lib().odis_mesh_compact_check(ctypes.byref(d), *[ctypes.byref(e) for e in errs])
+        reason = "" if rc == 0 else lib().odis_host_last_error().decode()
+        return (rc == 0, reason) + tuple(e.value for e in errs)
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -386,12 +399,27 @@ class Context:
         """Sample every stride-th chunk of steps with HIP events (0 = off)."""
         self._check(lib().odis_enable_timing(self._h, int(stride)))
 
-    def use_graphs(self, on=True):
-        self._check(lib().odis_use_graphs(self._h, 1 if on else 0))
+    def set_option(self, option, value):
+        self._check(lib().odis_set_option(self._h, int(option), int(value)))
 
-    def use_fused(self, on=True):
-        """One kernel per step (default) or a face and a cell kernel per step."""
-        self._check(lib().odis_use_fused(self._h, 1 if on else 0))
+    def option(self, option):
+        v = ctypes.c_int()
+        self._check(lib().odis_get_option(self._h, int(option), ctypes.byref(v)))
+        return v.value
+
+    def use_graphs(self, on=True):
+        """Replay 64-step chunks from captured HIP graphs (ODIS_OPT_GRAPHS)."""
+        self.set_option(OPT_GRAPHS, 1 if on else 0)
+
+    def use_compact(self, on=True):
+        """Cell-local TRiSK form (default when available) or the tabulated
+        operator (ODIS_OPT_COMPACT)."""
+        self.set_option(OPT_COMPACT, 1 if on else 0)
+
+    def use_one_kernel(self, on=True):
+        """One kernel per step over cell patches (default with the compact
+        form) or a face and a cell kernel per step (ODIS_OPT_ONE_KERNEL)."""
+        self.set_option(OPT_ONE_KERNEL, 1 if on else 0)
 
     def timing(self):
         a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()

@@ -131,6 +131,26 @@ extern "C" int odis_mesh_describe(const odis_mesh *mm, odis_mesh_desc *d) {
   d->div_coef = m.div_coef.data();
   d->cell_lat = m.lat.data();
   d->cell_lon = m.lon.data();
+  d->face_len = m.face_len.data();
+  d->face_dist = m.face_node_dist.data();
+  d->cell_area = m.cv_area.data();
+  return ODIS_OK;
+}
+
+extern "C" int odis_mesh_compact_check(const odis_mesh_desc *d, double *coef_err, double *div_err,
+                                       double *area_err) {
+  if (!d) return ODIS_E_INVALID;
+  odis::CompactTrisk c;
+  std::string why;
+  try {
+    why = odis::build_compact(*d, c);
+  } catch (const std::exception &ex) {
+    why = ex.what();
+  }
+  if (coef_err) *coef_err = c.coef_err;
+  if (div_err) *div_err = c.div_err;
+  if (area_err) *area_err = c.area_err;
+  if (!why.empty()) return host_fail(ODIS_E_UNSUPPORTED, why);
   return ODIS_OK;
 }
 

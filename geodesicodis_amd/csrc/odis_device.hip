@@ -22,16 +22,105 @@ namespace odis {
 // kernels
 // ===========================================================================
 
-template <typename T>
+// Fixed-shape block reduction (NW waves of 64): wave-level shuffle, then
+// the wave sums in a fixed tree.  The result is valid on thread 0.
+template <int NW>
 __device__ __forceinline__ double block_sum(double x, double *sh) {
-  // fixed-shape tree: wave-level shuffle then LDS across the 4 waves
   for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (lane == 0) sh[wave] = x;
   __syncthreads();
   double r = 0.0;
-  if (threadIdx.x == 0) r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  if (threadIdx.x == 0) {
+    if (NW == 1) r = sh[0];
+    else if (NW == 2) r = sh[0] + sh[1];
+    else if (NW == 4) r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    else r = ((sh[0] + sh[1]) + (sh[2] + sh[3])) + ((sh[4] + sh[5]) + (sh[6] + sh[7]));
+  }
   return r;
+}
+
+// Sum count partials in a fixed order (one workgroup of NW waves).
+template <int NW>
+__device__ __forceinline__ double sum_partials(const double *p, int count, double *sh) {
+  double acc = 0.0;
+  for (int j = threadIdx.x; j < count; j += NW * 64) acc += p[j];
+  return block_sum<NW>(acc, sh);
+}
+
+// Result of the momentum update of one face.
+template <typename T>
+struct FaceOut {
+  T vn;           // v^{n+1}
+  T f0;           // momentum tendency (history slot n % 3)
+  double energy;  // dissipation integrand of state n: A_e (v_e^2 + v_t^2) (or ^{3/2})
+};
+
+// Tangential velocity, dissipation integrand and (unless DISS_ONLY) the AB3
+// momentum update of face e of instance b:
+//   f0 = -g (G eta)_e + fcor_e v_t [- c |u| v_e]   (src/updateMomentum.cpp:42)
+//   v' = v + dt AB3(f0) + dt (-alpha v_e + (G U)_e)   (src/timeIntegrator.cpp:207-232)
+// COMPACT: v_t from the TRiSK partial sums and G U from the forcing fields at
+// the cells; else the tabulated stencil and per-face forcing gradients.
+template <typename T, bool DISS_ONLY, bool COMPACT>
+__device__ __forceinline__ FaceOut<T> face_update(const FaceArgs<T> &a, int e, int b, int energy_quad,
+                                                  const double *table, int s) {
+  const int E = a.E;
+  const double *ip = a.inst + b * 8;
+  const T *v = a.v_cur + b * a.v_stride;
+  FaceOut<T> out;
+  T vt;
+  if (COMPACT) {
+    const T *ts = a.ts + b * a.ts_stride;
+    vt = (ts[e] - ts[E + e]) / a.fdist[e];
+  } else {
+    vt = T(0);
+#pragma unroll
+    for (int j = 0; j < 10; ++j) vt += a.tri_coef[j * E + e] * v[a.tri_idx[j * E + e]];
+  }
+  const T ve = v[e];
+  const T sp = ve * ve + vt * vt;
+  const T en = energy_quad ? sqrt(sp) * sp : sp;
+  out.energy = double(a.farea[e]) * double(en);
+  out.vn = T(0);
+  out.f0 = T(0);
+  if (!DISS_ONLY) {
+    const long long n = (long long)table[0] + s;
+    const double *coef = table + 8 + size_t(s) * gridDim.y * 8;
+    const T g = T(ip[0]), alpha = T(ip[1]), dt = T(ip[2]), qd = T(ip[4]);
+    const bool ab3 = (n >= 2) || (ip[3] != 0.0);
+    const T *eta = a.eta + b * a.eta_stride;
+    const int c0 = a.g_idx[e], c1 = a.g_idx[E + e];
+    const T gc0 = a.g_coef[e], gc1 = a.g_coef[E + e];
+    const T ge = gc0 * eta[c0] + gc1 * eta[c1];
+    T f0 = -g * ge + a.fcor[e] * vt;
+    if (a.quad_drag) f0 -= qd * sqrt(sp) * ve;
+    // G U, U = sum_k c_k(t + dt) S_k (src/timeIntegrator.cpp:219)
+    T fu;
+    if (COMPACT) {
+      T u0 = T(0), u1 = T(0);
+      for (int k = 0; k < a.nfields; ++k) {
+        const T ck = T(coef[b * 8 + k]);
+        u0 += ck * a.cellS[k * a.N + c0];
+        u1 += ck * a.cellS[k * a.N + c1];
+      }
+      fu = gc0 * u0 + gc1 * u1;
+    } else {
+      fu = T(0);
+      for (int k = 0; k < a.nfields; ++k) fu += T(coef[b * 8 + k]) * a.fgs[k * E + e];
+    }
+    const T drag = -alpha * ve + fu;
+    const T *ring = a.ring + b * a.ring_stride;
+    const int s1 = int((n + 2) % 3), s2 = int((n + 1) % 3);
+    T upd;
+    if (ab3)
+      upd = (T(23. / 12.) * f0 + T(-16. / 12.) * ring[s1 * E + e] + T(5. / 12.) * ring[s2 * E + e]) * dt;
+    else
+      upd = f0 * dt;
+    out.f0 = f0;
+    out.vn = (ve + upd) + dt * drag;
+  }
+  return out;
 }
 
 // Face kernel.  DISS_ONLY: evaluate the dissipation partials of the current
@@ -40,99 +129,298 @@ __device__ __forceinline__ double block_sum(double x, double *sh) {
 // table[0] = n0, then per step s and instance b 8 coefficients at
 // table[8 + (s * B + b) * 8]; step s of the table is n = n0 + s.  Kernels
 // captured in a graph therefore stay valid for every chunk of steps.
-template <typename T, bool DISS_ONLY>
+template <typename T, bool DISS_ONLY, bool COMPACT>
 __global__ __launch_bounds__(kBlock) void k_face(FaceArgs<T> a, int energy_quad, const double *table, int s) {
   __shared__ double sh[4];
   const int b = blockIdx.y;
   const int tile = xcd_tile(blockIdx.x, a.nb);
   const int E = a.E;
   const int e = tile * kBlock + threadIdx.x;
-  const double *ip = a.inst + b * 8;
   double local = 0.0;
   if (e < E) {
-    const T *v = a.v_cur + b * a.v_stride;
-    T vt = T(0);
-#pragma unroll
-    for (int j = 0; j < 10; ++j) vt += a.tri_coef[j * E + e] * v[a.tri_idx[j * E + e]];
-    const T ve = v[e];
-    const T sp = ve * ve + vt * vt;
-    const T en = energy_quad ? sqrt(sp) * sp : sp;
-    local = double(a.farea[e]) * double(en);
+    const FaceOut<T> r = face_update<T, DISS_ONLY, COMPACT>(a, e, b, energy_quad, table, s);
+    local = r.energy;
     if (!DISS_ONLY) {
       const long long n = (long long)table[0] + s;
-      const double *coef = table + 8 + size_t(s) * gridDim.y * 8;
-      const T g = T(ip[0]), alpha = T(ip[1]), dt = T(ip[2]), qd = T(ip[4]);
-      const bool ab3 = (n >= 2) || (ip[3] != 0.0);
-      const T *eta = a.eta + b * a.eta_stride;
-      const T ge = a.g_coef[e] * eta[a.g_idx[e]] + a.g_coef[E + e] * eta[a.g_idx[E + e]];
-      T f0 = -g * ge + a.fcor[e] * vt;
-      if (a.quad_drag) f0 -= qd * sqrt(sp) * ve;
-      T fu = T(0);
-      for (int k = 0; k < a.nfields; ++k) fu += T(coef[b * 8 + k]) * a.fgs[k * E + e];
-      const T drag = -alpha * ve + fu;
-      T *ring = a.ring + b * a.ring_stride;
-      const int s0 = int(n % 3), s1 = int((n + 2) % 3), s2 = int((n + 1) % 3);
-      T upd;
-      if (ab3)
-        upd = (T(23. / 12.) * f0 + T(-16. / 12.) * ring[s1 * E + e] + T(5. / 12.) * ring[s2 * E + e]) * dt;
-      else
-        upd = f0 * dt;
-      ring[s0 * E + e] = f0;
-      a.v_nxt[b * a.v_stride + e] = (ve + upd) + dt * drag;
+      a.ring[b * a.ring_stride + int(n % 3) * E + e] = r.f0;
+      a.v_nxt[b * a.v_stride + e] = r.vn;
     }
   }
-  const double part = block_sum<T>(local, sh);
+  const double part = block_sum<kBlock / 64>(local, sh);
   const long long par = DISS_ONLY ? 0 : (((long long)table[0] + s) & 1);
   if (threadIdx.x == 0) a.partials[(par * gridDim.y + b) * a.nb + tile] = part;
 }
 
-// Sum the face kernel's partials of instance b in a fixed order.
-__device__ __forceinline__ double sum_partials(const double *p, int count, double *sh) {
-  double acc = 0.0;
-  for (int j = threadIdx.x; j < count; j += kBlock) acc += p[j];
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) sh[wave] = acc;
-  __syncthreads();
-  return (sh[0] + sh[1]) + (sh[2] + sh[3]);
+// TRiSK partial sums of a cell with K faces: for each face p of the cycle,
+// T_p = sum_{j=1}^{K-1} (R_p + ... + R_{p+j-1} - 1/2) F_{p+j}, F_q = dir_q l_q v_q
+// (indices mod K).  The running sum of kite fractions starts at the face
+// itself, in the order of the reference's weight loop (src/mesh.cpp:677-910).
+template <typename T, int K>
+__device__ __forceinline__ void trisk_partials(const T *F, const T *R, const int *f, int db, T *ts, int E) {
+#pragma unroll
+  for (int p = 0; p < K; ++p) {
+    T S = R[p];
+    T acc = (S - T(0.5)) * F[(p + 1) % K];
+#pragma unroll
+    for (int j = 2; j < K; ++j) {
+      S += R[(p + j - 1) % K];
+      acc += (S - T(0.5)) * F[(p + j) % K];
+    }
+    ts[((db >> p) & 1) * E + f[p]] = acc;
+  }
+}
+
+// TRiSK partial sums of a cell with k faces.
+template <typename T>
+__device__ __forceinline__ void cell_outputs(const CellArgs<T> &a, int b, int k, int db, const int *f, const T *F,
+                                             const T *R) {
+  T *ts = a.ts + b * a.ts_stride;
+  if (k == 6)
+    trisk_partials<T, 6>(F, R, f, db, ts, a.E);
+  else
+    trisk_partials<T, 5>(F, R, f, db, ts, a.E);
+}
+
+// Continuity + AB3 of cell i: eta^{n+1} = eta^n + dt AB3(h Div v^{n+1})
+// (src/updateEta.cpp, src/temporalOperators.cpp:36-47), history slot n % 3.
+// eta_old = eta^n, r1 / r2 = history slots (n+2) % 3 and (n+1) % 3, already
+// loaded.
+template <typename T>
+__device__ __forceinline__ T continuity_loaded(const CellArgs<T> &a, int i, int b, long long n, T div,
+                                               T eta_old, T r1, T r2) {
+  const double *ip = a.inst + b * 8;
+  const T h = T(ip[5]), dt = T(ip[2]);
+  const bool ab3 = (n >= 2) || (ip[3] != 0.0);
+  const T f0 = h * div;
+  T upd;
+  if (ab3)
+    upd = (T(23. / 12.) * f0 + T(-16. / 12.) * r1 + T(5. / 12.) * r2) * dt;
+  else
+    upd = f0 * dt;
+  a.ring[b * a.ring_stride + int(n % 3) * a.N + i] = f0;
+  const T eta_new = eta_old + upd;
+  a.eta_nxt[b * a.eta_stride + i] = eta_new;
+  return eta_new;
 }
 
 template <typename T>
+__device__ __forceinline__ T continuity(const CellArgs<T> &a, int i, int b, long long n, T div) {
+  const T *ring = a.ring + b * a.ring_stride;
+  const int N = a.N;
+  return continuity_loaded(a, i, b, n, div, a.eta[b * a.eta_stride + i], ring[int((n + 2) % 3) * N + i],
+                           ring[int((n + 1) % 3) * N + i]);
+}
+
+// Divergence sum_q -dir_q l_q v_q / A_i (src/mesh.cpp:3242-3247) from the
+// signed fluxes F_q = dir_q l_q v_q.
+template <typename T>
+__device__ __forceinline__ T divergence(const T *F, int k, T area) {
+  T sum = F[0];
+  sum += F[1];
+  sum += F[2];
+  sum += F[3];
+  sum += F[4];
+  if (k == 6) sum += F[5];
+  return -sum / area;
+}
+
+// Compact-form cell update from the cell's signed fluxes F_q = dir_q l_q v_q
+// (global face ids f): divergence, continuity (unless PREP), then the TRiSK
+// partial sums of the cell's faces.
+template <typename T, bool PREP>
+__device__ __forceinline__ void cell_compact(const CellArgs<T> &a, int i, int b, long long n, int k, int db,
+                                             const int *f, const T *F, const T *R) {
+  if (!PREP) continuity(a, i, b, n, divergence(F, k, a.area[i]));
+  cell_outputs(a, b, k, db, f, F, R);
+}
+
+// Cell kernel: continuity + AB3 into eta^{n+1}; the first workgroup of each
+// instance also reduces the face kernel's dissipation partials of this step.
+// COMPACT: divergence from the cell's face cycle, and the TRiSK partial sums
+// of v_new for the next face kernel.  PREP (COMPACT only): the partial sums
+// alone, for a velocity just uploaded.
+template <typename T, bool COMPACT, bool PREP>
 __global__ __launch_bounds__(kBlock) void k_cell(CellArgs<T> a, const double *table, int s_off) {
   __shared__ double sh[4];
-  const long long n = (long long)table[0] + s_off;
   const int b = blockIdx.y;
   const int tile = xcd_tile(blockIdx.x, a.nb);
   const int N = a.N;
   const int i = tile * kBlock + threadIdx.x;
-  const double *ip = a.inst + b * 8;
+  const long long n = PREP ? 0 : (long long)table[0] + s_off;
   if (i < N) {
     const T *v = a.v_new + b * a.v_stride;
-    T s = T(0);
+    if (COMPACT) {
+      const int info = a.info[i];
+      int f[6];
+      T F[6], R[6];
 #pragma unroll
-    for (int j = 0; j < 6; ++j) s += a.div_coef[j * N + i] * v[a.div_idx[j * N + i]];
-    const T h = T(ip[5]), dt = T(ip[2]);
-    const bool ab3 = (n >= 2) || (ip[3] != 0.0);
-    const T f0 = h * s;
-    T *ring = a.ring + b * a.ring_stride;
-    const int s0 = int(n % 3), s1 = int((n + 2) % 3), s2 = int((n + 1) % 3);
-    T upd;
-    if (ab3)
-      upd = (T(23. / 12.) * f0 + T(-16. / 12.) * ring[s1 * N + i] + T(5. / 12.) * ring[s2 * N + i]) * dt;
-    else
-      upd = f0 * dt;
-    ring[s0 * N + i] = f0;
-    T *eta = a.eta + b * a.eta_stride;
-    eta[i] = eta[i] + upd;
+      for (int q = 0; q < 6; ++q) {
+        f[q] = a.cyc[q * N + i];
+        R[q] = a.kite[q * N + i];
+        const T x = a.flen[f[q]] * v[f[q]];
+        F[q] = ((info >> (3 + q)) & 1) ? -x : x;
+      }
+      cell_compact<T, PREP>(a, i, b, n, info & 7, info >> 3, f, F, R);
+    } else {
+      T div = T(0);
+#pragma unroll
+      for (int j = 0; j < 6; ++j) div += a.div_coef[j * N + i] * v[a.div_idx[j * N + i]];
+      continuity(a, i, b, n, div);
+    }
   }
+  if (PREP || blockIdx.x != 0) return;
   // the first-dispatched workgroup of each instance reduces the face
   // kernel's dissipation partials of this step (complete: previous kernel)
-  if (blockIdx.x != 0) return;
-  const double tot = sum_partials(a.partials + ((n & 1) * gridDim.y + b) * size_t(a.nb_face), a.nb_face, sh);
+  const double tot =
+      sum_partials<kBlock / 64>(a.partials + ((n & 1) * gridDim.y + b) * size_t(a.nb_face), a.nb_face, sh);
   if (threadIdx.x == 0) {
     const double d = tot * a.dconst[b];
     a.series[size_t(b) * a.cap + size_t(n % a.cap)] = d;
     if (!isfinite(d)) a.nonfinite[b] = 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// one-kernel step over patches of cells (compact form)
+// ---------------------------------------------------------------------------
+// Workgroup j (4 P lanes) owns the P consecutive cells [j P, (j+1) P) of
+// the Hilbert order and the faces whose lower-ranked cell is among them (a
+// contiguous face range: faces are sorted by cell ranks).  It updates every
+// face of its cells -- owned ones, and the "foreign" faces on the patch
+// boundary owned by an earlier patch, which that patch computes identically
+// -- keeps l_e v_e^{n+1} of all of them in LDS, writes the owned ones, then
+// does continuity and the TRiSK partial sums of its cells from LDS.  Nothing
+// written by one workgroup is read by another in the same launch: v, eta
+// and the partial sums are double-buffered, the history slot written (n % 3)
+// is not one read.  The dissipation partials of step n are summed by the
+// first workgroup of the next launch (PENDING) or by k_series at the end of
+// a chunk.
+template <typename T>
+struct StepArgs {
+  FaceArgs<T> f;
+  CellArgs<T> c;
+  const int *cell_lo;           // [n_patch + 1] first cell of each patch
+  const int *own_lo;            // [n_patch + 1] first owned face of each patch
+  const int *for_ptr;           // [n_patch + 1] foreign face list offsets
+  const int *for_idx;           // foreign faces
+  const unsigned short *cyc_loc;  // [6][N] cycle faces, patch-local index
+  int n_patch;
+  int n_blocks;                 // workgroups per instance (= gridDim.x); each runs a contiguous run of patches
+};
+
+// One patch of the one-kernel step; returns this lane's dissipation
+// integrand sum over the owned faces it updated.
+template <typename T, int P, int LPC>
+__device__ __forceinline__ double step_patch(const StepArgs<T> &a, int j, int b, long long n, int energy_quad,
+                                             const double *table, int s, T *lv, int *gid) {
+  constexpr int NT = LPC * P;
+  const int E = a.f.E, N = a.f.N;
+  const int lo = a.own_lo[j], n_own = a.own_lo[j + 1] - lo;
+  const int flo = a.for_ptr[j], n_loc = n_own + (a.for_ptr[j + 1] - flo);
+  // the last P lanes (fewest faces) also own the patch cells; their cycle,
+  // kite fractions and state are loaded before the face phase so that those
+  // loads overlap it
+  const int ci = int(threadIdx.x) - (NT - P);
+  const int i = a.cell_lo[j] + ci;
+  const bool cell = ci >= 0 && i < a.cell_lo[j + 1];
+  int info = 0;
+  unsigned short l[6] = {0, 0, 0, 0, 0, 0};
+  T R[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+  T area = T(1), eta_old = T(0), r1 = T(0), r2 = T(0);
+  if (cell) {
+    info = a.c.info[i];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      l[q] = a.cyc_loc[q * N + i];
+      R[q] = a.c.kite[q * N + i];
+    }
+    area = a.c.area[i];
+    eta_old = a.c.eta[b * a.c.eta_stride + i];
+    const T *rg = a.c.ring + b * a.c.ring_stride;
+    r1 = rg[int((n + 2) % 3) * N + i];
+    r2 = rg[int((n + 1) % 3) * N + i];
+  }
+  double local = 0.0;
+  T *ring = a.f.ring + b * a.f.ring_stride + int(n % 3) * E;
+  T *vnxt = a.f.v_nxt + b * a.f.v_stride;
+  for (int q = threadIdx.x; q < n_loc; q += NT) {
+    const bool own = q < n_own;
+    const int e = own ? lo + q : a.for_idx[flo + q - n_own];
+    const FaceOut<T> r = face_update<T, false, true>(a.f, e, b, energy_quad, table, s);
+    if (own) {
+      local += r.energy;
+      ring[e] = r.f0;
+      vnxt[e] = r.vn;
+    }
+    lv[q] = a.c.flen[e] * r.vn;
+    gid[q] = e;
+  }
+  __syncthreads();
+  if (cell) {
+    int f[6];
+    T F[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      f[q] = gid[l[q]];
+      const T x = lv[l[q]];
+      F[q] = ((info >> (3 + q)) & 1) ? -x : x;
+    }
+    const int k = info & 7, db = info >> 3;
+    continuity_loaded(a.c, i, b, n, divergence(F, k, area), eta_old, r1, r2);
+    cell_outputs(a.c, b, k, db, f, F, R);
+  }
+  return local;
+}
+
+// MULTI: each workgroup runs a contiguous run of patches (fewer workgroups
+// than patches); else one patch per workgroup.
+template <typename T, bool PENDING, int P, int LPC, bool MULTI>
+__global__ __launch_bounds__(LPC * P) void k_step(StepArgs<T> a, int energy_quad, const double *table, int s) {
+  constexpr int NT = LPC * P;  // LPC lanes per cell; the faces of P cells are about 3.4 P
+  __shared__ T lv[6 * P];
+  __shared__ int gid[6 * P];
+  __shared__ double sh[NT / 64];
+  const int b = blockIdx.y;
+  const long long n = (long long)table[0] + s;
+  const int t = xcd_tile(blockIdx.x, a.n_blocks);
+  double local = 0.0;
+  if (MULTI) {
+    const int iters = (a.n_patch + a.n_blocks - 1) / a.n_blocks;
+    for (int it = 0; it < iters; ++it) {
+      const int j = t * iters + it;
+      if (j >= a.n_patch) break;    // uniform in the workgroup
+      if (it > 0) __syncthreads();  // the previous patch's cell phase has read lv / gid
+      local += step_patch<T, P, LPC>(a, j, b, n, energy_quad, table, s, lv, gid);
+    }
+  } else {
+    local = step_patch<T, P, LPC>(a, t, b, n, energy_quad, table, s, lv, gid);
+  }
+  const double part = block_sum<NT / 64>(local, sh);
+  if (threadIdx.x == 0) a.f.partials[((n & 1) * gridDim.y + b) * a.n_blocks + t] = part;
+  if (!PENDING || blockIdx.x != 0) return;
+  // dissipation of step n - 1: partials of the previous launch
+  __syncthreads();
+  const double tot = sum_partials<NT / 64>(
+      a.f.partials + (((n - 1) & 1) * gridDim.y + b) * size_t(a.n_blocks), a.n_blocks, sh);
+  if (threadIdx.x == 0) {
+    const double d = tot * a.c.dconst[b];
+    a.c.series[size_t(b) * a.c.cap + size_t((n - 1) % a.c.cap)] = d;
+    if (!isfinite(d)) a.c.nonfinite[b] = 1;
+  }
+}
+
+// Dissipation of the last step of a chunk of k_step launches.
+__global__ __launch_bounds__(kBlock) void k_series(const double *partials, int count, const double *dconst,
+                                                   double *series, int *nonfinite, int cap, const double *table,
+                                                   int s) {
+  __shared__ double sh[4];
+  const int b = blockIdx.x;
+  const long long n = (long long)table[0] + s;
+  const double tot = sum_partials<kBlock / 64>(partials + ((n & 1) * gridDim.x + b) * size_t(count), count, sh);
+  if (threadIdx.x == 0) {
+    const double d = tot * dconst[b];
+    series[size_t(b) * cap + size_t(n % cap)] = d;
+    if (!isfinite(d)) nonfinite[b] = 1;
   }
 }
 
@@ -141,7 +429,7 @@ __global__ __launch_bounds__(kBlock) void k_diss_final(const double *partials, i
                                                        double *out) {
   __shared__ double sh[4];
   const int b = blockIdx.x;
-  const double t = sum_partials(partials + size_t(b) * nb, nb, sh);
+  const double t = sum_partials<kBlock / 64>(partials + size_t(b) * nb, nb, sh);
   if (threadIdx.x == 0) out[b] = t * dconst[b];
 }
 
@@ -157,120 +445,6 @@ __global__ __launch_bounds__(kBlock) void k_velocity_en(int E, const int *tri_id
   const double nx = normal[2 * e], ny = normal[2 * e + 1], vn = double(v[e]);
   uv[2 * e] = nx * vn + ny * vt;
   uv[2 * e + 1] = ny * vn + (-nx) * vt;
-}
-
-// ---------------------------------------------------------------------------
-// fused step: one kernel per step
-// ---------------------------------------------------------------------------
-// Step n's kernel first completes the continuity update of step n-1 for the
-// two cells of its face (eta^n = eta^{n-1} + dt AB3(h Div v^n)), with the
-// same arithmetic as k_cell; the face that owns a cell (one per cell) writes
-// eta^n and the history slot, the others only use the value.  Then it does
-// the momentum update of step n with eta^n.  The last step of a chunk is
-// completed by k_cell.  eta is double-buffered (read n-1, write n).
-template <typename T>
-struct FusedArgs {
-  FaceArgs<T> f;                // eta = eta^{n-1} buffer when PENDING, else eta^n
-  int N;
-  const int *div_idx;           // [6][N]
-  const T *div_coef;            // [6][N]
-  T *eta_nxt;                   // [B][N] eta^n (owner writes)
-  T *ring_e;                    // [B][3][N]
-  long long ring_e_stride;
-  const unsigned char *owner;   // [E] bit 0: owns cell g_idx[e], bit 1: owns g_idx[E + e]
-  double *series;
-  int *nonfinite;
-  int cap;
-  const double *dconst;
-};
-
-template <typename T>
-__device__ __forceinline__ T complete_continuity(const FusedArgs<T> &a, int c, const T *v, const T *eta, T *ringe,
-                                                 long long m, bool ab3, T h, T dt, bool own, int b) {
-  const int N = a.N;
-  T acc = T(0);
-#pragma unroll
-  for (int j = 0; j < 6; ++j) acc += a.div_coef[j * N + c] * v[a.div_idx[j * N + c]];
-  const T f0 = h * acc;
-  const int s0 = int(m % 3), s1 = int((m + 2) % 3), s2 = int((m + 1) % 3);
-  T upd;
-  if (ab3)
-    upd = (T(23. / 12.) * f0 + T(-16. / 12.) * ringe[s1 * N + c] + T(5. / 12.) * ringe[s2 * N + c]) * dt;
-  else
-    upd = f0 * dt;
-  const T val = eta[c] + upd;
-  if (own) {
-    a.eta_nxt[b * a.f.eta_stride + c] = val;
-    ringe[s0 * N + c] = f0;
-  }
-  return val;
-}
-
-template <typename T, bool PENDING>
-__global__ __launch_bounds__(kBlock) void k_fused(FusedArgs<T> a, int energy_quad, const double *table, int s) {
-  __shared__ double sh[4];
-  __shared__ double sh2[4];
-  const FaceArgs<T> &F = a.f;
-  const int b = blockIdx.y;
-  const int tile = xcd_tile(blockIdx.x, F.nb);
-  const int E = F.E;
-  const int e = tile * kBlock + threadIdx.x;
-  const double *ip = F.inst + b * 8;
-  const long long n = (long long)table[0] + s;
-  const double *coef = table + 8 + size_t(s) * gridDim.y * 8;
-  const bool load = ip[3] != 0.0;
-  double local = 0.0;
-  if (e < E) {
-    const T *v = F.v_cur + b * F.v_stride;
-    T vt = T(0);
-#pragma unroll
-    for (int j = 0; j < 10; ++j) vt += F.tri_coef[j * E + e] * v[F.tri_idx[j * E + e]];
-    const T ve = v[e];
-    const T sp = ve * ve + vt * vt;
-    const T en = energy_quad ? sqrt(sp) * sp : sp;
-    local = double(F.farea[e]) * double(en);
-    const T g = T(ip[0]), alpha = T(ip[1]), dt = T(ip[2]), qd = T(ip[4]), h = T(ip[5]);
-    const T *eta = F.eta + b * F.eta_stride;
-    const int c0 = F.g_idx[e], c1 = F.g_idx[E + e];
-    T eta0, eta1;
-    if (PENDING) {
-      const long long m = n - 1;
-      const bool ab3m = (m >= 2) || load;
-      T *ringe = a.ring_e + b * a.ring_e_stride;
-      const unsigned int own = a.owner[e];
-      eta0 = complete_continuity(a, c0, v, eta, ringe, m, ab3m, h, dt, (own & 1u) != 0, b);
-      eta1 = complete_continuity(a, c1, v, eta, ringe, m, ab3m, h, dt, (own & 2u) != 0, b);
-    } else {
-      eta0 = eta[c0];
-      eta1 = eta[c1];
-    }
-    const bool ab3 = (n >= 2) || load;
-    const T ge = F.g_coef[e] * eta0 + F.g_coef[E + e] * eta1;
-    T f0 = -g * ge + F.fcor[e] * vt;
-    if (F.quad_drag) f0 -= qd * sqrt(sp) * ve;
-    T fu = T(0);
-    for (int k = 0; k < F.nfields; ++k) fu += T(coef[b * 8 + k]) * F.fgs[k * E + e];
-    const T drag = -alpha * ve + fu;
-    T *ring = F.ring + b * F.ring_stride;
-    const int s0 = int(n % 3), s1 = int((n + 2) % 3), s2 = int((n + 1) % 3);
-    T upd;
-    if (ab3)
-      upd = (T(23. / 12.) * f0 + T(-16. / 12.) * ring[s1 * E + e] + T(5. / 12.) * ring[s2 * E + e]) * dt;
-    else
-      upd = f0 * dt;
-    ring[s0 * E + e] = f0;
-    F.v_nxt[b * F.v_stride + e] = (ve + upd) + dt * drag;
-  }
-  const double part = block_sum<T>(local, sh);
-  if (threadIdx.x == 0) F.partials[((n & 1) * gridDim.y + b) * F.nb + tile] = part;
-  // dissipation of state n-1: partials of the previous kernel
-  if (!PENDING || blockIdx.x != 0) return;
-  const double tot = sum_partials(F.partials + (((n - 1) & 1) * gridDim.y + b) * size_t(F.nb), F.nb, sh2);
-  if (threadIdx.x == 0) {
-    const double d = tot * a.dconst[b];
-    a.series[size_t(b) * a.cap + size_t((n - 1) % a.cap)] = d;
-    if (!isfinite(d)) a.nonfinite[b] = 1;
-  }
 }
 
 // CSR rows of the lat-lon interpolation: one wavefront per row, the eta
@@ -419,8 +593,23 @@ struct Ops {
   T *div_coef = nullptr;
   T *v[2] = {nullptr, nullptr};
   T *ring_v = nullptr, *ring_e = nullptr;
-  T *eta_buf[2] = {nullptr, nullptr};  // canonical eta is eta_buf[ctx->ecur]
-  unsigned char *owner = nullptr;
+  T *eta[2] = {nullptr, nullptr};
+  // cell-local TRiSK form (compact kernels)
+  int *cyc = nullptr, *info = nullptr;
+  T *kite = nullptr, *area = nullptr, *flen = nullptr, *fdist = nullptr, *cellS = nullptr;
+  T *ts[2] = {nullptr, nullptr};  // [B][2][E] TRiSK partial sums of v[.] from the inner / outer cell
+  // patches of the one-kernel step
+  int *cell_lo = nullptr, *own_lo = nullptr, *for_ptr = nullptr, *for_idx = nullptr;
+  unsigned short *cyc_loc = nullptr;  // [6][N] cycle faces as patch-local indices
+};
+
+// Device-order operands of the cell-local TRiSK form.
+struct CompactHost {
+  std::vector<int> cyc, info;                                // [6][N], [N]
+  std::vector<double> kite, area, flen, fdist, cellS;        // [6][N], [N], [E], [E], [K][N]
+  // patches of the one-kernel step (empty: not available)
+  std::vector<int> cell_lo, own_lo, for_ptr, for_idx;        // [P + 1] x 3, [*]
+  std::vector<unsigned short> cyc_loc;                       // [6][N]
 };
 
 uint64_t hilbert3(uint32_t X0, uint32_t X1, uint32_t X2, int bits) {
@@ -455,16 +644,28 @@ uint64_t hilbert3(uint32_t X0, uint32_t X1, uint32_t X2, int bits) {
 struct odis_ctx {
   int N = 0, E = 0, B = 0, precision = ODIS_FP64;
   int nb_face = 0, nb_cell = 0;
+  int part_cap = 0;            // partials per (step parity, instance): max(face tiles, patches of 64 cells)
   int nfields = 0;
   int energy_quad = 0, quad_drag = 0;
   hipStream_t stream = nullptr;
   std::vector<int> cell_perm, cell_inv, face_perm, face_inv;  // ref -> dev, dev -> ref
   Ops<double> d;
   Ops<float> f;
-  int cur = 0, ecur = 0;        // v is v[cur], eta is eta_buf[ecur] between calls
-  int fused = 0;               // one kernel per step (SH coupling off)
+  int cur = 0;                 // v is v[cur] between calls
+  int compact_ok = 0;          // the descriptor has a verified cell-local TRiSK form
+  int compact = 0;             // use it (ODIS_OPT_COMPACT)
+  int ts_valid = 1;            // Ops::ts holds the partial sums of v[cur]
+  std::string compact_why;     // why compact_ok is 0
+  int one_kernel_ok = 0;       // patches for the one-kernel step are built
+  int one_kernel = 0;          // use it (ODIS_OPT_ONE_KERNEL; compact form, SH off)
+  int n_patch = 0;
+  int patch = 64;              // cells per patch (ODIS_OPT_PATCH_CELLS: 64, 128, 256)
+  int step_blocks = 0;         // workgroups of the one-kernel step (ODIS_OPT_STEP_BLOCKS; 0: one per patch)
+  int step_lanes = 4;          // lanes per cell of the one-kernel step (ODIS_OPT_STEP_LANES: 2 or 4)
+  std::string one_kernel_why;
+  std::vector<int> gi_host, cyc_host;  // device-order face cells and cell cycles, to rebuild patches
   long long step = 0;          // next step index n, shared by all instances
-  double *partials = nullptr;  // [2][B][nb_face], by step parity
+  double *partials = nullptr;  // [2][B][part_cap], by step parity
   double *series = nullptr;    // [B][cap]
   int *nonfinite = nullptr;    // [B]
   int cap = 1 << 16;
@@ -478,9 +679,9 @@ struct odis_ctx {
   double *coef_host[2] = {nullptr, nullptr};
   hipEvent_t coef_ev[2] = {nullptr, nullptr};
   int coef_next = 0, coef_last = 0;
-  // graphs of graph_steps steps, one per (coefficient buffer, v parity x 2 + eta parity)
+  // graphs of graph_steps steps, one per (coefficient buffer, v parity)
   int graph_steps = 0, use_graphs = 0;
-  hipGraphExec_t graphs[2][4] = {};
+  hipGraphExec_t graphs[2][2] = {};
   std::vector<TideDev> tides;
   std::vector<odis_params> params;
   std::vector<std::vector<double>> init_hist_v, init_hist_e;  // history as last set ([M][3])
@@ -612,10 +813,8 @@ template <typename T>
 static int build_ops(odis_ctx *ctx, Ops<T> &o, const std::vector<double> &tri_c, const std::vector<int> &tri_i,
                      const std::vector<double> &gc, const std::vector<int> &gi, const std::vector<double> &fcor,
                      const std::vector<double> &farea, const std::vector<double> &fgs,
-                     const std::vector<double> &dc, const std::vector<int> &di,
-                     const std::vector<unsigned char> &owner) {
+                     const std::vector<double> &dc, const std::vector<int> &di) {
   int rc;
-  if ((rc = upload(ctx, &o.owner, owner))) return rc;
   if ((rc = upload(ctx, &o.tri_idx, tri_i))) return rc;
   if ((rc = upload(ctx, &o.tri_coef, cast_vec<T>(tri_c)))) return rc;
   if ((rc = upload(ctx, &o.g_idx, gi))) return rc;
@@ -629,15 +828,107 @@ static int build_ops(odis_ctx *ctx, Ops<T> &o, const std::vector<double> &tri_c,
   if ((rc = dev_alloc(ctx, &o.v[0], B * E))) return rc;
   if ((rc = dev_alloc(ctx, &o.v[1], B * E))) return rc;
   if ((rc = dev_alloc(ctx, &o.ring_v, B * 3 * E))) return rc;
-  if ((rc = dev_alloc(ctx, &o.eta_buf[0], B * N))) return rc;
-  if ((rc = dev_alloc(ctx, &o.eta_buf[1], B * N))) return rc;
+  if ((rc = dev_alloc(ctx, &o.eta[0], B * N))) return rc;
+  if ((rc = dev_alloc(ctx, &o.eta[1], B * N))) return rc;
   if ((rc = dev_alloc(ctx, &o.ring_e, B * 3 * N))) return rc;
   CK(hipMemset(o.v[0], 0, B * E * sizeof(T)));
   CK(hipMemset(o.v[1], 0, B * E * sizeof(T)));
   CK(hipMemset(o.ring_v, 0, B * 3 * E * sizeof(T)));
-  CK(hipMemset(o.eta_buf[0], 0, B * N * sizeof(T)));
-  CK(hipMemset(o.eta_buf[1], 0, B * N * sizeof(T)));
+  CK(hipMemset(o.eta[0], 0, B * N * sizeof(T)));
+  CK(hipMemset(o.eta[1], 0, B * N * sizeof(T)));
   CK(hipMemset(o.ring_e, 0, B * 3 * N * sizeof(T)));
+  return ODIS_OK;
+}
+
+
+// Patches of at most P consecutive device cells (ceil(N / P) ranges of
+// near-equal size): owned face ranges, foreign face lists and patch-local
+// cycle indices.  Empty string on success.
+static std::string build_patches(int N, int E, const std::vector<int> &gi, int P, CompactHost &h) {
+  const int n_patch = (N + P - 1) / P;
+  h.cell_lo.resize(n_patch + 1);
+  for (int j = 0; j <= n_patch; j++) h.cell_lo[j] = int((long long)j * N / n_patch);
+  std::vector<int> minr(E);
+  for (int r = 0; r < E; r++) {
+    minr[r] = std::min(gi[r], gi[size_t(E) + r]);
+    if (r > 0 && minr[r] < minr[r - 1]) return "faces are not sorted by their lower cell rank";
+  }
+  h.own_lo.assign(n_patch + 1, E);
+  h.for_ptr.assign(n_patch + 1, 0);
+  h.for_idx.clear();
+  h.cyc_loc.assign(size_t(6) * N, 0);
+  for (int j = 0, r = 0; j < n_patch; j++) {
+    while (r < E && minr[r] < h.cell_lo[j]) r++;
+    h.own_lo[j] = r;
+  }
+  std::vector<int> loc(E, -1), foreign;
+  for (int j = 0; j < n_patch; j++) {
+    const int c_lo = h.cell_lo[j], c_hi = h.cell_lo[j + 1];
+    if (c_hi - c_lo > P) return "patch larger than its workgroup";
+    const int lo = h.own_lo[j], hi = h.own_lo[j + 1];
+    foreign.clear();
+    for (int c = c_lo; c < c_hi; c++)
+      for (int q = 0; q < 6; q++) {
+        const int f = h.cyc[size_t(q) * N + c];
+        if (minr[f] < c_lo) foreign.push_back(f);
+        else if (f < lo || f >= hi) return "a face of a patch cell is neither owned nor foreign";
+      }
+    std::sort(foreign.begin(), foreign.end());
+    foreign.erase(std::unique(foreign.begin(), foreign.end()), foreign.end());
+    const int n_loc = (hi - lo) + int(foreign.size());
+    if (n_loc > 6 * P) return "patch has more faces than its local store";
+    for (int f = lo; f < hi; f++) loc[f] = f - lo;
+    for (size_t q = 0; q < foreign.size(); q++) loc[foreign[q]] = (hi - lo) + int(q);
+    for (int c = c_lo; c < c_hi; c++)
+      for (int q = 0; q < 6; q++) {
+        const int l = loc[h.cyc[size_t(q) * N + c]];
+        if (l < 0 || l >= n_loc) return "patch-local face index out of range";
+        h.cyc_loc[size_t(q) * N + c] = (unsigned short)l;
+      }
+    for (int f = lo; f < hi; f++) loc[f] = -1;
+    for (int f : foreign) loc[f] = -1;
+    h.for_idx.insert(h.for_idx.end(), foreign.begin(), foreign.end());
+    h.for_ptr[j + 1] = int(h.for_idx.size());
+  }
+  if (h.for_idx.empty()) h.for_idx.push_back(0);  // keep the device array non-empty
+  return std::string();
+}
+
+// (Re)place the patch arrays of the one-kernel step.
+template <typename T>
+static int upload_patches(odis_ctx *ctx, Ops<T> &o, const CompactHost &h) {
+  for (void *p : {(void *)o.cell_lo, (void *)o.own_lo, (void *)o.for_ptr, (void *)o.for_idx, (void *)o.cyc_loc}) {
+    if (!p) continue;
+    (void)hipFree(p);
+    ctx->allocs.erase(std::remove(ctx->allocs.begin(), ctx->allocs.end(), p), ctx->allocs.end());
+  }
+  o.cell_lo = o.own_lo = o.for_ptr = o.for_idx = nullptr;
+  o.cyc_loc = nullptr;
+  int rc;
+  if ((rc = upload(ctx, &o.cell_lo, h.cell_lo))) return rc;
+  if ((rc = upload(ctx, &o.own_lo, h.own_lo))) return rc;
+  if ((rc = upload(ctx, &o.for_ptr, h.for_ptr))) return rc;
+  if ((rc = upload(ctx, &o.for_idx, h.for_idx))) return rc;
+  if ((rc = upload(ctx, &o.cyc_loc, h.cyc_loc))) return rc;
+  return ODIS_OK;
+}
+
+template <typename T>
+static int build_compact_ops(odis_ctx *ctx, Ops<T> &o, const CompactHost &h) {
+  int rc;
+  if ((rc = upload(ctx, &o.cyc, h.cyc))) return rc;
+  if ((rc = upload(ctx, &o.info, h.info))) return rc;
+  if ((rc = upload(ctx, &o.kite, cast_vec<T>(h.kite)))) return rc;
+  if ((rc = upload(ctx, &o.area, cast_vec<T>(h.area)))) return rc;
+  if ((rc = upload(ctx, &o.flen, cast_vec<T>(h.flen)))) return rc;
+  if ((rc = upload(ctx, &o.fdist, cast_vec<T>(h.fdist)))) return rc;
+  if ((rc = upload(ctx, &o.cellS, cast_vec<T>(h.cellS)))) return rc;
+  const size_t n = size_t(ctx->B) * 2 * ctx->E;
+  for (int k = 0; k < 2; k++) {  // v = 0 at creation: partial sums 0
+    if ((rc = dev_alloc(ctx, &o.ts[k], n))) return rc;
+    CK(hipMemset(o.ts[k], 0, n * sizeof(T)));
+  }
+  if (ctx->one_kernel_ok && (rc = upload_patches(ctx, o, h))) return rc;
   return ODIS_OK;
 }
 
@@ -646,7 +937,7 @@ extern "C" int odis_destroy(odis_ctx *ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto &e : ctx->ev) (void)hipEventDestroy(e);
   for (int k = 0; k < 2; k++)
-    for (int c = 0; c < 4; c++)
+    for (int c = 0; c < 2; c++)
       if (ctx->graphs[k][c]) (void)hipGraphExecDestroy(ctx->graphs[k][c]);
   for (int k = 0; k < 2; k++) {
     if (ctx->coef_ev[k]) (void)hipEventDestroy(ctx->coef_ev[k]);
@@ -795,30 +1086,64 @@ extern "C" int odis_create(const odis_mesh_desc *m, const odis_params *params, i
     }
   }
 
-  // one owner face per cell (its lowest device face index) writes the cell's
-  // continuity update in the fused step kernel
-  std::vector<unsigned char> owner(E, 0);
-  for (int r = 0; r < N; r++) {
-    int f = di[r];
-    for (int j = 1; j < 6; j++) f = std::min(f, di[size_t(j) * N + r]);
-    if (gi[f] == r) {
-      owner[f] |= 1u;
-    } else if (gi[size_t(E) + f] == r) {
-      owner[f] |= 2u;
+  // -- cell-local TRiSK form, when the descriptor supports it ----------------
+  CompactHost ch;
+  {
+    CompactTrisk ct;
+    std::string why;
+    try {
+      why = build_compact(*m, ct);
+    } catch (const std::exception &ex) {
+      why = ex.what();
+    }
+    ctx->compact_ok = why.empty() ? 1 : 0;
+    ctx->compact = ctx->compact_ok;
+    ctx->compact_why = why;
+    if (ctx->compact_ok) {
+      ch.cyc.resize(size_t(6) * N);
+      ch.kite.resize(size_t(6) * N);
+      ch.info.resize(N);
+      ch.area.resize(N);
+      ch.cellS.assign(size_t(std::max(K, 1)) * N, 0.0);
+      for (int r = 0; r < N; r++) {
+        const int i = ctx->cell_inv[r];
+        for (int q = 0; q < 6; q++) {
+          ch.cyc[size_t(q) * N + r] = ctx->face_perm[ct.cyc[size_t(i) * 6 + q]];
+          ch.kite[size_t(q) * N + r] = ct.R[size_t(i) * 6 + q];
+        }
+        ch.info[r] = ct.info[i];
+        ch.area[r] = m->cell_area[i];
+        for (int k = 0; k < K; k++) ch.cellS[size_t(k) * N + r] = cellS[size_t(k) * N + i];
+      }
+      ch.flen.resize(E);
+      ch.fdist.resize(E);
+      for (int r = 0; r < E; r++) {
+        ch.flen[r] = m->face_len[ctx->face_inv[r]];
+        ch.fdist[r] = m->face_dist[ctx->face_inv[r]];
+      }
+      const std::string pw = build_patches(N, E, gi, ctx->patch, ch);
+      ctx->one_kernel_ok = pw.empty() ? 1 : 0;
+      ctx->one_kernel = ctx->one_kernel_ok;
+      ctx->one_kernel_why = pw;
+      ctx->n_patch = (N + ctx->patch - 1) / ctx->patch;
+      ctx->gi_host = gi;
+      ctx->cyc_host = ch.cyc;
     } else {
-      delete ctx;
-      return fail(nullptr, ODIS_E_INVALID, "cell_faces disagrees with face_nodes");
+      ctx->one_kernel_why = "needs the compact TRiSK form";
     }
   }
+
   int rc = ODIS_OK;
   hipError_t he = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (he != hipSuccess) rc = fail(ctx, ODIS_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
   if (!rc) {
     if (precision == ODIS_FP64)
-      rc = build_ops(ctx, ctx->d, tri_c, tri_i, gc, gi, fcor, farea, fgs, dc, di, owner);
+      rc = build_ops(ctx, ctx->d, tri_c, tri_i, gc, gi, fcor, farea, fgs, dc, di);
     else
-      rc = build_ops(ctx, ctx->f, tri_c, tri_i, gc, gi, fcor, farea, fgs, dc, di, owner);
+      rc = build_ops(ctx, ctx->f, tri_c, tri_i, gc, gi, fcor, farea, fgs, dc, di);
   }
+  if (!rc && ctx->compact_ok)
+    rc = precision == ODIS_FP64 ? build_compact_ops(ctx, ctx->d, ch) : build_compact_ops(ctx, ctx->f, ch);
   ctx->nb_face = (E + kBlock - 1) / kBlock;
   ctx->nb_cell = (N + kBlock - 1) / kBlock;
   // per-instance parameters
@@ -840,7 +1165,8 @@ extern "C" int odis_create(const odis_mesh_desc *m, const odis_params *params, i
   }
   if (!rc) rc = upload(ctx, &ctx->inst, inst);
   if (!rc) rc = upload(ctx, &ctx->dconst, dconst);
-  if (!rc) rc = dev_alloc(ctx, &ctx->partials, size_t(2) * n_inst * ctx->nb_face);
+  ctx->part_cap = std::max(ctx->nb_face, (N + 63) / 64);
+  if (!rc) rc = dev_alloc(ctx, &ctx->partials, size_t(2) * n_inst * ctx->part_cap);
   if (!rc) rc = dev_alloc(ctx, &ctx->series, size_t(n_inst) * ctx->cap);
   if (!rc) rc = dev_alloc(ctx, &ctx->nonfinite, size_t(n_inst));
   if (!rc) rc = dev_alloc(ctx, &ctx->scratch_out, size_t(n_inst));
@@ -899,11 +1225,12 @@ static int put_state(odis_ctx *ctx, Ops<T> &o, int b, const double *v, const dou
     std::vector<T> h(E);
     for (int r = 0; r < E; r++) h[r] = T(v[ctx->face_inv[r]]);
     CK(hipMemcpy(o.v[ctx->cur] + bE, h.data(), E * sizeof(T), hipMemcpyHostToDevice));
+    ctx->ts_valid = 0;
   }
   if (eta) {
     std::vector<T> h(N);
     for (int r = 0; r < N; r++) h[r] = T(eta[ctx->cell_inv[r]]);
-    CK(hipMemcpy(o.eta_buf[ctx->ecur] + bN, h.data(), N * sizeof(T), hipMemcpyHostToDevice));
+    CK(hipMemcpy(o.eta[ctx->cur] + bN, h.data(), N * sizeof(T), hipMemcpyHostToDevice));
   }
   auto put_hist = [&](T *ring, int M, const std::vector<int> &inv, const double *ds,
                       std::vector<double> &keep) -> int {
@@ -953,7 +1280,7 @@ static int get_state_t(odis_ctx *ctx, Ops<T> &o, int b, double *v, double *dv, d
   }
   if (eta) {
     std::vector<T> h(N);
-    CK(hipMemcpy(h.data(), o.eta_buf[ctx->ecur] + bN, N * sizeof(T), hipMemcpyDeviceToHost));
+    CK(hipMemcpy(h.data(), o.eta[ctx->cur] + bN, N * sizeof(T), hipMemcpyDeviceToHost));
     for (int r = 0; r < N; r++) eta[ctx->cell_inv[r]] = double(h[r]);
   }
   auto get_hist = [&](T *ring, int M, const std::vector<int> &inv, double *ds,
@@ -1014,13 +1341,18 @@ static FaceArgs<T> face_args(odis_ctx *ctx, Ops<T> &o) {
   a.v_cur = o.v[ctx->cur];
   a.v_nxt = o.v[ctx->cur ^ 1];
   a.ring = o.ring_v;
-  a.eta = ctx->sh_mode != ODIS_SH_OFF ? static_cast<const T *>(ctx->eta_eff) : o.eta_buf[ctx->ecur];
+  a.eta = ctx->sh_mode != ODIS_SH_OFF ? static_cast<const T *>(ctx->eta_eff) : o.eta[ctx->cur];
   a.v_stride = ctx->E;
   a.ring_stride = 3LL * ctx->E;
   a.eta_stride = ctx->N;
   a.partials = ctx->partials;
   a.inst = ctx->inst;
   a.quad_drag = ctx->quad_drag;
+  a.N = ctx->N;
+  a.fdist = o.fdist;
+  a.ts = o.ts[ctx->cur];
+  a.ts_stride = 2LL * ctx->E;
+  a.cellS = o.cellS;
   return a;
 }
 
@@ -1033,7 +1365,8 @@ static CellArgs<T> cell_args(odis_ctx *ctx, Ops<T> &o) {
   a.div_idx = o.div_idx;
   a.div_coef = o.div_coef;
   a.v_new = o.v[ctx->cur ^ 1];
-  a.eta = o.eta_buf[ctx->ecur];
+  a.eta = o.eta[ctx->cur];
+  a.eta_nxt = o.eta[ctx->cur ^ 1];
   a.ring = o.ring_e;
   a.v_stride = ctx->E;
   a.eta_stride = ctx->N;
@@ -1044,6 +1377,14 @@ static CellArgs<T> cell_args(odis_ctx *ctx, Ops<T> &o) {
   a.cap = ctx->cap;
   a.inst = ctx->inst;
   a.dconst = ctx->dconst;
+  a.E = ctx->E;
+  a.cyc = o.cyc;
+  a.kite = o.kite;
+  a.info = o.info;
+  a.area = o.area;
+  a.flen = o.flen;
+  a.ts = o.ts[ctx->cur ^ 1];
+  a.ts_stride = 2LL * ctx->E;
   return a;
 }
 
@@ -1081,46 +1422,28 @@ static int sh_effective_height(odis_ctx *ctx, Ops<T> &o) {
   const dim3 blk(kBlock);
   if (ctx->sh_mode == ODIS_SH_SPLIT_BF16)
     hipLaunchKernelGGL((k_sh_analysis<T, true>), dim3(K, ctx->B), blk, 0, ctx->stream, K, N, ctx->sh_Yt,
-                       ctx->sh_Yp, o.eta_buf[ctx->ecur], (long long)N, ctx->sh_b);
+                       ctx->sh_Yp, o.eta[ctx->cur], (long long)N, ctx->sh_b);
   else
     hipLaunchKernelGGL((k_sh_analysis<T, false>), dim3(K, ctx->B), blk, 0, ctx->stream, K, N, ctx->sh_Yt,
-                       ctx->sh_Yp, o.eta_buf[ctx->ecur], (long long)N, ctx->sh_b);
+                       ctx->sh_Yp, o.eta[ctx->cur], (long long)N, ctx->sh_b);
   hipLaunchKernelGGL(k_sh_coef, dim3((K + 3) / 4, ctx->B), blk, 0, ctx->stream, K, ctx->sh_Minv, ctx->sh_D,
                      ctx->sh_b, ctx->sh_c);
   const size_t lds = sizeof(double) * K;
   if (ctx->sh_mode == ODIS_SH_SPLIT_BF16)
     hipLaunchKernelGGL((k_sh_synth<T, true>), dim3(ctx->nb_cell, ctx->B), blk, lds, ctx->stream, K, N, ctx->sh_Yt,
-                       ctx->sh_Yp, ctx->sh_c, o.eta_buf[ctx->ecur], (long long)N, static_cast<T *>(ctx->eta_eff));
+                       ctx->sh_Yp, ctx->sh_c, o.eta[ctx->cur], (long long)N, static_cast<T *>(ctx->eta_eff));
   else
     hipLaunchKernelGGL((k_sh_synth<T, false>), dim3(ctx->nb_cell, ctx->B), blk, lds, ctx->stream, K, N,
-                       ctx->sh_Yt, ctx->sh_Yp, ctx->sh_c, o.eta_buf[ctx->ecur], (long long)N,
+                       ctx->sh_Yt, ctx->sh_Yp, ctx->sh_c, o.eta[ctx->cur], (long long)N,
                        static_cast<T *>(ctx->eta_eff));
   CK(hipGetLastError());
   return ODIS_OK;
 }
 
-template <typename T>
-static FusedArgs<T> fused_args(odis_ctx *ctx, Ops<T> &o) {
-  FusedArgs<T> a;
-  a.f = face_args(ctx, o);
-  a.N = ctx->N;
-  a.div_idx = o.div_idx;
-  a.div_coef = o.div_coef;
-  a.eta_nxt = o.eta_buf[ctx->ecur ^ 1];
-  a.ring_e = o.ring_e;
-  a.ring_e_stride = 3LL * ctx->N;
-  a.owner = o.owner;
-  a.series = ctx->series;
-  a.nonfinite = ctx->nonfinite;
-  a.cap = ctx->cap;
-  a.dconst = ctx->dconst;
-  return a;
-}
-
-// Event pairs (start, end) recorded for the kernels of sampled chunks.
+// Event pairs (start, end) recorded for the kernels of sampled steps.
 struct ChunkEvents {
-  std::vector<int> mom;   // momentum kernel of each step (k_face or k_fused)
-  std::vector<int> cont;  // continuity kernels (k_cell)
+  std::vector<int> mom;   // momentum kernel (k_face, or the one-kernel k_step) of each sampled step
+  std::vector<int> cont;  // continuity kernel (k_cell) of each sampled face + cell step
 };
 
 static int reserve_events(odis_ctx *ctx, size_t n) {
@@ -1132,59 +1455,123 @@ static int reserve_events(odis_ctx *ctx, size_t n) {
   return ODIS_OK;
 }
 
+// TRiSK partial sums of the current velocity, when the last state upload
+// left them stale (compact form only).
+template <typename T>
+static int ensure_ts(odis_ctx *ctx, Ops<T> &o) {
+  if (!ctx->compact || ctx->ts_valid) return ODIS_OK;
+  CellArgs<T> ca = cell_args(ctx, o);
+  ca.v_new = o.v[ctx->cur];
+  ca.ts = o.ts[ctx->cur];
+  hipLaunchKernelGGL((k_cell<T, true, true>), dim3(ctx->nb_cell, ctx->B), dim3(kBlock), 0, ctx->stream, ca,
+                     static_cast<const double *>(nullptr), 0);
+  CK(hipGetLastError());
+  ctx->ts_valid = 1;
+  return ODIS_OK;
+}
+
+// Workgroups per instance of the one-kernel step.
+static int step_blocks(const odis_ctx *ctx) {
+  return ctx->step_blocks > 0 ? std::min(ctx->step_blocks, ctx->n_patch) : ctx->n_patch;
+}
+
+template <typename T>
+static StepArgs<T> step_args(odis_ctx *ctx, Ops<T> &o) {
+  StepArgs<T> a;
+  a.f = face_args(ctx, o);
+  a.c = cell_args(ctx, o);
+  a.cell_lo = o.cell_lo;
+  a.own_lo = o.own_lo;
+  a.for_ptr = o.for_ptr;
+  a.for_idx = o.for_idx;
+  a.cyc_loc = o.cyc_loc;
+  a.n_patch = ctx->n_patch;
+  a.n_blocks = step_blocks(ctx);
+  return a;
+}
+
+template <typename T, int P, int LPC, bool MULTI>
+static void launch_step_kernel(odis_ctx *ctx, const StepArgs<T> &a, const double *table, int s, hipEvent_t *ev) {
+  const dim3 gp(a.n_blocks, ctx->B), bp(LPC * P);
+  if (s == 0) {
+    if (ev)
+      hipExtLaunchKernelGGL((k_step<T, false, P, LPC, MULTI>), gp, bp, 0, ctx->stream, ev[0], ev[1], 0, a,
+                            ctx->energy_quad, table, s);
+    else
+      hipLaunchKernelGGL((k_step<T, false, P, LPC, MULTI>), gp, bp, 0, ctx->stream, a, ctx->energy_quad, table, s);
+  } else {
+    if (ev)
+      hipExtLaunchKernelGGL((k_step<T, true, P, LPC, MULTI>), gp, bp, 0, ctx->stream, ev[0], ev[1], 0, a,
+                            ctx->energy_quad, table, s);
+    else
+      hipLaunchKernelGGL((k_step<T, true, P, LPC, MULTI>), gp, bp, 0, ctx->stream, a, ctx->energy_quad, table, s);
+  }
+}
+
+template <typename T, int P>
+static void launch_step_lanes(odis_ctx *ctx, const StepArgs<T> &a, const double *table, int s, hipEvent_t *ev) {
+  const bool multi = a.n_blocks < a.n_patch;
+  if (ctx->step_lanes == 2) {
+    if (multi)
+      launch_step_kernel<T, P, 2, true>(ctx, a, table, s, ev);
+    else
+      launch_step_kernel<T, P, 2, false>(ctx, a, table, s, ev);
+  } else {
+    if (multi)
+      launch_step_kernel<T, P, 4, true>(ctx, a, table, s, ev);
+    else
+      launch_step_kernel<T, P, 4, false>(ctx, a, table, s, ev);
+  }
+}
+
+template <typename T, bool C>
+static void launch_pair(odis_ctx *ctx, const FaceArgs<T> &fa, const CellArgs<T> &ca, const double *table, int s,
+                        hipEvent_t *evf, hipEvent_t *evc) {
+  const dim3 gf(ctx->nb_face, ctx->B), gc(ctx->nb_cell, ctx->B), blk(kBlock);
+  if (evf)
+    hipExtLaunchKernelGGL((k_face<T, false, C>), gf, blk, 0, ctx->stream, evf[0], evf[1], 0, fa, ctx->energy_quad,
+                          table, s);
+  else
+    hipLaunchKernelGGL((k_face<T, false, C>), gf, blk, 0, ctx->stream, fa, ctx->energy_quad, table, s);
+  if (evc)
+    hipExtLaunchKernelGGL((k_cell<T, C, false>), gc, blk, 0, ctx->stream, evc[0], evc[1], 0, ca, table, s);
+  else
+    hipLaunchKernelGGL((k_cell<T, C, false>), gc, blk, 0, ctx->stream, ca, table, s);
+}
+
 // Enqueue the steps [n0, n0 + count) of a staged coefficient table
-// (n0 = ctx->step).  With the fused kernel a step is one launch plus one
-// k_cell at the end of the chunk; with the SH coupling it is the projection
-// kernels, k_face and k_cell.  evs: dispatch-stamped events on the kernels of
-// every ctx->timing-th step (null: none).
+// (n0 = ctx->step): per step the SH projection kernels (when a basis is
+// set), the face kernel and the cell kernel.  evs: dispatch-stamped events on
+// the kernels of every ctx->timing-th step (null: none).
 template <typename T>
 static int launch_chunk(odis_ctx *ctx, Ops<T> &o, const double *table, int count, ChunkEvents *evs,
                         size_t *ev_used) {
-  const dim3 gf(ctx->nb_face, ctx->B), gc(ctx->nb_cell, ctx->B), blk(kBlock);
   const long long n0 = ctx->step;
   if (evs) {
     const size_t sampled = size_t(count / ctx->timing + 2);
-    const int rc = reserve_events(ctx, *ev_used + 4 * sampled + 2);
+    const int rc = reserve_events(ctx, *ev_used + 4 * sampled);
     if (rc) return rc;
   }
-  bool any = false;
-  std::vector<int> dummy_rec;
-  auto take = [&](std::vector<int> &rec, int s) -> hipEvent_t * {
-    if (!evs || (s >= 0 && (n0 + s) % ctx->timing != 0) || (s < 0 && !any)) return nullptr;
-    any = true;
-    rec.push_back(int(*ev_used));
-    hipEvent_t *p = &ctx->ev[*ev_used];
-    *ev_used += 2;
-    return p;
-  };
-  if (ctx->fused && ctx->sh_mode == ODIS_SH_OFF) {
+  if (ctx->compact && ctx->one_kernel && ctx->sh_mode == ODIS_SH_OFF) {
+    if (step_blocks(ctx) > ctx->part_cap) return fail(ctx, ODIS_E_INVALID, "more workgroups than dissipation partials");
     for (int s = 0; s < count; s++) {
-      FusedArgs<T> a = fused_args(ctx, o);
-      hipEvent_t *ev = take(evs ? evs->mom : dummy_rec, s);
-      if (s == 0) {
-        if (ev)
-          hipExtLaunchKernelGGL((k_fused<T, false>), gf, blk, 0, ctx->stream, ev[0], ev[1], 0, a,
-                                ctx->energy_quad, table, s);
-        else
-          hipLaunchKernelGGL((k_fused<T, false>), gf, blk, 0, ctx->stream, a, ctx->energy_quad, table, s);
-      } else {
-        if (ev)
-          hipExtLaunchKernelGGL((k_fused<T, true>), gf, blk, 0, ctx->stream, ev[0], ev[1], 0, a,
-                                ctx->energy_quad, table, s);
-        else
-          hipLaunchKernelGGL((k_fused<T, true>), gf, blk, 0, ctx->stream, a, ctx->energy_quad, table, s);
-        ctx->ecur ^= 1;
+      StepArgs<T> a = step_args(ctx, o);
+      hipEvent_t *ev = nullptr;
+      if (evs && (n0 + s) % ctx->timing == 0) {
+        evs->mom.push_back(int(*ev_used));
+        ev = &ctx->ev[*ev_used];
+        *ev_used += 2;
       }
+      if (ctx->patch == 64)
+        launch_step_lanes<T, 64>(ctx, a, table, s, ev);
+      else if (ctx->patch == 256)
+        launch_step_lanes<T, 256>(ctx, a, table, s, ev);
+      else
+        launch_step_lanes<T, 128>(ctx, a, table, s, ev);
       ctx->cur ^= 1;
     }
-    // continuity of the chunk's last step, in place on eta_buf[ecur]
-    CellArgs<T> ca = cell_args(ctx, o);
-    ca.v_new = o.v[ctx->cur];
-    hipEvent_t *ev = take(evs ? evs->cont : dummy_rec, -1);
-    if (ev)
-      hipExtLaunchKernelGGL((k_cell<T>), gc, blk, 0, ctx->stream, ev[0], ev[1], 0, ca, table, count - 1);
-    else
-      hipLaunchKernelGGL((k_cell<T>), gc, blk, 0, ctx->stream, ca, table, count - 1);
+    hipLaunchKernelGGL(k_series, dim3(ctx->B), dim3(kBlock), 0, ctx->stream, ctx->partials, step_blocks(ctx),
+                       ctx->dconst, ctx->series, ctx->nonfinite, ctx->cap, table, count - 1);
     CK(hipGetLastError());
     return ODIS_OK;
   }
@@ -1195,18 +1582,18 @@ static int launch_chunk(odis_ctx *ctx, Ops<T> &o, const double *table, int count
     }
     FaceArgs<T> fa = face_args(ctx, o);
     CellArgs<T> ca = cell_args(ctx, o);
-    hipEvent_t *ev = take(evs ? evs->mom : dummy_rec, s);
-    const bool sampled = ev != nullptr;
-    if (ev)
-      hipExtLaunchKernelGGL((k_face<T, false>), gf, blk, 0, ctx->stream, ev[0], ev[1], 0, fa, ctx->energy_quad,
-                            table, s);
+    hipEvent_t *evf = nullptr, *evc = nullptr;
+    if (evs && (n0 + s) % ctx->timing == 0) {
+      evs->mom.push_back(int(*ev_used));
+      evs->cont.push_back(int(*ev_used + 2));
+      evf = &ctx->ev[*ev_used];
+      evc = &ctx->ev[*ev_used + 2];
+      *ev_used += 4;
+    }
+    if (ctx->compact)
+      launch_pair<T, true>(ctx, fa, ca, table, s, evf, evc);
     else
-      hipLaunchKernelGGL((k_face<T, false>), gf, blk, 0, ctx->stream, fa, ctx->energy_quad, table, s);
-    ev = sampled ? take(evs->cont, s) : nullptr;
-    if (ev)
-      hipExtLaunchKernelGGL((k_cell<T>), gc, blk, 0, ctx->stream, ev[0], ev[1], 0, ca, table, s);
-    else
-      hipLaunchKernelGGL((k_cell<T>), gc, blk, 0, ctx->stream, ca, table, s);
+      launch_pair<T, false>(ctx, fa, ca, table, s, evf, evc);
     CK(hipGetLastError());
     ctx->cur ^= 1;
   }
@@ -1214,24 +1601,20 @@ static int launch_chunk(odis_ctx *ctx, Ops<T> &o, const double *table, int count
 }
 
 // Graph of graph_steps steps reading coefficient buffer k, one per entry
-// parity of (v, eta); replaying it leaves the parities as the captured
-// launches did.
+// parity of v (graph_steps is even: a replay restores the parity).
 template <typename T>
 static int get_graph(odis_ctx *ctx, Ops<T> &o, int k, hipGraphExec_t *out) {
-  const int key = ctx->cur * 2 + ctx->ecur;
-  hipGraphExec_t &g = ctx->graphs[k][key];
+  hipGraphExec_t &g = ctx->graphs[k][ctx->cur];
   if (g) {
     *out = g;
     return ODIS_OK;
   }
-  const int cur0 = ctx->cur, ecur0 = ctx->ecur;
+  const int cur0 = ctx->cur;
   hipGraph_t graph = nullptr;
   CK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
   int rc = launch_chunk(ctx, o, ctx->coef_dev[k], ctx->graph_steps, nullptr, nullptr);
   hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
-  // replays advance the parities (advance_parities); restore them here
   ctx->cur = cur0;
-  ctx->ecur = ecur0;
   if (rc) return rc;
   if (e != hipSuccess) return fail(ctx, ODIS_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
   e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
@@ -1246,17 +1629,11 @@ static int get_graph(odis_ctx *ctx, Ops<T> &o, int k, hipGraphExec_t *out) {
 
 static void drop_graphs(odis_ctx *ctx) {
   for (int k = 0; k < 2; k++)
-    for (int c = 0; c < 4; c++)
+    for (int c = 0; c < 2; c++)
       if (ctx->graphs[k][c]) {
         (void)hipGraphExecDestroy(ctx->graphs[k][c]);
         ctx->graphs[k][c] = nullptr;
       }
-}
-
-// parities after launching count steps from (cur, ecur)
-static void advance_parities(odis_ctx *ctx, int count) {
-  if (count & 1) ctx->cur ^= 1;
-  if (ctx->fused && ctx->sh_mode == ODIS_SH_OFF && ((count - 1) & 1)) ctx->ecur ^= 1;
 }
 
 template <typename T>
@@ -1267,6 +1644,8 @@ static int step_t(odis_ctx *ctx, Ops<T> &o, int n_steps) {
   std::vector<ChunkEvents> sampled;
   size_t ev_used = 0;
   int done = 0;
+  int rc = ensure_ts(ctx, o);
+  if (rc) return rc;
   while (done < n_steps) {
     if (ctx->stop_flag) {
       ctx->stop_flag = 0;
@@ -1274,19 +1653,18 @@ static int step_t(odis_ctx *ctx, Ops<T> &o, int n_steps) {
     }
     const int left = n_steps - done;
     const bool graph_fits = ctx->use_graphs && left >= ctx->graph_steps;
-    int count = graph_fits ? ctx->graph_steps : std::min(ctx->chunk, left);
+    const int count = graph_fits ? ctx->graph_steps : std::min(ctx->chunk, left);
     // does [step, step + count) hold a multiple of the timing stride?
     const bool sample = ctx->timing > 0 && ((ctx->step + ctx->timing - 1) / ctx->timing) * ctx->timing <
                                                ctx->step + count;
     const bool graph = graph_fits && !sample;
     const double *table = nullptr;
-    int rc = stage_coefficients(ctx, ctx->step, count, &table);
-    if (rc) return rc;
+    if ((rc = stage_coefficients(ctx, ctx->step, count, &table))) return rc;
     if (graph) {
       hipGraphExec_t g = nullptr;
       if ((rc = get_graph(ctx, o, ctx->coef_last, &g))) return rc;
       CK(hipGraphLaunch(g, ctx->stream));
-      advance_parities(ctx, count);
+      if (count & 1) ctx->cur ^= 1;
     } else {
       ChunkEvents *evs = nullptr;
       if (sample) {
@@ -1301,33 +1679,23 @@ static int step_t(odis_ctx *ctx, Ops<T> &o, int n_steps) {
   if (!sampled.empty()) {
     CK(hipStreamSynchronize(ctx->stream));
     double tm = 0, tc = 0, ts = 0;
-    long long nm = 0, nc = 0, ns = 0;
-    for (const ChunkEvents &c : sampled) {
+    long long ns = 0;
+    for (const ChunkEvents &c : sampled)
       for (size_t j = 0; j < c.mom.size(); j++) {
-        float a = 0;
+        float a = 0, b = 0, w = 0;
         CK(hipEventElapsedTime(&a, ctx->ev[c.mom[j]], ctx->ev[c.mom[j] + 1]));
+        w = a;
+        if (j < c.cont.size()) {  // face + cell kernel steps
+          CK(hipEventElapsedTime(&b, ctx->ev[c.cont[j]], ctx->ev[c.cont[j] + 1]));
+          CK(hipEventElapsedTime(&w, ctx->ev[c.mom[j]], ctx->ev[c.cont[j] + 1]));
+        }
         tm += a;
-        nm++;
-      }
-      for (int base : c.cont) {
-        float a = 0;
-        CK(hipEventElapsedTime(&a, ctx->ev[base], ctx->ev[base + 1]));
-        tc += a;
-        nc++;
-      }
-      // step time: momentum start to continuity end of a sampled split
-      // step (momentum kernel alone when fused)
-      for (size_t j = 0; j < c.mom.size(); j++) {
-        const bool pair = !ctx->fused || ctx->sh_mode != ODIS_SH_OFF;
-        const int end = pair && j < c.cont.size() ? c.cont[j] : c.mom[j];
-        float a = 0;
-        CK(hipEventElapsedTime(&a, ctx->ev[c.mom[j]], ctx->ev[end + 1]));
-        ts += a;
+        tc += b;
+        ts += w;
         ns++;
       }
-    }
-    ctx->t_face = nm ? tm / nm : 0.0;
-    ctx->t_cell = nc ? tc / nc : 0.0;
+    ctx->t_face = ns ? tm / ns : 0.0;
+    ctx->t_cell = ns ? tc / ns : 0.0;
     ctx->t_step = ns ? ts / ns : 0.0;
   }
   return ODIS_OK;
@@ -1350,10 +1718,16 @@ extern "C" int odis_sync(odis_ctx *ctx) {
 
 template <typename T>
 static int diss_t(odis_ctx *ctx, Ops<T> &o) {
+  const int rc = ensure_ts(ctx, o);
+  if (rc) return rc;
   FaceArgs<T> fa = face_args(ctx, o);
   const dim3 gf(ctx->nb_face, ctx->B), blk(kBlock);
-  hipLaunchKernelGGL((k_face<T, true>), gf, blk, 0, ctx->stream, fa, ctx->energy_quad,
-                     static_cast<const double *>(nullptr), 0);
+  if (ctx->compact)
+    hipLaunchKernelGGL((k_face<T, true, true>), gf, blk, 0, ctx->stream, fa, ctx->energy_quad,
+                       static_cast<const double *>(nullptr), 0);
+  else
+    hipLaunchKernelGGL((k_face<T, true, false>), gf, blk, 0, ctx->stream, fa, ctx->energy_quad,
+                       static_cast<const double *>(nullptr), 0);
   hipLaunchKernelGGL(k_diss_final, dim3(ctx->B), blk, 0, ctx->stream, ctx->partials, ctx->nb_face, ctx->dconst,
                      ctx->scratch_out);
   CK(hipGetLastError());
@@ -1441,20 +1815,77 @@ extern "C" int odis_enable_timing(odis_ctx *ctx, int stride) {
   return ODIS_OK;
 }
 
-extern "C" int odis_use_graphs(odis_ctx *ctx, int on) {
+extern "C" int odis_set_option(odis_ctx *ctx, int option, int value) {
   if (!ctx) return ODIS_E_INVALID;
-  ctx->use_graphs = on ? 1 : 0;
-  return ODIS_OK;
-}
-
-extern "C" int odis_use_fused(odis_ctx *ctx, int on) {
-  if (!ctx) return ODIS_E_INVALID;
-  const int f = on ? 1 : 0;
-  if (f != ctx->fused) {
+  if (option == ODIS_OPT_GRAPHS) {
+    ctx->use_graphs = value ? 1 : 0;
+    return ODIS_OK;
+  }
+  if (option == ODIS_OPT_COMPACT) {
+    const int c = value ? 1 : 0;
+    if (c && !ctx->compact_ok) return fail(ctx, ODIS_E_UNSUPPORTED, "no compact TRiSK form: " + ctx->compact_why);
+    if (c != ctx->compact) {
+      CK(hipStreamSynchronize(ctx->stream));
+      drop_graphs(ctx);
+      ctx->compact = c;
+      ctx->ts_valid = 0;
+    }
+    return ODIS_OK;
+  }
+  if (option == ODIS_OPT_PATCH_CELLS) {
+    if (value != 64 && value != 128 && value != 256)
+      return fail(ctx, ODIS_E_INVALID, "patch cells must be 64, 128 or 256");
+    if (!ctx->one_kernel_ok) return fail(ctx, ODIS_E_UNSUPPORTED, "no one-kernel step: " + ctx->one_kernel_why);
+    if (value == ctx->patch) return ODIS_OK;
+    CompactHost h;
+    h.cyc = ctx->cyc_host;
+    const std::string why = build_patches(ctx->N, ctx->E, ctx->gi_host, value, h);
+    if (!why.empty()) return fail(ctx, ODIS_E_UNSUPPORTED, why);
     CK(hipStreamSynchronize(ctx->stream));
     drop_graphs(ctx);
-    ctx->fused = f;
+    const int rc = ctx->precision == ODIS_FP64 ? upload_patches(ctx, ctx->d, h) : upload_patches(ctx, ctx->f, h);
+    if (rc) return rc;
+    ctx->patch = value;
+    ctx->n_patch = (ctx->N + value - 1) / value;
+    return ODIS_OK;
   }
+  if (option == ODIS_OPT_STEP_LANES) {
+    if (value != 2 && value != 4) return fail(ctx, ODIS_E_INVALID, "step lanes must be 2 or 4");
+    CK(hipStreamSynchronize(ctx->stream));
+    drop_graphs(ctx);
+    ctx->step_lanes = value;
+    return ODIS_OK;
+  }
+  if (option == ODIS_OPT_STEP_BLOCKS) {
+    if (value < 0) return fail(ctx, ODIS_E_INVALID, "step blocks < 0");
+    CK(hipStreamSynchronize(ctx->stream));
+    drop_graphs(ctx);
+    ctx->step_blocks = value;
+    return ODIS_OK;
+  }
+  if (option == ODIS_OPT_ONE_KERNEL) {
+    const int c = value ? 1 : 0;
+    if (c && !ctx->one_kernel_ok)
+      return fail(ctx, ODIS_E_UNSUPPORTED, "no one-kernel step: " + ctx->one_kernel_why);
+    if (c != ctx->one_kernel) {
+      CK(hipStreamSynchronize(ctx->stream));
+      drop_graphs(ctx);
+      ctx->one_kernel = c;
+    }
+    return ODIS_OK;
+  }
+  return fail(ctx, ODIS_E_INVALID, "unknown option");
+}
+
+extern "C" int odis_get_option(odis_ctx *ctx, int option, int *value) {
+  if (!ctx || !value) return ODIS_E_INVALID;
+  if (option == ODIS_OPT_GRAPHS) *value = ctx->use_graphs;
+  else if (option == ODIS_OPT_COMPACT) *value = ctx->compact;
+  else if (option == ODIS_OPT_ONE_KERNEL) *value = ctx->one_kernel;
+  else if (option == ODIS_OPT_PATCH_CELLS) *value = ctx->patch;
+  else if (option == ODIS_OPT_STEP_BLOCKS) *value = step_blocks(ctx);
+  else if (option == ODIS_OPT_STEP_LANES) *value = ctx->step_lanes;
+  else return fail(ctx, ODIS_E_INVALID, "unknown option");
   return ODIS_OK;
 }
 
@@ -1517,10 +1948,10 @@ extern "C" int odis_interp_ll(odis_ctx *ctx, int inst, double *ll) {
   const dim3 grid((ctx->ll_rows + kBlock / 64 - 1) / (kBlock / 64)), blk(kBlock);
   if (ctx->precision == ODIS_FP64)
     hipLaunchKernelGGL((k_interp_ll<double>), grid, blk, 0, ctx->stream, ctx->ll_rows, ctx->ll_rp, ctx->ll_col,
-                       ctx->ll_val, ctx->d.eta_buf[ctx->ecur] + size_t(inst) * ctx->N, ctx->ll_out);
+                       ctx->ll_val, ctx->d.eta[ctx->cur] + size_t(inst) * ctx->N, ctx->ll_out);
   else
     hipLaunchKernelGGL((k_interp_ll<float>), grid, blk, 0, ctx->stream, ctx->ll_rows, ctx->ll_rp, ctx->ll_col,
-                       ctx->ll_val, ctx->f.eta_buf[ctx->ecur] + size_t(inst) * ctx->N, ctx->ll_out);
+                       ctx->ll_val, ctx->f.eta[ctx->cur] + size_t(inst) * ctx->N, ctx->ll_out);
   CK(hipGetLastError());
   CK(hipMemcpyAsync(ll, ctx->ll_out, sizeof(double) * ctx->ll_rows, hipMemcpyDeviceToHost, ctx->stream));
   CK(hipStreamSynchronize(ctx->stream));

@@ -103,6 +103,20 @@ void quantise_dt(double period, double target, double *dt, int *n_steps);
 void membrane_nu_beta(double radius, double hs, double ho, int l_max, std::vector<double> &nu,
                       std::vector<double> &beta, double *radius_ocean, double *grav_ocean);
 
+// The TRiSK operator in cell-local form (odis_compact.cpp): per cell, its
+// faces in counter-clockwise order (cyc), the kite fractions R of the
+// vertices between consecutive faces, and info = k | dir_bits << 3 (bit q
+// set: the cell is the outer node of cyc[q]).  Reference numbering, [N][6].
+struct CompactTrisk {
+  int N = 0, E = 0;
+  std::vector<int> cyc;
+  std::vector<double> R;
+  std::vector<int> info;
+  double coef_err = 0, div_err = 0, area_err = 0;  // largest relative deviations found
+};
+// Empty string on success, else why the descriptor has no compact form.
+std::string build_compact(const odis_mesh_desc &m, CompactTrisk &c);
+
 // Real 4pi-normalised spherical harmonics, k = l*l + l + m (odis_sh.cpp).
 void sh_basis_row(int L, double lat, double lon, double *Y);
 void sh_basis(int L, const std::vector<double> &lat, const std::vector<double> &lon, std::vector<double> &Yt);

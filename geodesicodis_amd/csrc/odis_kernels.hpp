@@ -5,12 +5,15 @@
 //
 //   face kernel (one lane per face e, reads state n):
 //     v_t   = sum_j c_ej v[f_j]                 TRiSK, <=10 terms (src/interpolation.cpp:43-46)
+//             compact form: (T(inner, e) - T(outer, e)) / d_e, the partial
+//             sums written by the previous cell kernel (odis_compact.cpp)
 //     diss += A_e (v_e^2 + v_t^2)               dissipation of state n (src/energy.cpp:31-44)
 //     f0    = -g (G eta)_e + fcor_e v_t         updateMomentum (src/updateMomentum.cpp:42)
 //     drag  = -alpha v_e + sum_k c_k(t+dt) (G S_k)_e      D v + G U (src/timeIntegrator.cpp:219)
 //     v'    = v + dt (a f0 + b f1 + c f2) + dt drag        AB3 + Euler drag (src/temporalOperators.cpp:36-47)
 //   cell kernel (one lane per cell i, reads v'):
 //     f0    = h (Div v')_i ; eta' = eta + dt (a f0 + b f1 + c f2)      updateEta + AB3
+//     compact form: also T(i, e) of v' for each face e of the cell
 //     workgroup 0 also sums the face kernel's dissipation partials of this
 //     step in a fixed order (deterministic FP64 result).
 //
@@ -22,8 +25,9 @@
 //
 // Layout: structure-of-arrays, stencil slot-major ([slot][face]) so each
 // slot load is one coalesced 64-lane access; AB3 history is a 3-slot ring
-// indexed by step % 3 (no history shift traffic); v is double-buffered
-// because neighbours read the old value.
+// indexed by step % 3 (no history shift traffic); v, eta and the TRiSK
+// partial sums are double-buffered (state n read, state n+1 written) because
+// neighbours read the old values.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -61,9 +65,15 @@ struct FaceArgs {
   T *ring;                   // [B][3][E]
   const T *eta;              // [B][N]
   long long v_stride, ring_stride, eta_stride;
-  double *partials;          // [B][nb]
+  double *partials;          // [2][B][nb] by step parity
   const double *inst;        // [B][8] g, alpha, dt, init_load, quad_coef, h
   int quad_drag;
+  // cell-local TRiSK form (COMPACT)
+  int N;
+  const T *fdist;            // [E] d_e
+  const T *ts;               // [B][2][E] TRiSK partial sums T(inner, e), T(outer, e) of v_cur
+  long long ts_stride;
+  const T *cellS;            // [K][N] forcing fields at the cells
 };
 
 template <typename T>
@@ -72,15 +82,25 @@ struct CellArgs {
   const int *div_idx;        // [6][N]
   const T *div_coef;         // [6][N]
   const T *v_new;            // [B][E]
-  T *eta;                    // [B][N]
+  const T *eta;              // [B][N] eta^n
+  T *eta_nxt;                // [B][N] eta^{n+1}
   T *ring;                   // [B][3][N]
   long long v_stride, eta_stride, ring_stride;
-  const double *partials;    // [B][nb_face]
+  const double *partials;    // [2][B][nb_face] by step parity
   double *series;            // [B][cap]
   int *nonfinite;            // [B]
   int cap;
   const double *inst;        // [B][8]
   const double *dconst;      // [B] dissipation constant
+  // cell-local TRiSK form (COMPACT)
+  int E;
+  const int *cyc;            // [6][N] faces counter-clockwise (pentagons: slot 5 = slot 0)
+  const T *kite;             // [6][N] kite fraction of the vertex after cyc[q]
+  const int *info;           // [N] k | dir_bits << 3 (bit q: outer node of cyc[q])
+  const T *area;             // [N] control-volume area A_i
+  const T *flen;             // [E] l_e
+  T *ts;                     // [B][2][E] TRiSK partial sums of v_new (written)
+  long long ts_stride;
 };
 
 // Workgroup i runs on XCD i % 8; give each XCD a contiguous range of

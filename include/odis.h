@@ -75,6 +75,12 @@ typedef struct odis_mesh_desc {
     const double *div_coef;     /* [N][6] -dir l_e / A_i (src/mesh.cpp:3242-3247)    */
     const double *cell_lat;     /* [N]    node latitude  (forcing fields)            */
     const double *cell_lon;     /* [N]    node longitude                             */
+    /* Optional (may be null): with them the step uses the cell-local form of
+     * the TRiSK operator (kite fractions per cell, derived from and checked
+     * against trisk_coef); without them the tabulated trisk_faces/coef.     */
+    const double *face_len;     /* [E]    l_e, Voronoi edge length (mesh.edge_len)  */
+    const double *face_dist;    /* [E]    d_e, node-to-node distance (mesh.node_dists) */
+    const double *cell_area;    /* [N]    A_i, control-volume area of div_coef      */
 } odis_mesh_desc;
 
 /* Per-instance physics, the Globals members the step reads
@@ -174,27 +180,55 @@ int odis_sh_synthesize(odis_ctx *ctx, const double *coeffs, double *field);
  * odis_step returns ODIS_E_STOPPED at the next step boundary. */
 int odis_request_stop(odis_ctx *ctx);
 
-/* Launch controls (no counterpart in the reference; results are bit for bit
- * the same in every combination):
- *  - odis_use_graphs: run chunks of 64 steps as captured HIP graphs (off by
- *    default; direct launches measured faster on MI355X);
- *  - odis_use_fused: one kernel per step that completes the previous step's
- *    continuity update before the momentum update (off by default: measured
- *    slower than the face + cell kernel pair; ignored while an SH basis is
- *    set). */
-int odis_use_graphs(odis_ctx *ctx, int on);
-int odis_use_fused(odis_ctx *ctx, int on);
+/* Launch options (no counterpart in the reference):
+ *  ODIS_OPT_GRAPHS   1: replay chunks of 64 steps from captured HIP graphs
+ *                    (default 0: direct launches measured faster on MI355X;
+ *                    results are bit for bit the same).
+ *  ODIS_OPT_COMPACT  1: cell-local TRiSK form (default when the descriptor
+ *                    carries face_len / face_dist / cell_area and the form
+ *                    reproduces trisk_coef); 0: tabulated trisk_faces/coef,
+ *                    the reference's operator coefficient by coefficient.
+ *                    The two differ by rounding only.  Setting 1 on a context
+ *                    without the compact form returns ODIS_E_UNSUPPORTED.
+ *  ODIS_OPT_ONE_KERNEL 1: with the compact form and no SH basis, one kernel
+ *                    per step over patches of cells (default); 0: a face
+ *                    and a cell kernel per step.  The state is the same bit
+ *                    for bit; dissipation sums differ by rounding (another
+ *                    grouping of the per-face terms).
+ *  ODIS_OPT_PATCH_CELLS at most 64 (default), 128 or 256 cells per patch of
+ *                    the one-kernel step.
+ *  ODIS_OPT_STEP_BLOCKS workgroups of the one-kernel step; each runs a
+ *                    contiguous run of patches (0, default: one per patch;
+ *                    reading it returns the count in use).  The dissipation
+ *                    sums depend on it by rounding only.
+ *  ODIS_OPT_STEP_LANES lanes per cell of the one-kernel step: 4 (default,
+ *                    about one face per lane) or 2 (about two). */
+#define ODIS_OPT_GRAPHS     1
+#define ODIS_OPT_COMPACT    2
+#define ODIS_OPT_ONE_KERNEL 3
+#define ODIS_OPT_PATCH_CELLS 4
+#define ODIS_OPT_STEP_BLOCKS 5
+#define ODIS_OPT_STEP_LANES  6
+int odis_set_option(odis_ctx *ctx, int option, int value);
+int odis_get_option(odis_ctx *ctx, int option, int *value);
 /* Timing for benchmarks: with stride T > 0, the kernels of every step n of
  * the next odis_step calls with n % T == 0 are launched with dispatch-stamped
  * start/stop HIP events on the context stream; odis_get_timing returns the
- * average duration (ms) of the momentum kernel (the face kernel, or the fused
- * step kernel), of the continuity kernel (the cell kernel) and of a whole
- * sampled step, over the sampled steps of the last odis_step call.  T = 0
- * disables. */
+ * average duration (ms) of the face kernel, of the cell kernel and of a whole
+ * sampled step (face start to cell end), over the sampled steps of the last
+ * odis_step call.  T = 0 disables. */
 int odis_enable_timing(odis_ctx *ctx, int stride);
 int odis_get_timing(odis_ctx *ctx, double *face_ms, double *cell_ms, double *step_ms);
 
 /* ---- host: grid and mesh (src/mesh.cpp:32-197, 436-1154, 2808-3287) ---- */
+
+/* Check of the cell-local TRiSK form on the host (odis_create runs the same
+ * check): ODIS_OK when the form reproduces every trisk_coef and div_coef to
+ * 1e-12 relative; the largest relative deviations found are returned (any
+ * pointer may be null).  ODIS_E_UNSUPPORTED with the reason in
+ * odis_host_last_error otherwise. */
+int odis_mesh_compact_check(const odis_mesh_desc *mesh, double *coef_err, double *div_err,
+                            double *area_err);
 
 /* Read a reference-format grid file (src/mesh.cpp:4016-4101). */
 int odis_grid_read(const char *path, odis_grid **out);

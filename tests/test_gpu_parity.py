@@ -396,15 +396,16 @@ def _assert_same_state(a, b, insts, where):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-def test_fused_step_matches_two_kernel_step(product, oracle, precision):
-    """The one-kernel step (continuity of step n-1 completed inside the
-    momentum kernel of step n) gives the same state, dissipation and
-    dissipation series bit for bit as the face + cell kernel pair: two
-    instances with different thickness and drag, linear and quadratic drag,
-    chunk boundaries (chunks hold up to 4096 steps), graph replays and a
-    restart from a loaded history.  In FP64 it also matches the oracle."""
+def test_compact_form_matches_tabulated_operator(product, oracle, precision):
+    """The cell-local TRiSK form (the default) and the tabulated operator
+    differ by rounding only: two instances with different thickness and
+    drag, linear and quadratic drag, across chunk boundaries (chunks hold up
+    to 4096 steps).  In the compact form, graph replays equal direct
+    launches bit for bit, and re-uploading the state (which recomputes the
+    TRiSK partial sums) continues the run bit for bit."""
     G = product
     g, _ = generated_grid(4)
+    tol = 1e-12 if precision == 0 else 1e-4
     for kw in (dict(), dict(friction=G.QUADRATIC, alpha=2e-3, quad_drag=1, ecc=0.05)):
         ps = []
         for h, fa in ((38e3, 1.0), (20e3, 3.0)):
@@ -412,34 +413,109 @@ def test_fused_step_matches_two_kernel_step(product, oracle, precision):
             p.alpha *= fa
             ps.append(p)
         mesh = G.Mesh(g, ps[0].radius)
-        fused = G.Context(mesh, ps, precision=precision)
-        split = G.Context(mesh, ps, precision=precision)
-        fused.use_fused(True)
-        split.use_fused(False)
-        for n in (1, 1, 2, 3, 64, 4100, 7):
-            fused.step(n)
-            split.step(n)
-            _assert_same_state(fused, split, (0, 1), (kw, n))
-            assert np.array_equal(fused.dissipation(), split.dissipation())
-        last = fused.get_state(0)["iter"]
-        for b in (0, 1):
-            sa = fused.dissipation_series(b, last - 4000, 4000)
-            sb = split.dissipation_series(b, last - 4000, 4000)
-            assert np.array_equal(sa, sb)
-        fused.use_graphs(True)
-        for n in (64, 64, 65, 128):
-            fused.step(n)
-            split.step(n)
-            _assert_same_state(fused, split, (0, 1), (kw, "graph", n))
-        st = split.get_state(1)
-        for c in (fused, split):
-            c.set_state(0, st["v"], st["dvdt"], st["eta"], st["detadt"], st["iter"])
+        comp = G.Context(mesh, ps, precision=precision)
+        tab = G.Context(mesh, ps, precision=precision)
+        graph = G.Context(mesh, ps, precision=precision)
+        assert comp.option(G.OPT_COMPACT) == 1
+        tab.use_compact(False)
+        graph.use_graphs(True)
+        for n in (1, 1, 2, 3, 64, 4100, 7, 128):
+            for c in (comp, tab, graph):
+                c.step(n)
+            for i in (0, 1):
+                a, b = comp.get_state(i), tab.get_state(i)
+                for k in ("v", "eta", "dvdt", "detadt"):
+                    assert rel_l2(a[k], b[k]) <= tol, (kw, n, i, k, rel_l2(a[k], b[k]))
+            _assert_same_state(comp, graph, (0, 1), (kw, "graph", n))
+            da, db = comp.dissipation(), tab.dissipation()
+            assert np.all(np.abs(da - db) <= 10 * tol * np.abs(db))
+            assert np.array_equal(da, graph.dissipation())
+        last = comp.get_state(0)["iter"]
+        sa = comp.dissipation_series(1, last - 4000, 4000)
+        sb = tab.dissipation_series(1, last - 4000, 4000)
+        assert np.all(np.abs(sa - sb) <= 10 * tol * np.abs(sb))
+        # re-upload the state of both instances: bit-for-bit continuation
+        for i in (0, 1):
+            st = comp.get_state(i)
+            comp.set_state(i, st["v"], st["dvdt"], st["eta"], st["detadt"], st["iter"])
+        for c in (comp, graph):
             c.step(37)
-        _assert_same_state(fused, split, (0, 1), (kw, "load"))
+        _assert_same_state(comp, graph, (0, 1), (kw, "reload"))
     if precision == 0:
         ctx, sim, _, _, _ = _pair(product, oracle, 4, dict(tide=product.ECC))
-        ctx.use_fused(True)
+        ctx.use_compact(False)
         for n in (1, 2, 300):
             ctx.step(n)
             sim.step(n)
             _compare(ctx, sim, TOL_SHORT)
+
+
+def test_kernel_timing_does_not_change_results(product):
+    """Dispatch-stamped events on sampled steps leave the state bit for bit
+    unchanged and report positive kernel times."""
+    G = product
+    g, _ = generated_grid(5)
+    p, _ = G.make_params()
+    mesh = G.Mesh(g, p.radius)
+    for one in (True, False):
+        a, b = G.Context(mesh, p), G.Context(mesh, p)
+        for c in (a, b):
+            c.use_one_kernel(one)
+        a.enable_timing(3)
+        a.step(100)
+        b.step(100)
+        t = a.timing()
+        assert t["face_ms"] > 0 and t["step_ms"] >= t["face_ms"]
+        # one-kernel steps have no separate cell kernel
+        assert (t["cell_ms"] == 0) if one else (t["cell_ms"] > 0)
+        _assert_same_state(a, b, (0,), ("timing", one))
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_one_kernel_step_matches_face_cell_pair(product, precision):
+    """One kernel per step over patches of 128 cells (boundary faces updated
+    by both patches, written by one) leaves the same state bit for bit as the
+    face + cell kernel pair; the dissipation differs only by the grouping of
+    its sum.  Covers two instances, quadratic drag, chunk boundaries, graph
+    replays and a state re-upload."""
+    G = product
+    for level in (3, 5):
+        g, _ = generated_grid(level)
+        ps = []
+        for h, fa in ((38e3, 1.0), (20e3, 3.0)):
+            p, _ = G.make_params(tide=G.ECC, h=h, friction=G.QUADRATIC, alpha=2e-3, quad_drag=1, ecc=0.05)
+            p.alpha *= fa
+            ps.append(p)
+        mesh = G.Mesh(g, ps[0].radius)
+        one = G.Context(mesh, ps, precision=precision)
+        pair = G.Context(mesh, ps, precision=precision)
+        assert one.option(G.OPT_ONE_KERNEL) == 1
+        pair.use_one_kernel(False)
+        for n in (1, 1, 2, 3, 64, 4100, 7):
+            one.step(n)
+            pair.step(n)
+            _assert_same_state(one, pair, (0, 1), (level, n))
+            da, db = one.dissipation(), pair.dissipation()
+            assert np.array_equal(da, db)  # same kernel for the current-state query
+        last = one.get_state(0)["iter"]
+        for b in (0, 1):
+            sa = one.dissipation_series(b, last - 4000, 4000)
+            sb = pair.dissipation_series(b, last - 4000, 4000)
+            assert np.all(np.abs(sa - sb) <= 1e-13 * np.abs(sb)), b
+        one.use_graphs(True)
+        for n in (64, 128, 3):
+            one.step(n)
+            pair.step(n)
+            _assert_same_state(one, pair, (0, 1), (level, "graph", n))
+        st = pair.get_state(1)
+        for c in (one, pair):
+            c.set_state(0, st["v"], st["dvdt"], st["eta"], st["detadt"], st["iter"])
+            c.step(21)
+        _assert_same_state(one, pair, (0, 1), (level, "reload"))
+        for cells, lanes, blocks in ((64, 2, 0), (256, 4, 0), (128, 2, 7), (64, 4, 0)):
+            one.set_option(G.OPT_PATCH_CELLS, cells)
+            one.set_option(G.OPT_STEP_LANES, lanes)
+            one.set_option(G.OPT_STEP_BLOCKS, blocks)
+            one.step(33)
+            pair.step(33)
+            _assert_same_state(one, pair, (0, 1), (level, "patch", cells, lanes, blocks))

@@ -201,4 +201,56 @@ def test_params_layout_matches_header(product):
     import ctypes
     assert ctypes.sizeof(product.Params) == 11 * 8 + 6 * 4 + 0
     assert [f for f, _ in product.Params._fields_][:3] == ["g", "h", "alpha"]
-    assert ctypes.sizeof(product.MeshDesc) == 8 + 11 * 8
+    assert ctypes.sizeof(product.MeshDesc) == 8 + 14 * 8
+
+
+@pytest.mark.parametrize("level", [1, 2, 3, 4, 5, 6])
+def test_compact_trisk_form_on_generated_grids(product, level):
+    """The cell-local TRiSK form (counter-clockwise face cycle + kite
+    fractions per cell) reproduces every tabulated TRiSK coefficient and
+    every divergence coefficient; face_area is d * l exactly."""
+    g, _ = generated_grid(level)
+    ok, why, ce, de, ae = product.Mesh(g, 252.1e3).compact_check()
+    assert ok, why
+    assert ce < 1e-14 and de < 1e-15 and ae == 0.0, (ce, de, ae)
+
+
+@pytest.mark.skipif(not have_reference_grids(), reason="reference grid files not mounted")
+@pytest.mark.parametrize("level", [3, 4, 5, 6])
+def test_compact_trisk_form_on_reference_grids(product, level):
+    g = product.Grid.read(os.path.join(REFERENCE_INPUTS, "grid_l%d.txt" % level))
+    ok, why, ce, de, ae = product.Mesh(g, 252.1e3).compact_check()
+    assert ok, why
+    assert ce < 1e-14 and de < 1e-15 and ae == 0.0, (ce, de, ae)
+
+
+def test_compact_trisk_form_rejects_other_operators(product):
+    """A descriptor whose TRiSK coefficients are not of the kite-fraction
+    form (one coefficient changed), whose stencil order is not a rotation
+    of one cycle (two slots swapped) or that lacks the lengths is refused,
+    so the context falls back to the tabulated operator."""
+    import ctypes
+    g, _ = generated_grid(3)
+    mesh = product.Mesh(g, 252.1e3)
+    d = mesh.describe()
+    E = d.n_faces
+    coef = np.ctypeslib.as_array(ctypes.cast(d.trisk_coef, ctypes.POINTER(ctypes.c_double)),
+                                 shape=(E * 10,)).copy()
+    faces = np.ctypeslib.as_array(ctypes.cast(d.trisk_faces, ctypes.POINTER(ctypes.c_int)),
+                                  shape=(E * 10,)).copy()
+    bad = coef.copy()
+    bad[10 * 7 + 2] *= 1.0 + 1e-9
+    d2 = mesh.describe()
+    d2.trisk_coef = bad.ctypes.data
+    ok, why, ce, _, _ = mesh.compact_check(d2)
+    assert not ok and "reproduce" in why and ce > 1e-10
+    sw = faces.copy()
+    sw[10 * 7 + 1], sw[10 * 7 + 2] = sw[10 * 7 + 2], sw[10 * 7 + 1]
+    d3 = mesh.describe()
+    d3.trisk_faces = sw.ctypes.data
+    ok, why, _, _, _ = mesh.compact_check(d3)
+    assert not ok and "rotation" in why, why
+    d4 = mesh.describe()
+    d4.face_len = None
+    ok, why, _, _, _ = mesh.compact_check(d4)
+    assert not ok and "face_len" in why
