@@ -11,7 +11,7 @@
  * file therefore restates the algorithm from the reference sources.  Pins:
  *
  *   part                          reference                         pinned by
- *   geometry primitives           include/mathRoutines.h:19-432     closed forms (tests/test_oracle_geometry.py)
+ *   geometry primitives           include/mathRoutines.h:19-432     closed forms (tests/test_oracle_pins.py)
  *   grid ingest / topology        src/mesh.cpp:436-506, 508-666     Euler counts, V=2N-4, E=3N-6; reference grid files
  *   control-volume area           src/mesh.cpp:1828-1867            sum of areas vs 4 pi R^2 on reference grids
  *   face length / node distance   src/mesh.cpp:545-623              sum d_e l_e / 4 pi R^2 = 1.99873 (l4), 1.99992 (l6)
