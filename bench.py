@@ -1,32 +1,71 @@
 """Benchmark of the LTE time step on one MI355X.
 
-Workload (default): the ab3Explicit step of GeodesicODIS on the 163842-cell
-icosahedral-geodesic grid (reference GRID_LVL 8, "level 7" in cells
-10*4^(l)+2 numbering), Enceladus ocean (R = 252.1 km, g = 0.113 m/s^2,
-Omega = 5.307e-5 rad/s, h = 38 km, linear drag 1e-8 s^-1, eccentricity tide
-e = 0.0047), rest start, FP64 plain method, dt = P/18300 = 6.46962 s.  The
-time step is the reference's quantised dt (P / N_s, N_s a multiple of 100)
-for a 6.5 s target: the reference's scheme (AB3 momentum, then AB3
-continuity on the updated velocity) is stable for omega_max dt < 0.546 and
-omega_max = 0.078 s^-1 on this grid, so 8 s steps diverge.  A "step" is one
-pass of the loop body over all cells and faces.
+Default workload ("shell"): the ab3Explicit step of GeodesicODIS on the
+163842-cell icosahedral-geodesic grid (reference GRID_LVL 8, "level 7" in
+10*4^l+2 numbering).  An Enceladus ocean lies under a 20 km elastic ice
+shell (LID_MEMBR: R = 252.1 km body, ocean top 232.1 km, h = 38 km,
+linear drag 1e-8 s^-1, eccentricity tide e = 0.0047).  The pressure term
+sees the self-gravity / shell coupling through the spherical-harmonic
+projection up to degree 32 (1089 harmonics), weighted by the membrane
+factors beta_l.  Rest start.  A "step" is one pass of the loop body over
+all cells and faces, SH projection included.
 
-Prints one JSON line: cell-steps/s of the whole job, the roofline of the
-dominant kernel (the one-kernel step, or the longer of the face and cell
-kernels; HIP-event timed on every 16th step inside the timed region) and a
-CPU baseline (the FP64 C oracle, OpenMP, on a bounded sample of the same
-workload).
+Time step: the reference's quantised dt (P / N_s, N_s a multiple of 100) for
+a 0.4 s target (N_s = 296000).  The shell stiffens the high degrees
+(beta_32 = 1.7e4), so the degree-32 gravity wave has omega = 1.2 s^-1.  The
+reference scheme (AB3 momentum, then AB3 continuity on the new velocity) is
+stable for omega dt < 0.546.
+
+`value` is the FP64 plain method (the drop-in).  Its SH basis is evaluated
+on the fly by the Legendre recurrence in FP64 (`--sh-basis stored` reads a
+stored FP64 basis instead).  The same line carries:
+  - the FP32 form on the same workload (FP32 stencils and recurrence, or a
+    stored split-BF16 basis with --sh-basis stored), with its error against
+    FP64 after the same steps;
+  - the free-surface ocean without the shell (workload "free": dt = P/18300,
+    where the stencils alone are timed).
+
+Prints one JSON line: cell-steps/s of the whole job; the roofline of the
+dominant kernel (HIP events stamped on the kernel dispatches of every 16th
+step, inside the timed region); and a CPU baseline (the FP64 C oracle with
+OpenMP, with the same projection as dense products, on a bounded sample of
+the same workload).
 """
 import argparse
 import json
 import os
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+SH_DEGREE = 32
+SHELL_INPUT = """radius; 252.1e3; top radius;
+k2; 0.0; k2;
+h2; 0.0; h2;
+love reduction factor; 1.0; set by the surface condition;
+ANGULAR VELOCITY; 5.307e-5; mean motion;
+surface gravity; 0.113; g;
+eccentricity; 0.0047; e;
+obliquity; 0.0; degrees;
+ocean thickness; 38e3; h_o;
+shell thickness; 20e3; h_s;
+sh degree; %d; l_max;
+friction coefficient; 1e-8; alpha;
+friction type; LINEAR; friction;
+potential; ECC; potential;
+surface type; LID_MEMBR; shell;
+advection; false; advection;
+solver type; AB3; solver;
+geodesic grid level; 8; level;
+output time; 10; out;
+initial conditions; NONE; ic;
+simulation end time; 20; orbits;
+time step; 0.4; dt target;
+""" % SH_DEGREE
 
 
 def face_bytes(E, N, nfields, p, compact):
@@ -65,6 +104,28 @@ def step_kernel_bytes(E, N, nfields, p):
     return face_bytes(E, N, nfields, p, True) + E * p + N * (6 * 2 + 6 * p + 4 + p + p + 3 * p) + E * 2 * p
 
 
+def sh_bytes(N, K, p, pb, chunks=8):
+    """Algorithmic bytes of the SH kernels per step: analysis (basis once,
+    eta once, chunk partials written), coefficients (the K x K inverse Gram
+    matrix, partials, factors), synthesis (basis once, eta read, eta_eff
+    written).  pb = bytes per basis entry (8 FP64, 4 split-BF16 word; 0 for
+    the basis evaluated on the fly, which reads 4 doubles per cell)."""
+    cell = 4 * 8 if pb == 0 else 0
+    return {"sh_analysis": K * N * pb + N * (p + cell) + chunks * K * 8,
+            "sh_coef": K * K * 8 + chunks * K * 8 + 2 * K * 8,
+            "sh_synth": K * N * pb + N * (2 * p + cell) + K * 8}
+
+
+def sh_flops(N, L):
+    """Algorithmic flops of one on-the-fly pass (analysis or synthesis) with
+    the rescaled Legendre recurrence q_l = mu q_{l-1} + beta_l q_{l-2}: per
+    cell and (l, m) the recurrence step (one multiplication, one
+    multiply-add: 3 flops) and the multiply-adds of the cos part and, for
+    m > 0, the sin part (2 flops each)."""
+    nlm = (L + 1) * (L + 2) // 2
+    return N * (3 * nlm + 2 * nlm + 2 * (nlm - (L + 1)))
+
+
 def measured_traffic(kname, tname, n_cells):
     """HBM bytes per launch of the kernel from the committed rocprofv3 PMC
     summary (FETCH_SIZE and WRITE_SIZE, copy-calibrated), when one exists for
@@ -81,16 +142,86 @@ def measured_traffic(kname, tname, n_cells):
     return None
 
 
+def timed_run(ctx, steps, warmup, stride):
+    """Warm-up, then `steps` timed steps between synchronisations; returns
+    (seconds, kernel times)."""
+    ctx.step(warmup, sync=True)
+    ctx.enable_timing(stride)
+    ctx.sync()
+    t1 = time.perf_counter()
+    ctx.step(steps, sync=True)
+    ctx.sync()
+    elapsed = time.perf_counter() - t1
+    tm = ctx.timing()
+    ctx.enable_timing(0)
+    return elapsed, tm
+
+
+def shell_setup(G):
+    """Params of the shell workload through the reference's input.in rules
+    (membraneNuBeta moves g and R to the ocean top) and its beta_l."""
+    d = tempfile.mkdtemp(prefix="odis_bench_")
+    with open(os.path.join(d, "input.in"), "w") as f:
+        f.write(SHELL_INPUT)
+    p, info = G.read_config(d)
+    beta = G.surface_beta(G.LID_MEMBR, SH_DEGREE, 252.1e3, 0.113, 20e3, 38e3)
+    return p, info["steps_per_orbit"], beta
+
+
+def apply_launch_options(G, ctx, args):
+    ctx.use_compact(not args.tabulated)
+    ctx.use_one_kernel(not args.tabulated and not args.face_cell)
+    ctx.use_graphs(args.graphs)
+    if not args.tabulated and not args.face_cell:
+        if args.patch_cells:
+            ctx.set_option(G.OPT_PATCH_CELLS, args.patch_cells)
+        if args.step_lanes:
+            ctx.set_option(G.OPT_STEP_LANES, args.step_lanes)
+        if args.step_blocks:
+            ctx.set_option(G.OPT_STEP_BLOCKS, args.step_blocks)
+
+
+def stencil_roofline(G, ctx, mesh, tm, p_bytes, one):
+    """Kernel name, bytes and time of the dominant stencil kernel, and the
+    algorithmic bytes of the stencil kernels per step."""
+    nfields = 2
+    compact = ctx.option(G.OPT_COMPACT) == 1
+    if one and compact:
+        sb = step_kernel_bytes(mesh.E, mesh.N, nfields, p_bytes)
+        return "k_step", sb, tm["face_ms"], {"step": sb}
+    fb = face_bytes(mesh.E, mesh.N, nfields, p_bytes, compact)
+    cb = cell_bytes(mesh.E, mesh.N, p_bytes, compact)
+    if tm["cell_ms"] > tm["face_ms"]:
+        return "k_cell", cb, tm["cell_ms"], {"face": fb, "cell": cb}
+    return "k_face", fb, tm["face_ms"], {"face": fb, "cell": cb}
+
+
+def rel_l2(a, b):
+    import numpy as np
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--workload", choices=["shell", "free"], default="shell",
+                    help="shell: ocean under a 20 km ice shell with SH coupling to degree 32 (default); "
+                         "free: free-surface ocean, stencils only")
     ap.add_argument("--level", type=int, default=8, help="reference grid level (8 = 163842 cells)")
-    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
+                    help="precision of the headline value (fp32: FP32 stencils, split-BF16 SH basis)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU work of the baseline sample")
+    ap.add_argument("--sh-basis", choices=["recur", "stored"], default="recur",
+                    help="SH basis evaluated on the fly by the Legendre recurrence (default) or stored "
+                         "(FP64, or split-BF16 for fp32)")
+    ap.add_argument("--sh-cells", type=int, default=0,
+                    help="cells per lane of the on-the-fly SH analysis (4, 8, 16; 0: library default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="only the headline measurement (no fast form / free-surface lines)")
     ap.add_argument("--tabulated", action="store_true",
                     help="tabulated TRiSK operator (10 indices and coefficients per face) instead of the "
                          "cell-local form")
@@ -125,56 +256,86 @@ def main():
               "(requested --gpus %d)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    import numpy as np
     import geodesicodis_amd as G
 
     t0 = time.time()
     grid = G.Grid.generate(args.level, 64)
-    p, steps_per_orbit = G.make_params(time_step=6.5)
+    shell = args.workload == "shell"
+    if shell:
+        p, steps_per_orbit, beta = shell_setup(G)
+    else:
+        p, steps_per_orbit = G.make_params(time_step=6.5)
+        beta = None
     mesh = G.Mesh(grid, p.radius)
-    prec = G.FP64 if args.precision == "fp64" else G.FP32
-    pbytes = 8 if prec == G.FP64 else 4
-    ctx = G.Context(mesh, p, precision=prec)
-    ctx.use_compact(not args.tabulated)
-    one = not args.tabulated and not args.face_cell
-    ctx.use_one_kernel(one)
-    if one and args.patch_cells:
-        ctx.set_option(G.OPT_PATCH_CELLS, args.patch_cells)
-    if one and args.step_lanes:
-        ctx.set_option(G.OPT_STEP_LANES, args.step_lanes)
-    if one and args.step_blocks:
-        ctx.set_option(G.OPT_STEP_BLOCKS, args.step_blocks)
-    ctx.use_graphs(args.graphs)
-    print("bench: grid %d cells / %d faces ready in %.1fs" % (mesh.N, mesh.E, time.time() - t0),
+    print("bench: %s workload, grid %d cells / %d faces ready in %.1fs" % (args.workload, mesh.N, mesh.E,
+                                                                          time.time() - t0),
           file=sys.stderr, flush=True)
+    K = (SH_DEGREE + 1) ** 2
 
-    ctx.step(args.warmup, sync=True)
-    ctx.enable_timing(args.timing_stride)  # dispatch-stamped HIP events on the kernels
-    ctx.sync()
-    t1 = time.perf_counter()
-    ctx.step(args.steps, sync=True)
-    ctx.sync()
-    elapsed = time.perf_counter() - t1
-    tm = ctx.timing()
-    ctx.enable_timing(0)
-    diss = float(ctx.dissipation()[0])
+    def make_ctx(prec):
+        ctx = G.Context(mesh, p, precision=prec)
+        apply_launch_options(G, ctx, args)
+        if shell:
+            if args.sh_basis == "recur":
+                if args.sh_cells:
+                    ctx.set_option(G.OPT_SH_CELLS, args.sh_cells)
+                ctx.set_sh(SH_DEGREE, beta, mode=G.SH_RECUR)
+            else:
+                ctx.set_sh(SH_DEGREE, beta, mode=G.SH_FP64 if prec == G.FP64 else G.SH_SPLIT_BF16)
+        return ctx
+
+    prec = G.FP64 if args.precision == "fp64" else G.FP32
+    p_bytes = 8 if prec == G.FP64 else 4
+    ctx = make_ctx(prec)
+    elapsed, tm = timed_run(ctx, args.steps, args.warmup, args.timing_stride)
     ms_per_step = 1e3 * elapsed / args.steps
     value = mesh.N * args.steps / elapsed
-
-    nfields = 2
-    compact = not args.tabulated
-    fb = face_bytes(mesh.E, mesh.N, nfields, pbytes, compact)
-    cb = cell_bytes(mesh.E, mesh.N, pbytes, compact)
-    tname = ("double" if prec == G.FP64 else "float") + (",compact" if compact else ",tabulated")
-    if one:
-        sb = step_kernel_bytes(mesh.E, mesh.N, nfields, pbytes)
-        fb, cb = sb, 0
-        kname, kbytes, kms = "k_step", sb, tm["face_ms"]
-    elif tm["cell_ms"] > tm["face_ms"]:  # dominant kernel: the longer of the two per step
-        kname, kbytes, kms = "k_cell", cb, tm["cell_ms"]
+    diss = ctx.dissipation()
+    state = ctx.get_state()
+    print("bench: headline %.3e cell-steps/s (%.1f us/step)" % (value, 1e3 * ms_per_step),
+          file=sys.stderr, flush=True)
+    one = ctx.option(G.OPT_ONE_KERNEL) == 1 and not shell
+    tname = ("double" if prec == G.FP64 else "float") + (",compact" if ctx.option(G.OPT_COMPACT) else ",tabulated")
+    kname, kbytes, kms, stencil_alg = stencil_roofline(G, ctx, mesh, tm, p_bytes, one)
+    kernels = {"stencil": {"kernel": "%s<%s>" % (kname, tname), "ms": kms, "alg_bytes": kbytes}}
+    alg = dict(stencil_alg)
+    recur = shell and args.sh_basis == "recur"
+    if shell:
+        pb = 0 if recur else (8 if prec == G.FP64 else 4)
+        compiled = SH_DEGREE in (8, 16, 32)
+        # partials: one per workgroup of 4 waves x 64 lanes x 16 cells (pair
+        # analysis, compiled degrees) or of 8 chunks of 64 x 8 cells
+        cpl = 4 * 64 * (args.sh_cells or 16) if compiled else 8 * 64 * (args.sh_cells or 8)
+        chunks = (mesh.N + cpl - 1) // cpl if recur else 8
+        shb = sh_bytes(mesh.N, K, p_bytes, pb, chunks)
+        alg.update(shb)
+        sh_t = {"sh_analysis": tm["sh_analysis_ms"], "sh_coef": tm["sh_coef_ms"], "sh_synth": tm["sh_synth_ms"]}
+        tn = "double" if prec == G.FP64 else "float"
+        for k, v in sh_t.items():
+            if recur and k != "sh_coef":
+                kname_sh = ({"sh_analysis": "k_sh_analysis_pair", "sh_synth": "k_sh_synth_chain"}[k] if compiled
+                            else "k_%s_rec" % k)
+                kernels[k] = {"kernel": "%s<%s>" % (kname_sh, tn), "ms": v, "alg_bytes": shb[k],
+                              "alg_flops": sh_flops(mesh.N, SH_DEGREE)}
+            else:
+                kernels[k] = {"kernel": "k_%s<%s>" % (k, tn if recur else ("double" if prec == G.FP64 else
+                                                                           "split_bf16")),
+                              "ms": v, "alg_bytes": shb[k]}
+    # dominant kernel: the longest per step; the on-the-fly SH passes are
+    # bound by FP64 (FP32) vector arithmetic, the others by HBM bytes
+    dom = max(kernels.values(), key=lambda r: r["ms"])
+    if "alg_flops" in dom:
+        peak = 78.6 if prec == G.FP64 else 157.3
+        achieved = dom["alg_flops"] / (dom["ms"] * 1e-3) / 1e12 if dom["ms"] > 0 else None
+        bound, unit = ("fp64" if prec == G.FP64 else "fp32"), "TFLOP/s"
     else:
-        kname, kbytes, kms = "k_face", fb, tm["face_ms"]
-    achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else None
-    traffic = measured_traffic(kname, tname, mesh.N)
+        peak = HBM_PEAK_GBS
+        achieved = dom["alg_bytes"] / (dom["ms"] * 1e-3) / 1e9 if dom["ms"] > 0 else None
+        bound, unit = "hbm", "GB/s"
+    dk, dt_ = dom["kernel"].split("<", 1)
+    traffic = measured_traffic(dk, dt_[:-1], mesh.N)
+    step_bytes = sum(alg.values())
     out = {
         "metric": "cell-steps/s at level 7 (163842 cells)",
         "value": value,
@@ -189,45 +350,101 @@ def main():
         "dtype": "f64" if prec == G.FP64 else "f32",
         "data": "synthetic (generated SCVT grid, rest start)",
         "config": {
-            "workload": "GeodesicODIS LTE AB3 step, plain method, single instance, "
-                        "Enceladus free-surface ocean, eccentricity tide, no self-gravity",
+            "workload": ("GeodesicODIS LTE AB3 step, plain method, single instance, Enceladus ocean under a 20 km "
+                         "ice shell (LID_MEMBR), self-gravity / shell coupling by SH projection to degree %d "
+                         "(%d harmonics), eccentricity tide" % (SH_DEGREE, K)) if shell else
+                        ("GeodesicODIS LTE AB3 step, plain method, single instance, Enceladus free-surface ocean, "
+                         "eccentricity tide, no self-gravity"),
             "cells": mesh.N, "faces": mesh.E, "instances": 1,
             "grid": "icosahedral bisection + 64 Lloyd sweeps, reference level %d" % args.level,
             "dt_s": p.dt, "steps_per_orbit": steps_per_orbit,
-            "trisk": "cell-local (kite fractions)" if compact else "tabulated",
-            "launch": ("one kernel per step (patches of <= %d cells, %d lanes per cell, %d workgroups)"
-                       % (ctx.option(G.OPT_PATCH_CELLS), ctx.option(G.OPT_STEP_LANES),
-                          ctx.option(G.OPT_STEP_BLOCKS))) if one
-                      else "face + cell kernel per step",
+            "sh_basis": (("on the fly (Legendre recurrence, %s)" % ("fp64" if prec == G.FP64 else "fp32")) if recur
+                         else ("stored fp64" if prec == G.FP64 else "stored split-bf16")) if shell else None,
+            "trisk": "cell-local (kite fractions)" if ctx.option(G.OPT_COMPACT) else "tabulated",
+            "launch": (("one kernel per step (patches of <= %d cells, %d lanes per cell, %d workgroups)"
+                        % (ctx.option(G.OPT_PATCH_CELLS), ctx.option(G.OPT_STEP_LANES),
+                           ctx.option(G.OPT_STEP_BLOCKS))) if one else
+                       ("SH analysis, coefficients, synthesis + face + cell kernel per step" if shell else
+                        "face + cell kernel per step")),
             "graphs": bool(args.graphs),
-            "kernel_ms": {"face": tm["face_ms"], "cell": tm["cell_ms"], "step_events": tm["step_ms"]},
-            "alg_bytes": {"face": fb, "cell": cb, "step": fb + cb},
-            "hbm_gbs_step_events": (fb + cb) / (tm["step_ms"] * 1e-3) / 1e9 if tm["step_ms"] > 0 else None,
-            "dissipation_W_m2": diss,
+            "kernel_ms": {k: r["ms"] for k, r in kernels.items()},
+            "step_events_ms": tm["step_ms"],
+            "alg_bytes": alg,
+            "alg_bytes_per_step": step_bytes,
+            "hbm_gbs_per_step": step_bytes / (ms_per_step * 1e-3) / 1e9,
+            "dissipation_W_m2": float(diss[0]),
         },
         "roofline": {
-            "bound": "hbm",
+            "bound": bound,
             "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS if achieved else None,
+            "peak": peak,
+            "unit": unit,
+            "frac": achieved / peak if achieved else None,
             "traffic": traffic,
-            "kernel": "%s<%s>" % (kname, tname),
-            "alg_bytes_per_launch": kbytes,
+            "kernel": dom["kernel"],
+            "alg_bytes_per_launch": dom["alg_bytes"],
         },
     }
     ctx.close()
+
+    if not args.no_extra:
+        if shell and prec == G.FP64:
+            # the fast form on the same workload, and its error against FP64
+            fctx = make_ctx(G.FP32)
+            fel, ftm = timed_run(fctx, args.steps, args.warmup, args.timing_stride)
+            fs = fctx.get_state()
+            fd = fctx.dissipation()
+            out["config"]["fp32_form"] = {
+                "sh_basis": "on the fly (Legendre recurrence, fp32)" if args.sh_basis == "recur" else
+                            "stored split-bf16",
+                "value": mesh.N * args.steps / fel, "ms_per_step": 1e3 * fel / args.steps,
+                "kernel_ms": {"face": ftm["face_ms"], "cell": ftm["cell_ms"], "sh_analysis": ftm["sh_analysis_ms"],
+                              "sh_coef": ftm["sh_coef_ms"], "sh_synth": ftm["sh_synth_ms"]},
+                "rel_err_vs_fp64_after_%d_steps" % (args.steps + args.warmup): {
+                    "eta": rel_l2(fs["eta"], state["eta"]), "v": rel_l2(fs["v"], state["v"]),
+                    "dissipation": abs(float(fd[0]) - float(diss[0])) / abs(float(diss[0]))},
+            }
+            fctx.close()
+        if shell:
+            # the stencils alone: free-surface ocean at its own stable step
+            pf, _ = G.make_params(time_step=6.5)
+            fmesh = G.Mesh(grid, pf.radius)
+            for name, fprec in (("free_surface_fp64", G.FP64), ("free_surface_fp32", G.FP32)):
+                c2 = G.Context(fmesh, pf, precision=fprec)
+                apply_launch_options(G, c2, args)
+                el2, tm2 = timed_run(c2, args.steps, args.warmup, args.timing_stride)
+                one2 = c2.option(G.OPT_ONE_KERNEL) == 1
+                tn2 = ("double" if fprec == G.FP64 else "float") + ",compact"
+                kn, kb, km, _ = stencil_roofline(G, c2, fmesh, tm2, 8 if fprec == G.FP64 else 4, one2)
+                out["config"][name] = {
+                    "value": fmesh.N * args.steps / el2, "ms_per_step": 1e3 * el2 / args.steps,
+                    "kernel": "%s<%s>" % (kn, tn2), "kernel_ms": km, "alg_bytes": kb,
+                    "hbm_frac": (kb / (km * 1e-3) / 1e9 / HBM_PEAK_GBS) if km > 0 else None}
+                c2.close()
 
     if not args.no_cpu_baseline:
         import oracle as O
         threads = min(16, os.cpu_count() or 1)
         os.environ.setdefault("OMP_NUM_THREADS", str(threads))
-        om = O.Mesh(grid.arrays(), p.radius)
+        arrs = grid.arrays()
+        om = O.Mesh(arrs, p.radius)
         sim = O.Sim(om, O.make_params(g=p.g, h=p.h, alpha=p.alpha, omega=p.omega, radius=p.radius,
-                                      ecc=p.ecc, dt=p.dt, tide=p.tide))
-        sim.step(5)
+                                      love_reduct=p.love_reduct, ecc=p.ecc, dt=p.dt, tide=p.tide,
+                                      surface=p.surface, shell_thickness=p.shell_thickness))
+        setup = 0.0
+        if shell:
+            ts = time.perf_counter()
+            Y = O.real_sh_basis(SH_DEGREE, arrs["lat"], arrs["lon"])
+            Pc = np.linalg.solve(Y.T @ Y, Y.T)
+            dfac = np.zeros(K)
+            for l in range(2, SH_DEGREE + 1):
+                dfac[l * l:(l + 1) ** 2] = beta[l] - 1.0
+            sim.set_projector(Y * dfac[None, :], Pc)
+            del Y, Pc
+            setup = time.perf_counter() - ts
+        sim.step(2)
         n, t_cpu = 0, 0.0
-        batch = 10
+        batch = 2 if shell else 10
         while t_cpu < args.cpu_seconds and n < 5000:
             ts = time.perf_counter()
             sim.step(batch)
@@ -238,8 +455,9 @@ def main():
             "unit": "cell-steps/s",
             "cores": int(os.environ.get("OMP_NUM_THREADS", threads)),
             "kind": "port",
-            "sample": "FP64 C oracle (OpenMP rows), same %d-cell grid and parameters, %d steps "
-                      "after 5 warm-up steps" % (mesh.N, n),
+            "sample": ("FP64 C oracle (OpenMP rows%s), same %d-cell grid and parameters, %d steps after 2 warm-up "
+                       "steps%s" % (", SH projection as dense LSQ analysis and synthesis products" if shell else "",
+                                    mesh.N, n, " (projector set-up %.1f s not timed)" % setup if shell else "")),
         }
     print(json.dumps(out), flush=True)
 

@@ -37,10 +37,11 @@ EXPORTS = [
     "odis_round_period", "odis_quantise_dt", "odis_config_read", "odis_host_last_error",
     "odis_run_case", "odis_interp_ll_set", "odis_interp_ll", "odis_sh_set_basis",
     "odis_sh_project", "odis_sh_synthesize", "odis_surface_beta", "odis_set_option",
-    "odis_get_option", "odis_mesh_compact_check",
+    "odis_get_option", "odis_mesh_compact_check", "odis_get_kernel_times",
 ]
-SH_OFF, SH_FP64, SH_SPLIT_BF16 = 0, 1, 2
-OPT_GRAPHS, OPT_COMPACT, OPT_ONE_KERNEL, OPT_PATCH_CELLS, OPT_STEP_BLOCKS, OPT_STEP_LANES = 1, 2, 3, 4, 5, 6
+SH_OFF, SH_FP64, SH_SPLIT_BF16, SH_RECUR = 0, 1, 2, 3
+OPT_GRAPHS, OPT_COMPACT, OPT_ONE_KERNEL, OPT_PATCH_CELLS, OPT_STEP_BLOCKS, OPT_STEP_LANES, OPT_SH_CELLS = \
+    1, 2, 3, 4, 5, 6, 7
 
 
 class OdisError(RuntimeError):
@@ -105,6 +106,7 @@ def lib():
             "odis_set_option": ([vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
             "odis_get_option": ([vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
             "odis_mesh_compact_check": ([vp, vp, vp, vp], ctypes.c_int),
+            "odis_get_kernel_times": ([vp, vp, ctypes.c_int], ctypes.c_int),
             "odis_get_timing": ([vp, dp, dp, dp], ctypes.c_int),
             "odis_grid_read": ([ctypes.c_char_p, vpp], ctypes.c_int),
             "odis_grid_generate": ([ctypes.c_int, ctypes.c_int, vpp], ctypes.c_int),
@@ -372,7 +374,8 @@ class Context:
 
     def set_sh(self, l_max, beta=None, mode=SH_FP64):
         """Self-gravity / shell coupling: beta is (n_inst, l_max + 1) or
-        (l_max + 1,) per-degree factors; mode SH_OFF / SH_FP64 / SH_SPLIT_BF16."""
+        (l_max + 1,) per-degree factors; mode SH_OFF / SH_FP64 / SH_SPLIT_BF16 /
+        SH_RECUR (basis evaluated on the fly)."""
         b = None
         if beta is not None:
             b = np.ascontiguousarray(np.broadcast_to(np.asarray(beta, np.float64),
@@ -422,9 +425,12 @@ class Context:
         self.set_option(OPT_ONE_KERNEL, 1 if on else 0)
 
     def timing(self):
-        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
-        self._check(lib().odis_get_timing(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
-        return dict(face_ms=a.value, cell_ms=b.value, step_ms=c.value)
+        """Average kernel times (ms) of the sampled steps of the last step()
+        call (odis_get_kernel_times)."""
+        t = np.zeros(6)
+        self._check(lib().odis_get_kernel_times(self._h, _dp(t), 6))
+        return dict(face_ms=t[0], cell_ms=t[1], step_ms=t[2], sh_analysis_ms=t[3], sh_coef_ms=t[4],
+                    sh_synth_ms=t[5])
 
     def close(self):
         if getattr(self, "_h", None):

@@ -464,26 +464,35 @@ __global__ __launch_bounds__(kBlock) void k_interp_ll(int n_rows, const int *rp,
 // ---------------------------------------------------------------------------
 // spherical-harmonic projection (self-gravity / ice-shell coupling)
 // ---------------------------------------------------------------------------
-// Basis Yt[k][i] (k = harmonic, i = device cell) stored basis-major so that
-// the analysis b_k = sum_i Yt[k][i] x_i streams row k (one workgroup per k)
-// and the synthesis y_i = sum_k c_k Yt[k][i] streams the rows with one lane
-// per cell.  The split-BF16 form packs hi = bf16(Y) in the upper and
-// lo = bf16(Y - hi) in the lower half of a 32-bit word.
+// Basis Yt[k][i] (k = harmonic, i = device cell) stored basis-major.  The
+// analysis b_k = sum_i Yt[k][i] x_i runs one workgroup per (harmonic, chunk
+// of kShChunks cell ranges) and writes partial sums, which k_sh_coef adds in
+// a fixed order; the synthesis y_i = sum_k c_k Yt[k][i] runs 64 cells per
+// workgroup, each of its 4 waves over a quarter of the harmonics, the
+// quarters added in a fixed order through LDS.  Every basis entry is read
+// once per pass, coalesced along the cells.  The split-BF16 form packs
+// hi = bf16(Y) in the upper and lo = bf16(Y - hi) in the lower half of a
+// 32-bit word.
+
+constexpr int kShChunks = 8;
 
 __device__ __forceinline__ float bf16_hi(unsigned int u) { return __uint_as_float(u & 0xFFFF0000u); }
 __device__ __forceinline__ float bf16_lo(unsigned int u) { return __uint_as_float(u << 16); }
 
+// grid (K * kShChunks, B): partial b of harmonic k over cell chunk q
 template <typename T, bool SPLIT>
 __global__ __launch_bounds__(kBlock) void k_sh_analysis(int K, int N, const double *Yt, const unsigned int *Yp,
-                                                        const T *x, long long x_stride, double *b_out) {
+                                                        const T *x, long long x_stride, double *b_part) {
   __shared__ double sh[4];
-  const int k = blockIdx.x, b = blockIdx.y;
+  const int k = blockIdx.x / kShChunks, q = blockIdx.x % kShChunks, b = blockIdx.y;
+  const int chunk = ((N + kShChunks - 1) / kShChunks + kBlock - 1) / kBlock * kBlock;
+  const int lo = q * chunk, hi = min(N, lo + chunk);
   const T *xb = x + b * x_stride;
   double acc = 0.0;
   if (SPLIT) {
     float facc = 0.f;
     const unsigned int *row = Yp + size_t(k) * N;
-    for (int i = threadIdx.x; i < N; i += kBlock) {
+    for (int i = lo + threadIdx.x; i < hi; i += kBlock) {
       const unsigned int u = row[i];
       const float xi = float(xb[i]);
       facc = fmaf(bf16_hi(u), xi, facc);
@@ -492,55 +501,654 @@ __global__ __launch_bounds__(kBlock) void k_sh_analysis(int K, int N, const doub
     acc = double(facc);
   } else {
     const double *row = Yt + size_t(k) * N;
-    for (int i = threadIdx.x; i < N; i += kBlock) acc += row[i] * double(xb[i]);
+    for (int i = lo + threadIdx.x; i < hi; i += kBlock) acc += row[i] * double(xb[i]);
   }
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) sh[wave] = acc;
-  __syncthreads();
-  if (threadIdx.x == 0) b_out[size_t(b) * K + k] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  const double t = block_sum<kBlock / 64>(acc, sh);
+  if (threadIdx.x == 0) b_part[(size_t(b) * kShChunks + q) * K + k] = t;
 }
 
-// c_k = D_k sum_j Minv[k][j] b_j : one wavefront per k
-__global__ __launch_bounds__(kBlock) void k_sh_coef(int K, const double *Minv, const double *D, const double *bvec,
-                                                    double *c) {
+// b_j = sum_q b_part[q][j] over the n_part analysis partials, in a fixed
+// order; written after the partials of every instance ([B][n_part][K]).
+// 64 j per workgroup; wave w adds the partials q = w mod 4 with four loads
+// in flight, then the four wave sums are added.
+__global__ __launch_bounds__(kBlock) void k_sh_bsum(int K, int n_part, double *b_part) {
+  __shared__ double part[kBlock / 64][64];
+  const int b = blockIdx.y, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int j = blockIdx.x * 64 + lane;
+  double t = 0.0;
+  if (j < K) {
+    const double *p = b_part + size_t(b) * n_part * K + j;
+    int q = w;
+    for (; q + 12 < n_part; q += 16) {
+      const double a0 = p[size_t(q) * K], a1 = p[size_t(q + 4) * K], a2 = p[size_t(q + 8) * K],
+                   a3 = p[size_t(q + 12) * K];
+      t += a0;
+      t += a1;
+      t += a2;
+      t += a3;
+    }
+    for (; q < n_part; q += 4) t += p[size_t(q) * K];
+  }
+  part[w][lane] = t;
+  __syncthreads();
+  if (w == 0 && j < K)
+    b_part[(size_t(gridDim.y) * n_part + b) * K + j] = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+}
+
+// Store c_k (k = l l + l + m) at its m-major place: for m = 0..L, l = m..L
+// the pair (c_{l,|m|}, c_{l,-|m|}) at 2 (m (L + 1) - m (m - 1) / 2 + l - m);
+// the unused sine slot of m = 0 is zero.
+// chain (nullable): the pair stores S_l c instead, S_l the chain scale.
+__device__ __forceinline__ void put_mmajor(int K, int b, int k, double ck, const double *chain, double *cmaj) {
+  int l = 0;
+  while ((l + 1) * (l + 1) <= k) ++l;
+  const int mm = k - l * l - l, m = mm < 0 ? -mm : mm;
+  int L = 0;
+  while ((L + 1) * (L + 1) < K) ++L;
+  double *cb = cmaj + size_t(b) * (L + 1) * (L + 2);
+  const int at = m * (L + 1) - m * (m - 1) / 2 + l - m;
+  if (m == 0) cb[2 * l + 1] = 0.0;
+  cb[2 * at + (mm < 0 ? 1 : 0)] = chain ? chain[2 * at + 1] * ck : ck;
+}
+
+// c_k = D_k sum_j Minv[k][j] b_j: b in LDS, one wavefront per k.  cmaj
+// (nullable) also receives c in m-major order: for m = 0..L, l = m..L the
+// pair (c_{l,m}, c_{l,-m}) at 2 (m (L + 1) - m (m - 1) / 2 + l - m).
+__global__ __launch_bounds__(kBlock) void k_sh_coef(int K, const double *Minv, const double *D, const double *bsum,
+                                                    double *c, const double *chain, double *cmaj) {
+  extern __shared__ double bs[];
+  const int b = blockIdx.y;
+  for (int j = threadIdx.x; j < K; j += kBlock) bs[j] = bsum[size_t(b) * K + j];
+  __syncthreads();
   const int lane = threadIdx.x & 63;
   const int k = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-  const int b = blockIdx.y;
   if (k >= K) return;
-  const double *bb = bvec + size_t(b) * K;
   const double *row = Minv + size_t(k) * K;
   double acc = 0.0;
-  for (int j = lane; j < K; j += 64) acc += row[j] * bb[j];
+  for (int j = lane; j < K; j += 64) acc += row[j] * bs[j];
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-  if (lane == 0) c[size_t(b) * K + k] = (D ? D[size_t(b) * K + k] : 1.0) * acc;
+  if (lane != 0) return;
+  const double ck = (D ? D[size_t(b) * K + k] : 1.0) * acc;
+  c[size_t(b) * K + k] = ck;
+  if (cmaj) put_mmajor(K, b, k, ck, chain, cmaj);
 }
 
-// y_i = base_i + sum_k c_k Yt[k][i]
+// c (index k = l l + l + m) -> its m-major copy, one lane per k
+__global__ __launch_bounds__(kBlock) void k_sh_to_mmajor(int K, const double *c, const double *chain,
+                                                         double *cmaj) {
+  const int b = blockIdx.y;
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k < K) put_mmajor(K, b, k, c[size_t(b) * K + k], chain, cmaj);
+}
+
+// y_i = base_i + sum_k c_k Yt[k][i]: 64 cells per workgroup, wave w sums the
+// harmonics [w K/4, (w+1) K/4)
 template <typename T, bool SPLIT>
 __global__ __launch_bounds__(kBlock) void k_sh_synth(int K, int N, const double *Yt, const unsigned int *Yp,
                                                      const double *c, const T *base, long long stride, T *out) {
-  extern __shared__ double cs[];
+  extern __shared__ double cs[];  // [K] coefficients, then [4][64] wave partials
+  double *part = cs + K;
   const int b = blockIdx.y;
   for (int k = threadIdx.x; k < K; k += kBlock) cs[k] = c[size_t(b) * K + k];
   __syncthreads();
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= N) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + lane;
+  const int per = (K + 3) / 4;
+  const int k0 = wave * per, k1 = min(K, k0 + per);
   double acc = 0.0;
-  if (SPLIT) {
-    float facc = 0.f;
-    for (int k = 0; k < K; ++k) {
-      const unsigned int u = Yp[size_t(k) * N + i];
-      const float ck = float(cs[k]);
-      facc = fmaf(bf16_hi(u), ck, facc);
-      facc = fmaf(bf16_lo(u), ck, facc);
+  if (i < N) {
+    if (SPLIT) {
+      float facc = 0.f;
+      for (int k = k0; k < k1; ++k) {
+        const unsigned int u = Yp[size_t(k) * N + i];
+        const float ck = float(cs[k]);
+        facc = fmaf(bf16_hi(u), ck, facc);
+        facc = fmaf(bf16_lo(u), ck, facc);
+      }
+      acc = double(facc);
+    } else {
+      for (int k = k0; k < k1; ++k) acc += cs[k] * Yt[size_t(k) * N + i];
     }
-    acc = double(facc);
-  } else {
-    for (int k = 0; k < K; ++k) acc += cs[k] * Yt[size_t(k) * N + i];
   }
+  part[wave * 64 + lane] = acc;
+  __syncthreads();
+  if (wave != 0 || i >= N) return;
+  const double sum = (part[lane] + part[64 + lane]) + (part[128 + lane] + part[192 + lane]);
   const T add = base ? base[b * stride + i] : T(0);
-  out[b * stride + i] = T(double(add) + acc);
+  out[b * stride + i] = T(double(add) + sum);
+}
+
+// ---------------------------------------------------------------------------
+// the same projection with the basis evaluated on the fly
+// ---------------------------------------------------------------------------
+// Y_lm(i) = P_lm(sin lat_i) cos(m lon_i) (m >= 0) or P_l|m| sin(|m| lon_i),
+// P_lm by the recurrence whose coefficients the host tabulates
+// (sh_recurrence_tables: a, b per (l, m), sect and f1 per m; the host basis
+// used for the Gram matrix comes from the same table), cos / sin(m lon) by
+// rotation from cos / sin(lon).  No basis is read from memory: per pass the
+// kernels read 4 values per cell and the coefficient table.
+//   analysis: wave (cell chunk, m pair) -- m and L - m, so every wave has
+//     L + 2 degrees -- kShCpt cells per lane, one wave sum per (l, +-m),
+//     partials per chunk added in a fixed order by k_sh_coef;
+//   synthesis: one lane per cell over all (l, m).
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Analysis: workgroup = kShWaves waves on kShWaves consecutive cell chunks
+// (64 CPT cells each) and one m pair (m, L - m), so every wave has L + 2
+// degrees.  A wave runs the degrees of each m in blocks of 8: per lane the
+// 16 (l, cos / sin) sums over its CPT cells, then a butterfly reduce-scatter
+// over the 64 lanes (xor 1, 2, 4, 8 halving the slots, then xor 16, 32)
+// leaves each of the 16 sums in one lane -- 17 shuffles for 16 sums.  The
+// workgroup adds its waves' sums through LDS in a fixed order and writes one
+// partial per (l, +-m).  Products are fused multiply-adds; the recurrence
+// table is read m-major (sect, f1, then per m the (a, b) pairs of
+// l = m+2..L) with wave-uniform indices.
+constexpr int kShWaves = 8;
+
+// 16 per-lane values -> the sum over the wave of value q in lane
+// (q & 1) << 3 | (q & 2) << 1 | (q & 4) >> 1 | (q & 8) >> 3 (and its +16, +32, +48)
+template <typename T>
+__device__ __forceinline__ T reduce_scatter16(T *v, int lane) {
+#pragma unroll
+  for (int st = 0; st < 4; ++st) {
+    const int d = 1 << st, half = 8 >> st;
+    const bool hi = (lane & d) != 0;
+#pragma unroll
+    for (int j = 0; j < half; ++j) {
+      const T keep = hi ? v[half + j] : v[j];
+      const T send = hi ? v[j] : v[half + j];
+      v[j] = keep + __shfl_xor(send, d, 64);
+    }
+  }
+  T r = v[0];
+  r += __shfl_xor(r, 16, 64);
+  r += __shfl_xor(r, 32, 64);
+  return r;
+}
+
+template <typename T, int LT, int CPT>
+__global__ __launch_bounds__(64 * kShWaves) void k_sh_analysis_rec(int Lrt, int N, const double *__restrict__ tabm,
+                                                                   const double *__restrict__ trig,
+                                                                   const T *__restrict__ x, long long x_stride,
+                                                                   int n_chunks, double *b_part) {
+  extern __shared__ double red[];  // [kShWaves][2 (L + 2)]
+  const int L = LT > 0 ? LT : Lrt;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int n_pairs = L / 2 + 1, n_slot = 2 * (L + 2);
+  const int cblk = blockIdx.x / n_pairs, pair = blockIdx.x % n_pairs;
+  const int n_cblk = (n_chunks + kShWaves - 1) / kShWaves;
+  const int b = blockIdx.y;
+  const int chunk = cblk * kShWaves + w;
+  const int K = (L + 1) * (L + 1);
+  const double *tsect = tabm, *tf1 = tabm + (L + 1);
+  const T *xb = x + b * x_stride;
+  double *rw = red + w * n_slot;
+  const int m0 = pair, m1 = L - pair;
+  // lane of sum q after reduce_scatter16 is rev4(q); this lane holds q = rev4(lane & 15)
+  const int q_lane = ((lane & 1) << 3) | ((lane & 2) << 1) | ((lane & 4) >> 1) | ((lane & 8) >> 3);
+  if (chunk < n_chunks) {
+    for (int h = 0; h < 2; ++h) {
+      const int m = h == 0 ? m0 : m1;
+      if (h == 1 && m1 == m0) break;
+      double *slot = rw + (h == 0 ? 0 : 2 * (L - m0 + 1));
+      int oab = 2 * (L + 1);
+      for (int q = 0; q < m; ++q) oab += 2 * max(0, L - q - 1);
+      const double *ab = tabm + oab;  // (a, b) of l = m + 2 + j at 2 j
+      T mu[CPT], xc[CPT], xs[CPT], p1[CPT], p2[CPT];
+#pragma unroll
+      for (int j = 0; j < CPT; ++j) {
+        const int i = (chunk * CPT + j) * 64 + lane;
+        const bool ok = i < N;
+        mu[j] = ok ? T(trig[i]) : T(0);
+        const T s = ok ? T(trig[N + i]) : T(0), cl = ok ? T(trig[2 * N + i]) : T(1),
+                sl = ok ? T(trig[3 * N + i]) : T(0);
+        const T xv = ok ? xb[i] : T(0);
+        T cm = T(1), sm = T(0), pmm = T(1);
+        for (int q = 1; q <= m; ++q) {
+          const T c2 = fma(cm, cl, -(sm * sl));
+          sm = fma(sm, cl, cm * sl);
+          cm = c2;
+          pmm = pmm * (T(tsect[q]) * s);
+        }
+        xc[j] = xv * cm;
+        xs[j] = xv * sm;
+        p1[j] = pmm;  // P_{l-1} for l = m + 1 after the first degree
+        p2[j] = T(0);
+      }
+      const int nl = L - m + 1;  // degrees m .. L, index t = l - m
+      const T f = T(tf1[m]);
+      for (int t0 = 0; t0 < nl; t0 += 8) {
+        T v[16];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int t = t0 + u;  // wave-uniform
+          T ac = T(0), as = T(0);
+          if (t < nl) {
+            if (t == 0) {
+#pragma unroll
+              for (int j = 0; j < CPT; ++j) {
+                ac = fma(p1[j], xc[j], ac);
+                as = fma(p1[j], xs[j], as);
+              }
+            } else if (t == 1) {
+#pragma unroll
+              for (int j = 0; j < CPT; ++j) {
+                const T p = f * mu[j] * p1[j];
+                p2[j] = p1[j];
+                p1[j] = p;
+                ac = fma(p, xc[j], ac);
+                as = fma(p, xs[j], as);
+              }
+            } else {
+              const T a = T(ab[2 * (t - 2)]), bb = T(ab[2 * (t - 2) + 1]);
+#pragma unroll
+              for (int j = 0; j < CPT; ++j) {
+                const T p = fma(a * mu[j], p1[j], -(bb * p2[j]));
+                p2[j] = p1[j];
+                p1[j] = p;
+                ac = fma(p, xc[j], ac);
+                as = fma(p, xs[j], as);
+              }
+            }
+          }
+          v[2 * u] = ac;
+          v[2 * u + 1] = as;
+        }
+        const T r = reduce_scatter16(v, lane);
+        // lane with (lane & 15) = rev4(q) writes sum q = 2 u + s: degree t0 + u
+        if (lane < 16 && t0 + (q_lane >> 1) < nl) slot[2 * t0 + q_lane] = double(r);
+      }
+    }
+  }
+  __syncthreads();
+  // fixed-order sum over the workgroup's waves, one lane per slot
+  const int t = threadIdx.x;
+  if (t >= n_slot) return;
+  const int base1 = 2 * (L - m0 + 1);
+  int m, l;
+  if (t < base1) {
+    m = m0;
+    l = m0 + t / 2;
+  } else {
+    if (m1 == m0) return;
+    m = m1;
+    l = m1 + (t - base1) / 2;
+    if (l > L) return;
+  }
+  const bool sn = (t & 1) != 0;
+  if (sn && m == 0) return;
+  const int n_w = min(kShWaves, n_chunks - cblk * kShWaves);
+  double sum = 0.0;
+  for (int q = 0; q < n_w; ++q) sum += red[q * n_slot + t];
+  b_part[(size_t(b) * n_cblk + cblk) * K + l * l + l + (sn ? -m : m)] = sum;
+}
+
+// Analysis by m pairs, for a degree L fixed at compile time.  A workgroup of
+// kPairWaves waves owns one pair (m0, m1) = (p, L - p) and a block of
+// 64 kPairWaves R cells; each lane runs the two Legendre chains of its pair
+// for its R cells and keeps all 2 (L + 2) sums (cos, sin per degree) in
+// registers, so the cross-lane reduction happens once per block instead of
+// once per degree.  The chains use the rescaled recurrence
+//     q_m = P_mm,  q_l = mu q_{l-1} + beta_l q_{l-2},  P_lm = S_l q_l,
+// (beta_l = -b_l / (a_l a_{l-1}), S_l = a_{m+1} ... a_l, a_{m+1} = f1_m), two
+// operations per degree; S is applied to the block sums.  cos(m lon),
+// sin(m lon) and cos^m(lat) come from binary powers of (cos lon, sin lon)
+// and cos lat, and P_mm = C_m cos^m(lat) with C_m = sect_1 ... sect_m.
+// chain (m-major): for m = 0..L and k = 0..L-m the pair (beta, S) of degree
+// l = m + k at 2 (m (L + 1) - m (m - 1) / 2 + k) (k = 0: beta = 1, S = 1;
+// k = 1: beta = 0), then C_0..C_L.
+constexpr int kPairWaves = 4;
+
+template <int LT>
+struct PairShape {
+  static constexpr int NT = LT + 2;            // degrees of a pair
+  static constexpr int NV = 2 * NT;            // sums per lane
+  static constexpr int CH = (NV + 3) / 4;      // sums per reduction pass
+  static constexpr int NPASS = (NV + CH - 1) / CH;
+};
+
+template <typename T>
+__device__ __forceinline__ void cmul(T &c, T &s, T zc, T zs) {
+  const T c2 = fma(c, zc, -(s * zs));
+  s = fma(s, zc, c * zs);
+  c = c2;
+}
+
+// z <- z w (complex), r <- r u
+template <typename T>
+__device__ __forceinline__ void pow_step(T &c, T &s, T &r, T zc, T zs, T zr, bool first) {
+  if (first) {
+    c = zc;
+    s = zs;
+    r = zr;
+  } else {
+    cmul(c, s, zc, zs);
+    r *= zr;
+  }
+}
+
+// (cos m lon, sin m lon, cos^m lat) for M0 and M1 (M1 >= M0) by binary powers
+template <typename T, int M0, int M1>
+__device__ __forceinline__ void pair_powers(T zc, T zs, T zr, T &ac, T &as, T &ar, T &bc, T &bs, T &br) {
+  ac = T(1), as = T(0), ar = T(1), bc = T(1), bs = T(0), br = T(1);
+  bool fa = true, fb = true;
+#pragma unroll
+  for (int k = 0; (1 << k) <= M1; ++k) {
+    if ((M0 >> k) & 1) {
+      pow_step(ac, as, ar, zc, zs, zr, fa);
+      fa = false;
+    }
+    if ((M1 >> k) & 1) {
+      pow_step(bc, bs, br, zc, zs, zr, fb);
+      fb = false;
+    }
+    if ((2 << k) <= M1) {
+      const T c2 = fma(zc, zc, -(zs * zs));
+      zs = T(2) * (zc * zs);
+      zc = c2;
+      zr *= zr;
+    }
+  }
+}
+
+// One chain of NK degrees starting at table entry OFF, sums into acc[2 T0 ..]
+template <typename T, int T0, int NK, int OFF, int NV>
+__device__ __forceinline__ void run_chain(const double *__restrict__ chain, T mu, T pmm, T xc, T xs, T (&acc)[NV]) {
+  T q1 = T(0), q2 = pmm;
+#pragma unroll
+  for (int k = 0; k < NK; ++k) {
+    T q;
+    if (k == 0)
+      q = pmm;
+    else if (k == 1)
+      q = mu * q1;
+    else
+      q = fma(mu, q1, T(chain[2 * (OFF + k)]) * q2);
+    q2 = q1;
+    q1 = q;
+    acc[2 * (T0 + k)] = fma(q, xc, acc[2 * (T0 + k)]);
+    acc[2 * (T0 + k) + 1] = fma(q, xs, acc[2 * (T0 + k) + 1]);
+  }
+}
+
+// R cells per lane for the pair (P, LT - P), all shapes fixed at compile time
+template <typename T, int LT, int P>
+__device__ __forceinline__ void pair_cells(int N, const double *__restrict__ chain, const double *__restrict__ trig,
+                                           const T *__restrict__ xb, int i0, int span, int R,
+                                           T (&acc)[PairShape<LT>::NV]) {
+  constexpr int L = LT, M0 = P, M1 = L - P, NA = L - M0 + 1, NB = M1 != M0 ? M0 + 1 : 0;
+  constexpr int OFFA = M0 * (L + 1) - M0 * (M0 - 1) / 2, OFFB = M1 * (L + 1) - M1 * (M1 - 1) / 2;
+  const double *cm_tab = chain + (L + 1) * (L + 2);
+  const T CA = T(cm_tab[M0]), CB = T(cm_tab[M1]);
+  // operands of the next cell are loaded while this one runs; cells past N
+  // read cell N - 1 with x = 0
+  auto load = [&](int r, double &mu, double &s, double &zc, double &zs, T &xv) {
+    const int i = i0 + r * span, ii = min(i, N - 1);
+    mu = trig[ii];
+    s = trig[N + ii];
+    zc = trig[2 * N + ii];
+    zs = trig[3 * N + ii];
+    xv = xb[ii];
+    if (i >= N) xv = T(0);
+  };
+  double nmu, ns, nzc, nzs;
+  T nxv;
+  load(0, nmu, ns, nzc, nzs, nxv);
+  for (int r = 0; r < R; ++r) {
+    const T mu = T(nmu), s = T(ns), zc = T(nzc), zs = T(nzs), xv = nxv;
+    if (r + 1 < R) load(r + 1, nmu, ns, nzc, nzs, nxv);
+    T ac, as, ar, bc, bs, br;
+    pair_powers<T, M0, M1>(zc, zs, s, ac, as, ar, bc, bs, br);
+    run_chain<T, 0, NA, OFFA>(chain, mu, CA * ar, xv * ac, xv * as, acc);
+    if constexpr (NB > 0) run_chain<T, NA, NB, OFFB>(chain, mu, CB * br, xv * bc, xv * bs, acc);
+  }
+}
+
+template <typename T, int LT, int P>
+__device__ __forceinline__ void pair_dispatch(int pair, int N, const double *__restrict__ chain,
+                                              const double *__restrict__ trig, const T *__restrict__ xb, int i0,
+                                              int span, int R, T (&acc)[PairShape<LT>::NV]) {
+  if (pair == P) {
+    pair_cells<T, LT, P>(N, chain, trig, xb, i0, span, R, acc);
+  } else if constexpr (P < LT / 2) {
+    pair_dispatch<T, LT, P + 1>(pair, N, chain, trig, xb, i0, span, R, acc);
+  }
+}
+
+template <typename T, int LT>
+__global__ __launch_bounds__(64 * kPairWaves) void k_sh_analysis_pair(int N, const double *__restrict__ chain,
+                                                                      const double *__restrict__ trig,
+                                                                      const T *__restrict__ x, long long x_stride,
+                                                                      int R, int n_cblk, int n_logical,
+                                                                      double *b_part) {
+  using S = PairShape<LT>;
+  constexpr int L = LT, K = (L + 1) * (L + 1), n_pairs = L / 2 + 1;
+  constexpr int NTASK = kPairWaves * S::CH * 4;  // (wave, sum, lane group of 16)
+  __shared__ double red[kPairWaves * 64 * (S::CH + 1)];
+  __shared__ double red2[NTASK];
+  __shared__ double tot[S::NV];
+  // blocks that share cells run on one XCD (dispatch round-robins over 8)
+  const int g = blockIdx.x, gpad = gridDim.x;
+  const int logical = (g & 7) * (gpad >> 3) + (g >> 3);
+  if (logical >= n_logical) return;  // whole workgroup
+  const int cblk = logical / n_pairs, pair = logical % n_pairs;
+  const int b = blockIdx.y;
+  const int m0 = pair, m1 = L - pair;
+  const bool two = m1 != m0;
+  const int nA = L - m0 + 1, n_tot = nA + (two ? m0 + 1 : 0);
+  const int offA = m0 * (L + 1) - m0 * (m0 - 1) / 2, offB = m1 * (L + 1) - m1 * (m1 - 1) / 2;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  T acc[S::NV];
+#pragma unroll
+  for (int v = 0; v < S::NV; ++v) acc[v] = T(0);
+  const int span = 64 * kPairWaves;
+  pair_dispatch<T, LT, 0>(__builtin_amdgcn_readfirstlane(pair), N, chain, trig, x + b * x_stride,
+                          cblk * span * R + w * 64 + lane, span, R, acc);
+  // block sums: per pass, CH sums of every lane through LDS; tasks of 16
+  // lanes, then a fixed-order sum over waves and lane groups
+#pragma unroll
+  for (int ps = 0; ps < S::NPASS; ++ps) {
+    double *rw = red + (w * 64 + lane) * (S::CH + 1);
+#pragma unroll
+    for (int v = 0; v < S::CH; ++v)
+      if (ps * S::CH + v < S::NV) rw[v] = double(acc[ps * S::CH + v]);
+    __syncthreads();
+    for (int task = threadIdx.x; task < NTASK; task += 64 * kPairWaves) {
+      const int tw = task / (S::CH * 4), rem = task % (S::CH * 4), v = rem >> 2, lg = rem & 3;
+      const double *src = red + (tw * 64 + lg * 16) * (S::CH + 1) + v;
+      double u = 0.0;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) u += src[q * (S::CH + 1)];
+      red2[task] = u;
+    }
+    __syncthreads();
+    if (threadIdx.x < S::CH && ps * S::CH + threadIdx.x < S::NV) {
+      double u = 0.0;
+      for (int tw = 0; tw < kPairWaves; ++tw)
+        for (int lg = 0; lg < 4; ++lg) u += red2[tw * S::CH * 4 + threadIdx.x * 4 + lg];
+      tot[ps * S::CH + threadIdx.x] = u;
+    }
+    __syncthreads();
+  }
+  const int v = threadIdx.x;
+  if (v >= 2 * n_tot) return;
+  const int t = v >> 1;
+  const bool sn = (v & 1) != 0;
+  const int m = t < nA ? m0 : m1, kk = t < nA ? t : t - nA, l = m + kk;
+  if (sn && m == 0) return;
+  const double scale = chain[2 * ((t < nA ? offA : offB) + kk) + 1];
+  b_part[(size_t(b) * n_cblk + cblk) * K + l * l + l + (sn ? -m : m)] = scale * tot[v];
+}
+
+// Synthesis: workgroup = 64 cells x 4 waves; wave g sums the contiguous
+// degrees [mr[g], mr[g+1]) (split on the host into near-equal (l, m) counts)
+// for the 64 cells, so every table and coefficient index is wave-uniform
+// and contiguous in l (scalar loads in batches): the coefficients in the
+// m-major copy of k_sh_coef, the recurrence table m-major
+// (tabm: sect[L+1], f1[L+1], then per m the pairs (a, b) for l = m+2..L).
+// cos / sin(m lon) by rotation, P_mm by the sectoral product.  The four wave
+// sums are added through LDS in a fixed order.
+template <typename T, bool SIN>
+__device__ __forceinline__ T synth_degree(int L, int m, const double *__restrict__ ab, const double *__restrict__ cc,
+                                          T f1, T mu, T pmm, T cm, T sm, T acc) {
+  T t = T(cc[0]) * cm;
+  if (SIN) t = fma(T(cc[1]), sm, t);
+  acc = fma(t, pmm, acc);
+  if (m + 1 > L) return acc;
+  T p2 = pmm, p1 = f1 * mu * pmm;
+  T u = T(cc[2]) * cm;
+  if (SIN) u = fma(T(cc[3]), sm, u);
+  acc = fma(u, p1, acc);
+  const int n = L - m - 1;  // degrees m + 2 .. L
+#pragma unroll 4
+  for (int j = 0; j < n; ++j) {
+    const T p = fma(T(ab[2 * j]) * mu, p1, -(T(ab[2 * j + 1]) * p2));
+    p2 = p1;
+    p1 = p;
+    T v = T(cc[2 * j + 4]) * cm;
+    if (SIN) v = fma(T(cc[2 * j + 5]), sm, v);
+    acc = fma(v, p, acc);
+  }
+  return acc;
+}
+
+template <typename T, int LT>
+__global__ __launch_bounds__(kBlock) void k_sh_synth_rec(int Lrt, int N, const double *__restrict__ tabm,
+                                                         const double *__restrict__ trig,
+                                                         const double *__restrict__ cmaj, const T *base,
+                                                         long long stride, T *out, int4 mr4) {
+  __shared__ double part[kBlock];
+  const int L = LT > 0 ? LT : Lrt;
+  const int b = blockIdx.y;
+  const double *tsect = tabm, *tf1 = tabm + (L + 1);
+  const double *cb = cmaj + size_t(b) * (L + 1) * (L + 2);
+  const int lane = threadIdx.x & 63;
+  const int g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i = blockIdx.x * 64 + lane;
+  const bool ok = i < N;
+  const int mlo = g == 0 ? 0 : (g == 1 ? mr4.x : (g == 2 ? mr4.y : mr4.z));
+  const int mhi = g == 0 ? mr4.x : (g == 1 ? mr4.y : (g == 2 ? mr4.z : mr4.w));
+  const int ii = ok ? i : 0;
+  const T mu = T(trig[ii]), s = T(trig[N + ii]), cl = T(trig[2 * N + ii]), sl = T(trig[3 * N + ii]);
+  T acc = T(0), cm = T(1), sm = T(0), pmm = T(1);
+  int oab = 0;  // offset of degree m's (a, b) pairs after sect and f1
+  for (int q = 1; q < mlo; ++q) {  // advance to m = mlo - 1 ...
+    const T c2 = fma(cm, cl, -(sm * sl));
+    sm = fma(sm, cl, cm * sl);
+    cm = c2;
+    pmm = pmm * (T(tsect[q]) * s);
+  }
+  for (int q = 0; q < mlo; ++q) oab += 2 * max(0, L - q - 1);
+#pragma unroll 1
+  for (int m = mlo; m < mhi; ++m) {
+    const double *ab = tabm + 2 * (L + 1) + oab;
+    const double *cc = cb + 2 * (m * (L + 1) - m * (m - 1) / 2);
+    if (m == 0) {
+      acc = synth_degree<T, false>(L, m, ab, cc, T(tf1[m]), mu, pmm, cm, sm, acc);
+    } else {  // ... and one more per degree
+      const T c2 = fma(cm, cl, -(sm * sl));
+      sm = fma(sm, cl, cm * sl);
+      cm = c2;
+      pmm = pmm * (T(tsect[m]) * s);
+      acc = synth_degree<T, true>(L, m, ab, cc, T(tf1[m]), mu, pmm, cm, sm, acc);
+    }
+    oab += 2 * max(0, L - m - 1);
+  }
+  part[threadIdx.x] = double(acc);
+  __syncthreads();
+  if (threadIdx.x >= 64 || !ok) return;
+  const double sum = (part[lane] + part[64 + lane]) + (part[128 + lane] + part[192 + lane]);
+  const T add = base ? base[b * stride + i] : T(0);
+  out[b * stride + i] = add + T(sum);
+}
+
+// Synthesis with the rescaled recurrence, for a degree L fixed at compile
+// time: 64 cells x 4 waves per workgroup, wave g summing the contiguous
+// m-range of synth_split (cos / sin (m lon) by rotation, P_mm = P_{m-1,m-1}
+// sect_m cos(lat)).  Each m sums its degrees against the scaled
+// coefficients S_l c_lm (cmaj written with the chain scales), four
+// operations per degree:
+//     q_l = mu q_{l-1} + beta_l q_{l-2},  A += (S c)_{l,m} q_l,  B += (S c)_{l,-m} q_l,
+// and y = sum_m A_m cos(m lon) + B_m sin(m lon); the four waves' sums are
+// added through LDS in a fixed order.
+template <typename T, int LT>
+__global__ __launch_bounds__(kBlock) void k_sh_synth_chain(int N, const double *__restrict__ chain,
+                                                           const double *__restrict__ tabm,
+                                                           const double *__restrict__ trig,
+                                                           const double *__restrict__ cmaj, const T *base,
+                                                           long long stride, T *out, int4 mr4) {
+  __shared__ double part[kBlock];
+  constexpr int L = LT;
+  const int b = blockIdx.y;
+  const double *cb = cmaj + size_t(b) * (L + 1) * (L + 2);
+  const int lane = threadIdx.x & 63;
+  const int g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i = blockIdx.x * 64 + lane;
+  const bool ok = i < N;
+  const int mlo = g == 0 ? 0 : (g == 1 ? mr4.x : (g == 2 ? mr4.y : mr4.z));
+  const int mhi = g == 0 ? mr4.x : (g == 1 ? mr4.y : (g == 2 ? mr4.z : mr4.w));
+  const int ii = ok ? i : 0;
+  const T mu = T(trig[ii]), s = T(trig[N + ii]), cl = T(trig[2 * N + ii]), sl = T(trig[3 * N + ii]);
+  T y = T(0), cm = T(1), sm = T(0), pmm = T(1);
+  for (int q = 1; q < mlo; ++q) {  // advance to m = mlo - 1
+    const T c2 = fma(cm, cl, -(sm * sl));
+    sm = fma(sm, cl, cm * sl);
+    cm = c2;
+    pmm = pmm * (T(tabm[q]) * s);
+  }
+#pragma unroll 1
+  for (int m = mlo; m < mhi; ++m) {
+    if (m > 0) {
+      const T c2 = fma(cm, cl, -(sm * sl));
+      sm = fma(sm, cl, cm * sl);
+      cm = c2;
+      pmm = pmm * (T(tabm[m]) * s);
+    }
+    const int off = m * (L + 1) - m * (m - 1) / 2, nk = L - m + 1;
+    const double *ch = chain + 2 * off, *cc = cb + 2 * off;
+    // degrees m and m + 1, then the recurrence (the sine slots of m = 0 are zero)
+    T A = T(cc[0]) * pmm, B = T(cc[1]) * pmm;
+    T q2 = pmm, q1 = pmm;
+    if (nk > 1) {
+      q1 = mu * pmm;
+      A = fma(T(cc[2]), q1, A);
+      B = fma(T(cc[3]), q1, B);
+    }
+#pragma unroll 4
+    for (int k = 2; k < nk; ++k) {
+      const T q = fma(mu, q1, T(ch[2 * k]) * q2);
+      q2 = q1;
+      q1 = q;
+      A = fma(T(cc[2 * k]), q, A);
+      B = fma(T(cc[2 * k + 1]), q, B);
+    }
+    y = fma(A, cm, y);
+    y = fma(B, sm, y);
+  }
+  part[threadIdx.x] = double(y);
+  __syncthreads();
+  if (threadIdx.x >= 64 || !ok) return;
+  const double sum = (part[lane] + part[64 + lane]) + (part[128 + lane] + part[192 + lane]);
+  const T add = base ? base[b * stride + i] : T(0);
+  out[b * stride + i] = add + T(sum);
+}
+
+// Degree ranges of the 4 synthesis waves of a workgroup: near-equal numbers of
+// (l, m) pairs (sum of L - m + 1) per lane.
+static int4 synth_split(int L) {
+  const int total = (L + 1) * (L + 2) / 2;
+  int cut[3], g = 0, acc = 0;
+  for (int m = 0; m <= L && g < 3; ++m) {
+    acc += L - m + 1;
+    while (g < 3 && acc * 4 >= total * (g + 1)) cut[g++] = m + 1;
+  }
+  while (g < 3) cut[g++] = L + 1;
+  return make_int4(cut[0], std::max(cut[0], cut[1]), std::max(cut[1], cut[2]), L + 1);
 }
 
 // Gram matrix M = Yt Yt^T (K x K), 32 x 32 output tiles, FP64.
@@ -690,10 +1298,16 @@ struct odis_ctx {
   int timing = 0;
   std::vector<hipEvent_t> ev;
   double t_face = 0, t_cell = 0, t_step = 0;
+  double t_sh[3] = {0, 0, 0};  // SH analysis, coefficients, synthesis
   // spherical-harmonic coupling
   int sh_mode = ODIS_SH_OFF, sh_L = 0, sh_K = 0;
   double *sh_Yt = nullptr, *sh_Minv = nullptr, *sh_D = nullptr, *sh_b = nullptr, *sh_c = nullptr;
   unsigned int *sh_Yp = nullptr;
+  double *sh_tab = nullptr, *sh_trig = nullptr;  // recurrence table, [4][N] sin/cos lat, cos/sin lon
+  double *sh_tabm = nullptr, *sh_cmaj = nullptr; // m-major table, m-major coefficients [B][(L+1)(L+2)]
+  double *sh_chain = nullptr;                    // rescaled recurrence (beta, S) per (l, m), then C_m
+  int sh_part = 0;                               // capacity of analysis partials per instance
+  int sh_cpt = 0;                                // cells per lane of the on-the-fly analysis (0: auto)
   void *eta_eff = nullptr;  // [B][N] of the stencil precision
   std::vector<void *> sh_allocs;
   std::vector<double> cell_lat, cell_lon;  // reference order
@@ -1415,35 +2029,139 @@ static int stage_coefficients(odis_ctx *ctx, long long n0, int count, const doub
   return ODIS_OK;
 }
 
-// eta_eff = eta + Yt^T D Minv Yt eta for every instance (three kernels)
+// On-the-fly analysis with CPT cells per lane: the degree-32 kernel with its
+// loops unrolled at compile time, or the runtime-degree one.
+template <typename T, int CPT>
+static void launch_analysis_rec(odis_ctx *ctx, int nb, int L, const T *x, long long stride, int n_chunks,
+                                hipEvent_t e0, hipEvent_t e1) {
+  const int n_cblk = (n_chunks + kShWaves - 1) / kShWaves;
+  const dim3 ga(n_cblk * (L / 2 + 1), nb), blk(64 * kShWaves);
+  const size_t lds = sizeof(double) * kShWaves * 2 * (L + 2);
+  if (L == 32)
+    hipExtLaunchKernelGGL((k_sh_analysis_rec<T, 32, CPT>), ga, blk, lds, ctx->stream, e0, e1, 0, L, ctx->N,
+                          ctx->sh_tabm, ctx->sh_trig, x, stride, n_chunks, ctx->sh_b);
+  else
+    hipExtLaunchKernelGGL((k_sh_analysis_rec<T, 0, CPT>), ga, blk, lds, ctx->stream, e0, e1, 0, L, ctx->N,
+                          ctx->sh_tabm, ctx->sh_trig, x, stride, n_chunks, ctx->sh_b);
+}
+
+// Degrees with a compiled pair analysis (the others use the per-degree one).
+static bool pair_degree(int L) { return L == 8 || L == 16 || L == 32; }
+
+template <typename T, int LT>
+static void launch_analysis_pair(odis_ctx *ctx, int nb, const T *x, long long stride, int cpt, int n_cblk,
+                                 hipEvent_t e0, hipEvent_t e1) {
+  const int n_logical = n_cblk * (LT / 2 + 1);
+  const dim3 ga((n_logical + 7) / 8 * 8, nb), blk(64 * kPairWaves);
+  hipExtLaunchKernelGGL((k_sh_analysis_pair<T, LT>), ga, blk, 0, ctx->stream, e0, e1, 0, ctx->N, ctx->sh_chain,
+                        ctx->sh_trig, x, stride, cpt, n_cblk, n_logical, ctx->sh_b);
+}
+
+// out = base + Yt^T D Minv Yt x for nb instances (x, base, out with instance
+// stride `stride`; D null: plain LSQ coefficients; base null: synthesis
+// alone).  ana / syn: run the analysis + coefficients / the synthesis.
+// ev (nullable): 6 dispatch-stamped events (analysis, coefficients,
+// synthesis).
 template <typename T>
-static int sh_effective_height(odis_ctx *ctx, Ops<T> &o) {
-  const int K = ctx->sh_K, N = ctx->N;
+static int sh_apply(odis_ctx *ctx, const T *x, long long stride, int nb, const double *D, const T *base, T *out,
+                    double *c, bool ana, bool syn, hipEvent_t *ev) {
+  const int K = ctx->sh_K, N = ctx->N, L = ctx->sh_L;
   const dim3 blk(kBlock);
-  if (ctx->sh_mode == ODIS_SH_SPLIT_BF16)
-    hipLaunchKernelGGL((k_sh_analysis<T, true>), dim3(K, ctx->B), blk, 0, ctx->stream, K, N, ctx->sh_Yt,
-                       ctx->sh_Yp, o.eta[ctx->cur], (long long)N, ctx->sh_b);
-  else
-    hipLaunchKernelGGL((k_sh_analysis<T, false>), dim3(K, ctx->B), blk, 0, ctx->stream, K, N, ctx->sh_Yt,
-                       ctx->sh_Yp, o.eta[ctx->cur], (long long)N, ctx->sh_b);
-  hipLaunchKernelGGL(k_sh_coef, dim3((K + 3) / 4, ctx->B), blk, 0, ctx->stream, K, ctx->sh_Minv, ctx->sh_D,
-                     ctx->sh_b, ctx->sh_c);
-  const size_t lds = sizeof(double) * K;
-  if (ctx->sh_mode == ODIS_SH_SPLIT_BF16)
-    hipLaunchKernelGGL((k_sh_synth<T, true>), dim3(ctx->nb_cell, ctx->B), blk, lds, ctx->stream, K, N, ctx->sh_Yt,
-                       ctx->sh_Yp, ctx->sh_c, o.eta[ctx->cur], (long long)N, static_cast<T *>(ctx->eta_eff));
-  else
-    hipLaunchKernelGGL((k_sh_synth<T, false>), dim3(ctx->nb_cell, ctx->B), blk, lds, ctx->stream, K, N,
-                       ctx->sh_Yt, ctx->sh_Yp, ctx->sh_c, o.eta[ctx->cur], (long long)N,
-                       static_cast<T *>(ctx->eta_eff));
+  const bool split = ctx->sh_mode == ODIS_SH_SPLIT_BF16;
+  // on the fly: chunks of 64 CPT cells, partials per workgroup of kShWaves chunks
+  const bool by_pair = ctx->sh_mode == ODIS_SH_RECUR && pair_degree(L);
+  // cells per lane: 16 for the pair analysis, 8 for the per-degree one
+  const int cpt = ctx->sh_cpt ? ctx->sh_cpt : (by_pair ? 16 : 8);
+  const int n_chunks = (N + 64 * cpt - 1) / (64 * cpt);
+  // cmaj holds S_l c for the compiled-degree synthesis
+  const double *chain = by_pair ? ctx->sh_chain : nullptr;
+  const int n_cblk = (N + 64 * kPairWaves * cpt - 1) / (64 * kPairWaves * cpt);
+  const int n_part = by_pair ? n_cblk
+                             : (ctx->sh_mode == ODIS_SH_RECUR ? (n_chunks + kShWaves - 1) / kShWaves : kShChunks);
+  if (n_part > ctx->sh_part) return fail(ctx, ODIS_E_INVALID, "analysis partials exceed their buffer");
+  if (by_pair) {
+    if (ana) {
+      hipEvent_t e0 = ev ? ev[0] : nullptr, e1 = ev ? ev[1] : nullptr;
+      if (L == 32)
+        launch_analysis_pair<T, 32>(ctx, nb, x, stride, cpt, n_cblk, e0, e1);
+      else if (L == 16)
+        launch_analysis_pair<T, 16>(ctx, nb, x, stride, cpt, n_cblk, e0, e1);
+      else
+        launch_analysis_pair<T, 8>(ctx, nb, x, stride, cpt, n_cblk, e0, e1);
+    }
+  } else if (ctx->sh_mode == ODIS_SH_RECUR) {
+    if (ana) {
+      hipEvent_t e0 = ev ? ev[0] : nullptr, e1 = ev ? ev[1] : nullptr;
+      if (cpt == 4)
+        launch_analysis_rec<T, 4>(ctx, nb, L, x, stride, n_chunks, e0, e1);
+      else if (cpt == 16)
+        launch_analysis_rec<T, 16>(ctx, nb, L, x, stride, n_chunks, e0, e1);
+      else
+        launch_analysis_rec<T, 8>(ctx, nb, L, x, stride, n_chunks, e0, e1);
+    }
+  } else if (ana) {
+    const dim3 ga(K * kShChunks, nb);
+    if (split)
+      hipExtLaunchKernelGGL((k_sh_analysis<T, true>), ga, blk, 0, ctx->stream, ev ? ev[0] : nullptr,
+                            ev ? ev[1] : nullptr, 0, K, N, ctx->sh_Yt, ctx->sh_Yp, x, stride, ctx->sh_b);
+    else
+      hipExtLaunchKernelGGL((k_sh_analysis<T, false>), ga, blk, 0, ctx->stream, ev ? ev[0] : nullptr,
+                            ev ? ev[1] : nullptr, 0, K, N, ctx->sh_Yt, ctx->sh_Yp, x, stride, ctx->sh_b);
+  }
+  if (ana) {
+    // partial sums, then c = D Minv b; events around both
+    const double *bsum = ctx->sh_b + size_t(nb) * n_part * K;
+    hipExtLaunchKernelGGL(k_sh_bsum, dim3((K + 63) / 64, nb), blk, 0, ctx->stream, ev ? ev[2] : nullptr,
+                          nullptr, 0, K, n_part, ctx->sh_b);
+    hipExtLaunchKernelGGL(k_sh_coef, dim3((K + 3) / 4, nb), blk, sizeof(double) * K, ctx->stream, nullptr,
+                          ev ? ev[3] : nullptr, 0, K, ctx->sh_Minv, D, bsum, c, chain,
+                          ctx->sh_mode == ODIS_SH_RECUR ? ctx->sh_cmaj : static_cast<double *>(nullptr));
+  }
+  if (syn && ctx->sh_mode == ODIS_SH_RECUR) {
+    if (!ana)  // coefficients from the caller: their m-major copy first
+      hipLaunchKernelGGL(k_sh_to_mmajor, dim3((K + kBlock - 1) / kBlock, nb), blk, 0, ctx->stream, K, c, chain,
+                         ctx->sh_cmaj);
+    const dim3 gs((N + 63) / 64, nb);
+    const int4 mr = synth_split(L);
+    hipEvent_t e4 = ev ? ev[4] : nullptr, e5 = ev ? ev[5] : nullptr;
+    if (L == 32)
+      hipExtLaunchKernelGGL((k_sh_synth_chain<T, 32>), gs, blk, 0, ctx->stream, e4, e5, 0, N, ctx->sh_chain,
+                            ctx->sh_tabm, ctx->sh_trig, ctx->sh_cmaj, base, stride, out, mr);
+    else if (L == 16)
+      hipExtLaunchKernelGGL((k_sh_synth_chain<T, 16>), gs, blk, 0, ctx->stream, e4, e5, 0, N, ctx->sh_chain,
+                            ctx->sh_tabm, ctx->sh_trig, ctx->sh_cmaj, base, stride, out, mr);
+    else if (L == 8)
+      hipExtLaunchKernelGGL((k_sh_synth_chain<T, 8>), gs, blk, 0, ctx->stream, e4, e5, 0, N, ctx->sh_chain,
+                            ctx->sh_tabm, ctx->sh_trig, ctx->sh_cmaj, base, stride, out, mr);
+    else
+      hipExtLaunchKernelGGL((k_sh_synth_rec<T, 0>), gs, blk, 0, ctx->stream, e4, e5, 0, L, N, ctx->sh_tabm,
+                            ctx->sh_trig, ctx->sh_cmaj, base, stride, out, mr);
+  } else if (syn) {
+    const dim3 gs((N + 63) / 64, nb);
+    const size_t lds_s = sizeof(double) * (K + kBlock);
+    if (split)
+      hipExtLaunchKernelGGL((k_sh_synth<T, true>), gs, blk, lds_s, ctx->stream, ev ? ev[4] : nullptr,
+                            ev ? ev[5] : nullptr, 0, K, N, ctx->sh_Yt, ctx->sh_Yp, c, base, stride, out);
+    else
+      hipExtLaunchKernelGGL((k_sh_synth<T, false>), gs, blk, lds_s, ctx->stream, ev ? ev[4] : nullptr,
+                            ev ? ev[5] : nullptr, 0, K, N, ctx->sh_Yt, ctx->sh_Yp, c, base, stride, out);
+  }
   CK(hipGetLastError());
   return ODIS_OK;
+}
+
+// eta_eff = eta + Yt^T D Minv Yt eta for every instance (three kernels)
+template <typename T>
+static int sh_effective_height(odis_ctx *ctx, Ops<T> &o, hipEvent_t *ev) {
+  return sh_apply<T>(ctx, o.eta[ctx->cur], ctx->N, ctx->B, ctx->sh_D, o.eta[ctx->cur],
+                     static_cast<T *>(ctx->eta_eff), ctx->sh_c, true, true, ev);
 }
 
 // Event pairs (start, end) recorded for the kernels of sampled steps.
 struct ChunkEvents {
   std::vector<int> mom;   // momentum kernel (k_face, or the one-kernel k_step) of each sampled step
   std::vector<int> cont;  // continuity kernel (k_cell) of each sampled face + cell step
+  std::vector<int> sh;    // 3 event pairs (analysis, coefficients, synthesis) of each sampled SH step
 };
 
 static int reserve_events(odis_ctx *ctx, size_t n) {
@@ -1549,7 +2267,7 @@ static int launch_chunk(odis_ctx *ctx, Ops<T> &o, const double *table, int count
   const long long n0 = ctx->step;
   if (evs) {
     const size_t sampled = size_t(count / ctx->timing + 2);
-    const int rc = reserve_events(ctx, *ev_used + 4 * sampled);
+    const int rc = reserve_events(ctx, *ev_used + 10 * sampled);
     if (rc) return rc;
   }
   if (ctx->compact && ctx->one_kernel && ctx->sh_mode == ODIS_SH_OFF) {
@@ -1576,14 +2294,21 @@ static int launch_chunk(odis_ctx *ctx, Ops<T> &o, const double *table, int count
     return ODIS_OK;
   }
   for (int s = 0; s < count; s++) {
+    const bool stamp = evs && (n0 + s) % ctx->timing == 0;
     if (ctx->sh_mode != ODIS_SH_OFF) {
-      const int rc = sh_effective_height(ctx, o);
+      hipEvent_t *evsh = nullptr;
+      if (stamp) {
+        evs->sh.push_back(int(*ev_used));
+        evsh = &ctx->ev[*ev_used];
+        *ev_used += 6;
+      }
+      const int rc = sh_effective_height(ctx, o, evsh);
       if (rc) return rc;
     }
     FaceArgs<T> fa = face_args(ctx, o);
     CellArgs<T> ca = cell_args(ctx, o);
     hipEvent_t *evf = nullptr, *evc = nullptr;
-    if (evs && (n0 + s) % ctx->timing == 0) {
+    if (stamp) {
       evs->mom.push_back(int(*ev_used));
       evs->cont.push_back(int(*ev_used + 2));
       evf = &ctx->ev[*ev_used];
@@ -1697,6 +2422,30 @@ static int step_t(odis_ctx *ctx, Ops<T> &o, int n_steps) {
     ctx->t_face = ns ? tm / ns : 0.0;
     ctx->t_cell = ns ? tc / ns : 0.0;
     ctx->t_step = ns ? ts / ns : 0.0;
+    double tsh[3] = {0, 0, 0};
+    long long nsh = 0;
+    for (const ChunkEvents &c : sampled)
+      for (int base : c.sh) {
+        for (int q = 0; q < 3; q++) {
+          float a = 0;
+          CK(hipEventElapsedTime(&a, ctx->ev[base + 2 * q], ctx->ev[base + 2 * q + 1]));
+          tsh[q] += a;
+        }
+        nsh++;
+      }
+    for (int q = 0; q < 3; q++) ctx->t_sh[q] = nsh ? tsh[q] / nsh : 0.0;
+    if (nsh) {  // a sampled SH step runs from the analysis to the cell kernel
+      ctx->t_step = 0.0;
+      long long k = 0;
+      for (const ChunkEvents &c : sampled)
+        for (size_t j = 0; j < c.sh.size() && j < c.cont.size(); j++) {
+          float w = 0;
+          CK(hipEventElapsedTime(&w, ctx->ev[c.sh[j]], ctx->ev[c.cont[j] + 1]));
+          ctx->t_step += w;
+          k++;
+        }
+      ctx->t_step = k ? ctx->t_step / k : 0.0;
+    }
   }
   return ODIS_OK;
 }
@@ -1849,6 +2598,14 @@ extern "C" int odis_set_option(odis_ctx *ctx, int option, int value) {
     ctx->n_patch = (ctx->N + value - 1) / value;
     return ODIS_OK;
   }
+  if (option == ODIS_OPT_SH_CELLS) {
+    if (value != 0 && value != 4 && value != 8 && value != 16)
+      return fail(ctx, ODIS_E_INVALID, "SH cells per lane: 0 (automatic), 4, 8 or 16");
+    CK(hipStreamSynchronize(ctx->stream));
+    drop_graphs(ctx);
+    ctx->sh_cpt = value;
+    return ODIS_OK;
+  }
   if (option == ODIS_OPT_STEP_LANES) {
     if (value != 2 && value != 4) return fail(ctx, ODIS_E_INVALID, "step lanes must be 2 or 4");
     CK(hipStreamSynchronize(ctx->stream));
@@ -1885,7 +2642,15 @@ extern "C" int odis_get_option(odis_ctx *ctx, int option, int *value) {
   else if (option == ODIS_OPT_PATCH_CELLS) *value = ctx->patch;
   else if (option == ODIS_OPT_STEP_BLOCKS) *value = step_blocks(ctx);
   else if (option == ODIS_OPT_STEP_LANES) *value = ctx->step_lanes;
+  else if (option == ODIS_OPT_SH_CELLS) *value = ctx->sh_cpt;
   else return fail(ctx, ODIS_E_INVALID, "unknown option");
+  return ODIS_OK;
+}
+
+extern "C" int odis_get_kernel_times(odis_ctx *ctx, double *ms, int n) {
+  if (!ctx || !ms || n < 0) return ODIS_E_INVALID;
+  const double v[ODIS_T_COUNT] = {ctx->t_face, ctx->t_cell, ctx->t_step, ctx->t_sh[0], ctx->t_sh[1], ctx->t_sh[2]};
+  for (int q = 0; q < n && q < ODIS_T_COUNT; q++) ms[q] = v[q];
   return ODIS_OK;
 }
 
@@ -2014,7 +2779,7 @@ static bool spd_inverse(std::vector<double> &A, int n) {
 
 extern "C" int odis_sh_set_basis(odis_ctx *ctx, int l_max, const double *beta, int mode) {
   if (!ctx) return ODIS_E_INVALID;
-  if (mode != ODIS_SH_OFF && mode != ODIS_SH_FP64 && mode != ODIS_SH_SPLIT_BF16)
+  if (mode != ODIS_SH_OFF && mode != ODIS_SH_FP64 && mode != ODIS_SH_SPLIT_BF16 && mode != ODIS_SH_RECUR)
     return fail(ctx, ODIS_E_INVALID, "bad sh mode");
   CK(hipStreamSynchronize(ctx->stream));
   drop_graphs(ctx);
@@ -2022,11 +2787,19 @@ extern "C" int odis_sh_set_basis(odis_ctx *ctx, int l_max, const double *beta, i
   ctx->sh_allocs.clear();
   ctx->sh_mode = ODIS_SH_OFF;
   ctx->sh_L = ctx->sh_K = 0;
+  ctx->sh_Yt = ctx->sh_Minv = ctx->sh_D = ctx->sh_b = ctx->sh_c = ctx->sh_tab = ctx->sh_trig = nullptr;
+  ctx->sh_tabm = ctx->sh_cmaj = ctx->sh_chain = nullptr;
+  ctx->sh_Yp = nullptr;
+  ctx->eta_eff = nullptr;
   if (mode == ODIS_SH_OFF) return ODIS_OK;
   if (l_max < 0 || l_max > 100) return fail(ctx, ODIS_E_INVALID, "l_max out of range");
   const int K = (l_max + 1) * (l_max + 1), N = ctx->N;
   if (K > N) return fail(ctx, ODIS_E_INVALID, "(l_max+1)^2 exceeds the number of cells");
-  if (size_t(K) * sizeof(double) > 64 * 1024) return fail(ctx, ODIS_E_INVALID, "l_max too large for LDS");
+  // the synthesis keeps the K coefficients and 4 x 64 wave partials (stored
+  // basis) or the recurrence table (on the fly) in LDS
+  const size_t lds_words = mode == ODIS_SH_RECUR ? size_t(K) + size_t(l_max + 1) * (l_max + 2) + 2 * (l_max + 1)
+                                                 : size_t(K) + kBlock;
+  if (lds_words * sizeof(double) > 64 * 1024) return fail(ctx, ODIS_E_INVALID, "l_max too large for LDS");
   // basis on the host in reference order, then columns in device order
   std::vector<double> Yref;
   odis::sh_basis(l_max, ctx->cell_lat, ctx->cell_lon, Yref);
@@ -2056,6 +2829,75 @@ extern "C" int odis_sh_set_basis(odis_ctx *ctx, int l_max, const double *beta, i
   if (!spd_inverse(M, K)) return fail(ctx, ODIS_E_INVALID, "basis Gram matrix is singular");
   CK(hipMemcpy(dM, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice));
   ctx->sh_Minv = dM;
+  ctx->sh_part = kShChunks;
+  if (mode == ODIS_SH_RECUR) {
+    // the stored basis was only needed for the Gram matrix
+    (void)hipFree(ctx->sh_Yt);
+    ctx->sh_allocs.erase(std::remove(ctx->sh_allocs.begin(), ctx->sh_allocs.end(), (void *)ctx->sh_Yt),
+                         ctx->sh_allocs.end());
+    ctx->sh_Yt = nullptr;
+    std::vector<double> a, bt, sect, f1;
+    odis::sh_recurrence_tables(l_max, a, bt, sect, f1);
+    std::vector<double> tab;
+    tab.insert(tab.end(), a.begin(), a.end());
+    tab.insert(tab.end(), bt.begin(), bt.end());
+    tab.insert(tab.end(), sect.begin(), sect.end());
+    tab.insert(tab.end(), f1.begin(), f1.end());
+    std::vector<double> trig(size_t(4) * N);
+    for (int r = 0; r < N; r++) {
+      const int i = ctx->cell_inv[r];
+      trig[r] = std::sin(ctx->cell_lat[i]);
+      trig[size_t(N) + r] = std::cos(ctx->cell_lat[i]);
+      trig[size_t(2) * N + r] = std::cos(ctx->cell_lon[i]);
+      trig[size_t(3) * N + r] = std::sin(ctx->cell_lon[i]);
+    }
+    if ((rc = alloc(reinterpret_cast<void **>(&ctx->sh_tab), tab.size() * sizeof(double)))) return rc;
+    CK(hipMemcpy(ctx->sh_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    // m-major: sect, f1, then per m the pairs (a, b) for l = m + 2 .. L
+    std::vector<double> tabm(sect.begin(), sect.end());
+    tabm.insert(tabm.end(), f1.begin(), f1.end());
+    for (int m = 0; m <= l_max; m++)
+      for (int l = m + 2; l <= l_max; l++) {
+        tabm.push_back(a[odis::sh_lm_index(l, m)]);
+        tabm.push_back(bt[odis::sh_lm_index(l, m)]);
+      }
+    if ((rc = alloc(reinterpret_cast<void **>(&ctx->sh_tabm), tabm.size() * sizeof(double)))) return rc;
+    CK(hipMemcpy(ctx->sh_tabm, tabm.data(), tabm.size() * sizeof(double), hipMemcpyHostToDevice));
+    // rescaled chains: q_l = mu q_{l-1} + beta_l q_{l-2}, P_lm = S_l q_l
+    std::vector<double> chain;
+    for (int m = 0; m <= l_max; m++) {
+      double S = 1.0, a_prev = f1[m];
+      chain.push_back(1.0);  // k = 0: q = P_mm through q_{-2}
+      chain.push_back(1.0);
+      if (m < l_max) {
+        S = f1[m];
+        chain.push_back(0.0);  // k = 1: q = mu P_mm
+        chain.push_back(S);
+      }
+      for (int l = m + 2; l <= l_max; l++) {
+        const double al = a[odis::sh_lm_index(l, m)], bl = bt[odis::sh_lm_index(l, m)];
+        S *= al;
+        chain.push_back(-bl / (al * a_prev));
+        chain.push_back(S);
+        a_prev = al;
+      }
+    }
+    double C = 1.0;
+    for (int m = 0; m <= l_max; m++) {
+      if (m > 0) C *= sect[m];
+      chain.push_back(C);
+    }
+    if ((rc = alloc(reinterpret_cast<void **>(&ctx->sh_chain), chain.size() * sizeof(double)))) return rc;
+    CK(hipMemcpy(ctx->sh_chain, chain.data(), chain.size() * sizeof(double), hipMemcpyHostToDevice));
+    if ((rc = alloc(reinterpret_cast<void **>(&ctx->sh_cmaj), size_t(ctx->B) * (l_max + 1) * (l_max + 2) *
+                                                                   sizeof(double))))
+      return rc;
+    if ((rc = alloc(reinterpret_cast<void **>(&ctx->sh_trig), trig.size() * sizeof(double)))) return rc;
+    CK(hipMemcpy(ctx->sh_trig, trig.data(), trig.size() * sizeof(double), hipMemcpyHostToDevice));
+    // the most partials: 4 cells per lane
+    ctx->sh_part = std::max(((N + 64 * 4 - 1) / (64 * 4) + kShWaves - 1) / kShWaves,
+                            (N + 64 * kPairWaves * 4 - 1) / (64 * kPairWaves * 4));
+  }
   if (mode == ODIS_SH_SPLIT_BF16) {
     std::vector<unsigned int> P(Y.size());
     for (size_t q = 0; q < Y.size(); q++) {
@@ -2075,7 +2917,10 @@ extern "C" int odis_sh_set_basis(odis_ctx *ctx, int l_max, const double *beta, i
         for (int m = -l; m <= l; m++) D[size_t(b) * K + l * l + l + m] = beta[size_t(b) * (l_max + 1) + l] - 1.0;
   if ((rc = alloc(reinterpret_cast<void **>(&ctx->sh_D), D.size() * sizeof(double)))) return rc;
   CK(hipMemcpy(ctx->sh_D, D.data(), D.size() * sizeof(double), hipMemcpyHostToDevice));
-  if ((rc = alloc(reinterpret_cast<void **>(&ctx->sh_b), size_t(ctx->B) * K * sizeof(double)))) return rc;
+  // analysis partials [B][sh_part][K], then their sums [B][K]
+  if ((rc = alloc(reinterpret_cast<void **>(&ctx->sh_b),
+                  size_t(ctx->B) * (ctx->sh_part + 1) * K * sizeof(double))))
+    return rc;
   if ((rc = alloc(reinterpret_cast<void **>(&ctx->sh_c), size_t(ctx->B) * K * sizeof(double)))) return rc;
   const size_t tsz = ctx->precision == ODIS_FP64 ? sizeof(double) : sizeof(float);
   if ((rc = alloc(&ctx->eta_eff, size_t(ctx->B) * N * tsz))) return rc;
@@ -2092,27 +2937,19 @@ extern "C" int odis_sh_project(odis_ctx *ctx, const double *field, double *coeff
   const int K = ctx->sh_K, N = ctx->N;
   std::vector<double> x(N);
   for (int r = 0; r < N; r++) x[r] = field[ctx->cell_inv[r]];
-  double *dx = nullptr, *db = nullptr, *dc = nullptr;
+  double *dx = nullptr, *dc = nullptr;
+  CK(hipStreamSynchronize(ctx->stream));
   CK(hipMalloc(&dx, sizeof(double) * N));
-  hipError_t e = hipMalloc(&db, sizeof(double) * K);
-  if (e == hipSuccess) e = hipMalloc(&dc, sizeof(double) * K);
+  hipError_t e = hipMalloc(&dc, sizeof(double) * K);
   if (e == hipSuccess) e = hipMemcpy(dx, x.data(), sizeof(double) * N, hipMemcpyHostToDevice);
-  if (e == hipSuccess) {
-    if (ctx->sh_mode == ODIS_SH_SPLIT_BF16)
-      hipLaunchKernelGGL((k_sh_analysis<double, true>), dim3(K, 1), dim3(kBlock), 0, ctx->stream, K, N, ctx->sh_Yt,
-                         ctx->sh_Yp, dx, (long long)N, db);
-    else
-      hipLaunchKernelGGL((k_sh_analysis<double, false>), dim3(K, 1), dim3(kBlock), 0, ctx->stream, K, N,
-                         ctx->sh_Yt, ctx->sh_Yp, dx, (long long)N, db);
-    hipLaunchKernelGGL(k_sh_coef, dim3((K + 3) / 4, 1), dim3(kBlock), 0, ctx->stream, K, ctx->sh_Minv,
-                       static_cast<const double *>(nullptr), db, dc);
-    e = hipGetLastError();
-  }
-  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  if (e == hipSuccess) e = hipMemcpy(coeffs, dc, sizeof(double) * K, hipMemcpyDeviceToHost);
+  int rc = ODIS_OK;
+  if (e == hipSuccess)
+    rc = sh_apply<double>(ctx, dx, N, 1, nullptr, nullptr, nullptr, dc, true, false, nullptr);
+  if (e == hipSuccess && !rc) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess && !rc) e = hipMemcpy(coeffs, dc, sizeof(double) * K, hipMemcpyDeviceToHost);
   (void)hipFree(dx);
-  (void)hipFree(db);
   (void)hipFree(dc);
+  if (rc) return rc;
   if (e != hipSuccess) return fail(ctx, ODIS_E_HIP, hipGetErrorString(e));
   return ODIS_OK;
 }
@@ -2122,24 +2959,20 @@ extern "C" int odis_sh_synthesize(odis_ctx *ctx, const double *coeffs, double *f
   if (ctx->sh_mode == ODIS_SH_OFF) return fail(ctx, ODIS_E_INVALID, "no spherical-harmonic basis set");
   const int K = ctx->sh_K, N = ctx->N;
   double *dy = nullptr, *dc = nullptr;
+  CK(hipStreamSynchronize(ctx->stream));
   CK(hipMalloc(&dy, sizeof(double) * N));
   hipError_t e = hipMalloc(&dc, sizeof(double) * K);
   if (e == hipSuccess) e = hipMemcpy(dc, coeffs, sizeof(double) * K, hipMemcpyHostToDevice);
-  if (e == hipSuccess) {
-    const size_t lds = sizeof(double) * K;
-    if (ctx->sh_mode == ODIS_SH_SPLIT_BF16)
-      hipLaunchKernelGGL((k_sh_synth<double, true>), dim3(ctx->nb_cell, 1), dim3(kBlock), lds, ctx->stream, K, N,
-                         ctx->sh_Yt, ctx->sh_Yp, dc, static_cast<const double *>(nullptr), (long long)N, dy);
-    else
-      hipLaunchKernelGGL((k_sh_synth<double, false>), dim3(ctx->nb_cell, 1), dim3(kBlock), lds, ctx->stream, K,
-                         N, ctx->sh_Yt, ctx->sh_Yp, dc, static_cast<const double *>(nullptr), (long long)N, dy);
-    e = hipGetLastError();
-  }
+  int rc = ODIS_OK;
+  if (e == hipSuccess)
+    rc = sh_apply<double>(ctx, static_cast<const double *>(nullptr), N, 1, nullptr, nullptr, dy, dc, false, true,
+                          nullptr);
   std::vector<double> y(N);
-  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  if (e == hipSuccess) e = hipMemcpy(y.data(), dy, sizeof(double) * N, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && !rc) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess && !rc) e = hipMemcpy(y.data(), dy, sizeof(double) * N, hipMemcpyDeviceToHost);
   (void)hipFree(dy);
   (void)hipFree(dc);
+  if (rc) return rc;
   if (e != hipSuccess) return fail(ctx, ODIS_E_HIP, hipGetErrorString(e));
   for (int r = 0; r < N; r++) field[ctx->cell_inv[r]] = y[r];
   return ODIS_OK;

@@ -118,6 +118,13 @@ struct CompactTrisk {
 std::string build_compact(const odis_mesh_desc &m, CompactTrisk &c);
 
 // Real 4pi-normalised spherical harmonics, k = l*l + l + m (odis_sh.cpp).
+// Recurrence coefficients of P_lm (index sh_lm_index(l, m), m <= l):
+//   P_mm = P_{m-1,m-1} sect[m] cos(lat) (P_11 = sect[1] cos(lat)),
+//   P_{m+1,m} = f1[m] sin(lat) P_mm,
+//   P_lm = a_lm sin(lat) P_{l-1,m} - b_lm P_{l-2,m}.
+inline int sh_lm_index(int l, int m) { return l * (l + 1) / 2 + m; }
+void sh_recurrence_tables(int L, std::vector<double> &a, std::vector<double> &b, std::vector<double> &sect,
+                          std::vector<double> &f1);
 void sh_basis_row(int L, double lat, double lon, double *Y);
 void sh_basis(int L, const std::vector<double> &lat, const std::vector<double> &lon, std::vector<double> &Yt);
 std::vector<double> surface_beta(int surface, int L, double radius, double g, double shell, double ocean);

@@ -21,15 +21,35 @@
 
 namespace odis {
 
-void sh_basis_row(int L, double lat, double lon, double *Y) {
+void sh_recurrence_tables(int L, std::vector<double> &a, std::vector<double> &b, std::vector<double> &sect,
+                          std::vector<double> &f1) {
+  const size_t n = size_t(L + 1) * (L + 2) / 2;
+  a.assign(n, 0.0);
+  b.assign(n, 0.0);
+  sect.assign(L + 1, 0.0);
+  f1.assign(L + 1, 0.0);
+  for (int m = 0; m <= L; m++) {
+    sect[m] = m == 0 ? 1.0 : (m == 1 ? std::sqrt(3.0) : std::sqrt((2.0 * m + 1.0) / (2.0 * m)));
+    f1[m] = std::sqrt(2.0 * m + 3.0);
+    for (int l = m + 2; l <= L; l++) {
+      const double ll = l, mm = m;
+      a[sh_lm_index(l, m)] = std::sqrt((2 * ll + 1) * (2 * ll - 1) / ((ll - mm) * (ll + mm)));
+      b[sh_lm_index(l, m)] =
+          std::sqrt((2 * ll + 1) * (ll - mm - 1) * (ll + mm - 1) / ((2 * ll - 3) * (ll - mm) * (ll + mm)));
+    }
+  }
+}
+
+static void basis_row(int L, double lat, double lon, const std::vector<double> &a, const std::vector<double> &b,
+                      const std::vector<double> &sect, const std::vector<double> &f1, double *Y) {
   const int K = (L + 1) * (L + 1);
   for (int k = 0; k < K; k++) Y[k] = 0.0;
   const double x = std::sin(lat), s = std::cos(lat);
   // P_mm by the sectoral recurrence, then upward in l at fixed m
   double pmm = 1.0;  // P_00
   for (int m = 0; m <= L; m++) {
-    if (m == 1) pmm = std::sqrt(3.0) * s;
-    else if (m > 1) pmm *= std::sqrt((2.0 * m + 1.0) / (2.0 * m)) * s;
+    if (m == 1) pmm = sect[1] * s;
+    else if (m > 1) pmm *= sect[m] * s;
     const double cm = std::cos(m * lon), sm = std::sin(m * lon);
     double p2 = 0.0, p1 = pmm;  // P_{l-2,m}, P_{l-1,m}
     for (int l = m; l <= L; l++) {
@@ -37,12 +57,9 @@ void sh_basis_row(int L, double lat, double lon, double *Y) {
       if (l == m) {
         p = pmm;
       } else if (l == m + 1) {
-        p = std::sqrt(2.0 * m + 3.0) * x * pmm;
+        p = f1[m] * x * pmm;
       } else {
-        const double ll = l, mm = m;
-        const double a = std::sqrt((2 * ll + 1) * (2 * ll - 1) / ((ll - mm) * (ll + mm)));
-        const double b = std::sqrt((2 * ll + 1) * (ll - mm - 1) * (ll + mm - 1) / ((2 * ll - 3) * (ll - mm) * (ll + mm)));
-        p = a * x * p1 - b * p2;
+        p = a[sh_lm_index(l, m)] * x * p1 - b[sh_lm_index(l, m)] * p2;
       }
       if (l > m) {
         p2 = p1;
@@ -54,14 +71,21 @@ void sh_basis_row(int L, double lat, double lon, double *Y) {
   }
 }
 
+void sh_basis_row(int L, double lat, double lon, double *Y) {
+  std::vector<double> a, b, sect, f1;
+  sh_recurrence_tables(L, a, b, sect, f1);
+  basis_row(L, lat, lon, a, b, sect, f1, Y);
+}
+
 void sh_basis(int L, const std::vector<double> &lat, const std::vector<double> &lon, std::vector<double> &Yt) {
   if (L < 0 || L > 200) throw std::runtime_error("sh degree out of range");
   const int K = (L + 1) * (L + 1);
   const size_t N = lat.size();
   Yt.assign(size_t(K) * N, 0.0);
-  std::vector<double> row(K);
+  std::vector<double> row(K), a, b, sect, f1;
+  sh_recurrence_tables(L, a, b, sect, f1);
   for (size_t i = 0; i < N; i++) {
-    sh_basis_row(L, lat[i], lon[i], row.data());
+    basis_row(L, lat[i], lon[i], a, b, sect, f1, row.data());
     for (int k = 0; k < K; k++) Yt[size_t(k) * N + i] = row[k];
   }
 }

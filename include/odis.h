@@ -165,11 +165,15 @@ int odis_interp_ll(odis_ctx *ctx, int inst, double *ll);
  * (as SHExpandLSQ, src/extractSHCoeffGG.f95:29), Y_lm real 4pi-normalised
  * harmonics, index k = l*l + l + m.  beta holds n_inst rows of (l_max + 1)
  * per-degree factors (e.g. membraneNuBeta's beta_l for LID_MEMBR).
- * mode: ODIS_SH_OFF, ODIS_SH_FP64 (FP64 basis, FP64 sums) or
- * ODIS_SH_SPLIT_BF16 (basis stored as bf16 hi + bf16 lo, FP32 accumulation). */
+ * mode: ODIS_SH_OFF; ODIS_SH_FP64 (basis stored in FP64, FP64 sums);
+ * ODIS_SH_SPLIT_BF16 (basis stored as bf16 hi + bf16 lo, FP32 accumulation);
+ * ODIS_SH_RECUR (basis evaluated on the fly by the Legendre recurrence in
+ * the context's precision; nothing of size N x K is kept).  The Gram
+ * matrix and its inverse are FP64 in every mode. */
 #define ODIS_SH_OFF 0
 #define ODIS_SH_FP64 1
 #define ODIS_SH_SPLIT_BF16 2
+#define ODIS_SH_RECUR 3
 int odis_sh_set_basis(odis_ctx *ctx, int l_max, const double *beta, int mode);
 /* Coefficients (K = (l_max+1)^2) of a cell field in reference numbering, and
  * the synthesis of a field from coefficients, with the context's basis. */
@@ -202,13 +206,18 @@ int odis_request_stop(odis_ctx *ctx);
  *                    reading it returns the count in use).  The dissipation
  *                    sums depend on it by rounding only.
  *  ODIS_OPT_STEP_LANES lanes per cell of the one-kernel step: 4 (default,
- *                    about one face per lane) or 2 (about two). */
+ *                    about one face per lane) or 2 (about two).
+ *  ODIS_OPT_SH_CELLS cells per lane of the on-the-fly SH analysis: 0
+ *                    (default: 16 for degrees 8, 16 and 32, which have
+ *                    compiled kernels, 8 for the others), 4, 8 or 16; the
+ *                    coefficients depend on it by rounding only. */
 #define ODIS_OPT_GRAPHS     1
 #define ODIS_OPT_COMPACT    2
 #define ODIS_OPT_ONE_KERNEL 3
 #define ODIS_OPT_PATCH_CELLS 4
 #define ODIS_OPT_STEP_BLOCKS 5
 #define ODIS_OPT_STEP_LANES  6
+#define ODIS_OPT_SH_CELLS    7
 int odis_set_option(odis_ctx *ctx, int option, int value);
 int odis_get_option(odis_ctx *ctx, int option, int *value);
 /* Timing for benchmarks: with stride T > 0, the kernels of every step n of
@@ -219,6 +228,16 @@ int odis_get_option(odis_ctx *ctx, int option, int *value);
  * odis_step call.  T = 0 disables. */
 int odis_enable_timing(odis_ctx *ctx, int stride);
 int odis_get_timing(odis_ctx *ctx, double *face_ms, double *cell_ms, double *step_ms);
+/* The same averages and those of the SH kernels of sampled steps, indexed
+ * by ODIS_T_*; fills min(n, ODIS_T_COUNT) entries. */
+#define ODIS_T_MOMENTUM    0
+#define ODIS_T_CONTINUITY  1
+#define ODIS_T_STEP        2
+#define ODIS_T_SH_ANALYSIS 3
+#define ODIS_T_SH_COEF     4
+#define ODIS_T_SH_SYNTH    5
+#define ODIS_T_COUNT       6
+int odis_get_kernel_times(odis_ctx *ctx, double *ms, int n);
 
 /* ---- host: grid and mesh (src/mesh.cpp:32-197, 436-1154, 2808-3287) ---- */
 

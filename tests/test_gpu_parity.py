@@ -285,11 +285,12 @@ def test_interp_gg2ll(product, oracle, precision):
 # spherical-harmonic self-gravity / ice-shell coupling
 # ---------------------------------------------------------------------------
 
-@pytest.mark.parametrize("mode,tol", [(1, 1e-11), (2, 2e-5)])
+@pytest.mark.parametrize("mode,tol", [(1, 1e-11), (2, 2e-5), (3, 1e-11)])
 def test_sh_synthesis_and_projection(product, oracle, mode, tol):
     """Synthesis of unit coefficients reproduces scipy's harmonics on the
     nodes; the GPU least-squares projection recovers band-limited random
-    coefficients (FP64: rounding level; split-BF16: ~1e-6)."""
+    coefficients (stored FP64 basis and on-the-fly recurrence: rounding
+    level; split-BF16: ~1e-6)."""
     G, O = product, oracle
     g, arrs = generated_grid(5)
     p, _ = G.make_params()
@@ -303,15 +304,15 @@ def test_sh_synthesis_and_projection(product, oracle, mode, tol):
         e = np.zeros(K)
         e[k] = 1.0
         # bf16 hi + lo keeps ~16 significant bits: |Y| <= sqrt(2l+1)-ish
-        bound = 1e-12 if mode == 1 else 2 ** -16 * np.max(np.abs(Y[:, k]))
+        bound = 2 ** -16 * np.max(np.abs(Y[:, k])) if mode == 2 else 1e-12
         assert np.max(np.abs(ctx.sh_synthesize(e) - Y[:, k])) < bound
     c_true = np.random.default_rng(11).standard_normal(K)
     c = ctx.sh_project(Y @ c_true)
     assert rel_l2(c, c_true) < tol
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_self_gravity_shell_step_parity(product, oracle, precision):
+@pytest.mark.parametrize("precision,mode", [(0, 1), (1, 1), (0, 3), (1, 3)])
+def test_self_gravity_shell_step_parity(product, oracle, precision, mode):
     """Ice shell under LID_MEMBR factors up to degree 8: the pressure term of
     the GPU step acts on eta + sum (beta_l - 1) c_lm Y_lm; the oracle gets the
     same term from numpy (pinv of scipy's harmonics).  FP64 to 1e-12 per
@@ -325,7 +326,7 @@ def test_self_gravity_shell_step_parity(product, oracle, precision):
                          shell_thickness=20e3, love_reduct=0.99)
     mesh = G.Mesh(g, p.radius)
     ctx = G.Context(mesh, p, precision=precision)
-    ctx.set_sh(L, beta, mode=G.SH_FP64)
+    ctx.set_sh(L, beta, mode=mode)
     Y = O.real_sh_basis(L, arrs["lat"], arrs["lon"])
     dfac = np.zeros((L + 1) ** 2)
     for l in range(2, L + 1):
@@ -519,3 +520,34 @@ def test_one_kernel_step_matches_face_cell_pair(product, precision):
             one.step(33)
             pair.step(33)
             _assert_same_state(one, pair, (0, 1), (level, "patch", cells, lanes, blocks))
+
+
+@pytest.mark.parametrize("precision,L", [(0, 16), (1, 16), (0, 12), (1, 12)])
+def test_sh_recurrence_matches_stored_basis(product, precision, L):
+    """The basis evaluated on the fly and the stored basis give the same
+    coupled trajectory: to rounding in FP64, within the FP32 bound in FP32
+    (40962 cells, two instances with different shell factors).  Degree 16
+    runs the compiled pair analysis and chain synthesis, degree 12 the
+    runtime-degree kernels."""
+    G = product
+    g, _ = generated_grid(6)
+    p, _ = G.make_params(time_step=2.0, radius=232.1e3, g=0.1166, surface=G.LID_MEMBR,
+                         shell_thickness=20e3, love_reduct=0.99)
+    mesh = G.Mesh(g, p.radius)
+    b1 = np.array(G.surface_beta(G.LID_MEMBR, L, 252.1e3, 0.113, 20e3, 38e3))
+    b2 = np.array(G.surface_beta(G.LID_MEMBR, L, 252.1e3, 0.113, 10e3, 38e3))
+    beta = np.stack([b1, b2])
+    ctxs = []
+    for mode in (G.SH_FP64, G.SH_RECUR):
+        c = G.Context(mesh, [p, p], precision=precision)
+        c.set_sh(L, beta, mode=mode)
+        ctxs.append(c)
+    tol = 1e-12 if precision == 0 else 1e-4
+    for n, cells in ((1, 4), (2, 16), (150, 8)):
+        ctxs[1].set_option(G.OPT_SH_CELLS, cells)
+        for c in ctxs:
+            c.step(n)
+        for i in (0, 1):
+            a, b = ctxs[0].get_state(i), ctxs[1].get_state(i)
+            for k in ("v", "eta"):
+                assert rel_l2(b[k], a[k]) <= tol, (n, i, k, rel_l2(b[k], a[k]))
