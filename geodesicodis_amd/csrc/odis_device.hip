@@ -507,55 +507,58 @@ __global__ __launch_bounds__(kBlock) void k_sh_analysis(int K, int N, const doub
   if (threadIdx.x == 0) b_part[(size_t(b) * kShChunks + q) * K + k] = t;
 }
 
-// b_j = sum_q b_part[q][j] over the n_part analysis partials, in a fixed
-// order; written after the partials of every instance ([B][n_part][K]).
-// 64 j per workgroup; wave w adds the partials q = w mod 4 with four loads
-// in flight, then the four wave sums are added.
-__global__ __launch_bounds__(kBlock) void k_sh_bsum(int K, int n_part, double *b_part) {
-  __shared__ double part[kBlock / 64][64];
-  const int b = blockIdx.y, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int j = blockIdx.x * 64 + lane;
-  double t = 0.0;
-  if (j < K) {
-    const double *p = b_part + size_t(b) * n_part * K + j;
-    int q = w;
-    for (; q + 12 < n_part; q += 16) {
-      const double a0 = p[size_t(q) * K], a1 = p[size_t(q + 4) * K], a2 = p[size_t(q + 8) * K],
-                   a3 = p[size_t(q + 12) * K];
-      t += a0;
-      t += a1;
-      t += a2;
-      t += a3;
-    }
-    for (; q < n_part; q += 4) t += p[size_t(q) * K];
-  }
-  part[w][lane] = t;
-  __syncthreads();
-  if (w == 0 && j < K)
-    b_part[(size_t(gridDim.y) * n_part + b) * K + j] = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
-}
-
 // Store c_k (k = l l + l + m) at its m-major place: for m = 0..L, l = m..L
 // the pair (c_{l,|m|}, c_{l,-|m|}) at 2 (m (L + 1) - m (m - 1) / 2 + l - m);
-// the unused sine slot of m = 0 is zero.
-// chain (nullable): the pair stores S_l c instead, S_l the chain scale.
+// the unused sine slot of m = 0 is zero.  chain (nullable): the degree
+// table of the chain synthesis instead, 4 doubles per (l, m) at
+// 4 (m (L + 1) - m (m - 1) / 2 + l - m): (beta_l, S_l c_{l,m}, S_l c_{l,-m}, 0),
+// S_l the chain scale; beta and the padding are set with the basis.
 __device__ __forceinline__ void put_mmajor(int K, int b, int k, double ck, const double *chain, double *cmaj) {
   int l = 0;
   while ((l + 1) * (l + 1) <= k) ++l;
   const int mm = k - l * l - l, m = mm < 0 ? -mm : mm;
   int L = 0;
   while ((L + 1) * (L + 1) < K) ++L;
-  double *cb = cmaj + size_t(b) * (L + 1) * (L + 2);
   const int at = m * (L + 1) - m * (m - 1) / 2 + l - m;
-  if (m == 0) cb[2 * l + 1] = 0.0;
-  cb[2 * at + (mm < 0 ? 1 : 0)] = chain ? chain[2 * at + 1] * ck : ck;
+  if (chain) {
+    double *cb = cmaj + size_t(b) * 2 * (L + 1) * (L + 2);
+    if (m == 0) cb[4 * at + 2] = 0.0;
+    cb[4 * at + 1 + (mm < 0 ? 1 : 0)] = chain[2 * at + 1] * ck;
+  } else {
+    double *cb = cmaj + size_t(b) * (L + 1) * (L + 2);
+    if (m == 0) cb[2 * l + 1] = 0.0;
+    cb[2 * at + (mm < 0 ? 1 : 0)] = ck;
+  }
 }
 
-// c_k = D_k sum_j Minv[k][j] b_j: b in LDS, one wavefront per k.  cmaj
-// (nullable) also receives c in m-major order: for m = 0..L, l = m..L the
-// pair (c_{l,m}, c_{l,-m}) at 2 (m (L + 1) - m (m - 1) / 2 + l - m).
-__global__ __launch_bounds__(kBlock) void k_sh_coef(int K, const double *Minv, const double *D, const double *bsum,
-                                                    double *c, const double *chain, double *cmaj) {
+// b_j = sum_q b_part[q][j] over the n_part analysis partials of instance b
+// ([B][n_part][K]), added in q order; written after the partials of every
+// instance.  One lane per j; the loads of 16 partials are issued before
+// they are added (the partials come from other XCDs' L2, so latency, not
+// bandwidth, sets the time).
+__global__ __launch_bounds__(kBlock) void k_sh_bsum(int K, int n_part, double *b_part) {
+  const int b = blockIdx.y;
+  const int j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= K) return;
+  const double *p = b_part + size_t(b) * n_part * K + j;
+  double t = 0.0;
+  for (int q0 = 0; q0 < n_part; q0 += 16) {
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = q0 + u < n_part ? p[size_t(q0 + u) * K] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+      if (q0 + u < n_part) t += v[u];
+  }
+  b_part[(size_t(gridDim.y) * n_part + b) * K + j] = t;
+}
+
+// c_k = D_k sum_j Minv[k][j] b_j: b in LDS, one wavefront per k, eight row
+// loads in flight per lane, a fixed-order wave sum.  cmaj (nullable) also
+// receives c in m-major order (put_mmajor).
+__global__ __launch_bounds__(kBlock) void k_sh_coef(int K, const double *__restrict__ Minv, const double *D,
+                                                    const double *__restrict__ bsum, double *c, const double *chain,
+                                                    double *cmaj) {
   extern __shared__ double bs[];
   const int b = blockIdx.y;
   for (int j = threadIdx.x; j < K; j += kBlock) bs[j] = bsum[size_t(b) * K + j];
@@ -565,7 +568,14 @@ __global__ __launch_bounds__(kBlock) void k_sh_coef(int K, const double *Minv, c
   if (k >= K) return;
   const double *row = Minv + size_t(k) * K;
   double acc = 0.0;
-  for (int j = lane; j < K; j += 64) acc += row[j] * bs[j];
+  for (int j0 = lane; j0 < K; j0 += 8 * 64) {
+    double r[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) r[u] = j0 + 64 * u < K ? row[j0 + 64 * u] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (j0 + 64 * u < K) acc += r[u] * bs[j0 + 64 * u];
+  }
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
   if (lane != 0) return;
   const double ck = (D ? D[size_t(b) * K + k] : 1.0) * acc;
@@ -856,27 +866,42 @@ __device__ __forceinline__ void pair_powers(T zc, T zs, T zr, T &ac, T &as, T &a
   }
 }
 
-// One chain of NK degrees starting at table entry OFF, sums into acc[2 T0 ..]
+// One chain of NK degrees starting at table entry OFF for two cells (a, b)
+// at once (two independent recurrences), sums into acc[2 T0 ..]
 template <typename T, int T0, int NK, int OFF, int NV>
-__device__ __forceinline__ void run_chain(const double *__restrict__ chain, T mu, T pmm, T xc, T xs, T (&acc)[NV]) {
-  T q1 = T(0), q2 = pmm;
+__device__ __forceinline__ void run_chain2(const double *__restrict__ chain, T mua, T mub, T pa, T pb, T xca, T xsa,
+                                          T xcb, T xsb, T (&acc)[NV]) {
+  T qa1 = T(0), qa2 = pa, qb1 = T(0), qb2 = pb;
 #pragma unroll
   for (int k = 0; k < NK; ++k) {
-    T q;
-    if (k == 0)
-      q = pmm;
-    else if (k == 1)
-      q = mu * q1;
-    else
-      q = fma(mu, q1, T(chain[2 * (OFF + k)]) * q2);
-    q2 = q1;
-    q1 = q;
-    acc[2 * (T0 + k)] = fma(q, xc, acc[2 * (T0 + k)]);
-    acc[2 * (T0 + k) + 1] = fma(q, xs, acc[2 * (T0 + k) + 1]);
+    T qa, qb;
+    if (k == 0) {
+      qa = pa;
+      qb = pb;
+    } else if (k == 1) {
+      qa = mua * qa1;
+      qb = mub * qb1;
+    } else {
+      const T be = T(chain[2 * (OFF + k)]);
+      qa = fma(mua, qa1, be * qa2);
+      qb = fma(mub, qb1, be * qb2);
+    }
+    qa2 = qa1;
+    qa1 = qa;
+    qb2 = qb1;
+    qb1 = qb;
+    acc[2 * (T0 + k)] = fma(qb, xcb, fma(qa, xca, acc[2 * (T0 + k)]));
+    acc[2 * (T0 + k) + 1] = fma(qb, xsb, fma(qa, xsa, acc[2 * (T0 + k) + 1]));
   }
 }
 
-// R cells per lane for the pair (P, LT - P), all shapes fixed at compile time
+// operands of one cell (cells past N read cell N - 1 with x = 0)
+struct PairCell {
+  double mu, s, zc, zs;
+};
+
+// R cells per lane (R even) for the pair (P, LT - P), all shapes fixed at
+// compile time; two cells per step, the next two loaded meanwhile
 template <typename T, int LT, int P>
 __device__ __forceinline__ void pair_cells(int N, const double *__restrict__ chain, const double *__restrict__ trig,
                                            const T *__restrict__ xb, int i0, int span, int R,
@@ -885,27 +910,34 @@ __device__ __forceinline__ void pair_cells(int N, const double *__restrict__ cha
   constexpr int OFFA = M0 * (L + 1) - M0 * (M0 - 1) / 2, OFFB = M1 * (L + 1) - M1 * (M1 - 1) / 2;
   const double *cm_tab = chain + (L + 1) * (L + 2);
   const T CA = T(cm_tab[M0]), CB = T(cm_tab[M1]);
-  // operands of the next cell are loaded while this one runs; cells past N
-  // read cell N - 1 with x = 0
-  auto load = [&](int r, double &mu, double &s, double &zc, double &zs, T &xv) {
+  auto load = [&](int r, PairCell &c, T &xv) {
     const int i = i0 + r * span, ii = min(i, N - 1);
-    mu = trig[ii];
-    s = trig[N + ii];
-    zc = trig[2 * N + ii];
-    zs = trig[3 * N + ii];
+    c.mu = trig[ii];
+    c.s = trig[N + ii];
+    c.zc = trig[2 * N + ii];
+    c.zs = trig[3 * N + ii];
     xv = xb[ii];
     if (i >= N) xv = T(0);
   };
-  double nmu, ns, nzc, nzs;
-  T nxv;
-  load(0, nmu, ns, nzc, nzs, nxv);
-  for (int r = 0; r < R; ++r) {
-    const T mu = T(nmu), s = T(ns), zc = T(nzc), zs = T(nzs), xv = nxv;
-    if (r + 1 < R) load(r + 1, nmu, ns, nzc, nzs, nxv);
-    T ac, as, ar, bc, bs, br;
-    pair_powers<T, M0, M1>(zc, zs, s, ac, as, ar, bc, bs, br);
-    run_chain<T, 0, NA, OFFA>(chain, mu, CA * ar, xv * ac, xv * as, acc);
-    if constexpr (NB > 0) run_chain<T, NA, NB, OFFB>(chain, mu, CB * br, xv * bc, xv * bs, acc);
+  PairCell na, nb;
+  T nxa, nxb;
+  load(0, na, nxa);
+  load(1, nb, nxb);
+  for (int r = 0; r < R; r += 2) {
+    const PairCell ca = na, cb = nb;
+    const T xa = nxa, xbv = nxb;
+    if (r + 2 < R) {
+      load(r + 2, na, nxa);
+      load(r + 3, nb, nxb);
+    }
+    T aac, aas, aar, abc, abs_, abr, bac, bas, bar, bbc, bbs, bbr;
+    pair_powers<T, M0, M1>(T(ca.zc), T(ca.zs), T(ca.s), aac, aas, aar, abc, abs_, abr);
+    pair_powers<T, M0, M1>(T(cb.zc), T(cb.zs), T(cb.s), bac, bas, bar, bbc, bbs, bbr);
+    const T mua = T(ca.mu), mub = T(cb.mu);
+    run_chain2<T, 0, NA, OFFA>(chain, mua, mub, CA * aar, CA * bar, xa * aac, xa * aas, xbv * bac, xbv * bas, acc);
+    if constexpr (NB > 0)
+      run_chain2<T, NA, NB, OFFB>(chain, mua, mub, CB * abr, CB * bbr, xa * abc, xa * abs_, xbv * bbc, xbv * bbs,
+                                  acc);
   }
 }
 
@@ -1068,72 +1100,106 @@ __global__ __launch_bounds__(kBlock) void k_sh_synth_rec(int Lrt, int N, const d
 }
 
 // Synthesis with the rescaled recurrence, for a degree L fixed at compile
-// time: 64 cells x 4 waves per workgroup, wave g summing the contiguous
-// m-range of synth_split (cos / sin (m lon) by rotation, P_mm = P_{m-1,m-1}
-// sect_m cos(lat)).  Each m sums its degrees against the scaled
-// coefficients S_l c_lm (cmaj written with the chain scales), four
-// operations per degree:
+// time: 128 cells x 4 waves per workgroup, each lane two cells (i, i + 64),
+// wave g summing the contiguous m-range of synth_split (cos / sin (m lon)
+// by rotation, P_mm = P_{m-1,m-1} sect_m cos(lat)).  Each m sums its
+// degrees against the degree table (beta_l, S_l c_{l,m}, S_l c_{l,-m}, 0)
+// written by put_mmajor, four operations per degree and cell:
 //     q_l = mu q_{l-1} + beta_l q_{l-2},  A += (S c)_{l,m} q_l,  B += (S c)_{l,-m} q_l,
 // and y = sum_m A_m cos(m lon) + B_m sin(m lon); the four waves' sums are
 // added through LDS in a fixed order.
+template <typename T>
+struct SynthCell {
+  T mu, s, cl, sl, cm, sm, pmm, y;
+};
+
+template <typename T>
+__device__ __forceinline__ void synth_next_m(SynthCell<T> &c, double sect) {
+  const T c2 = fma(c.cm, c.cl, -(c.sm * c.sl));
+  c.sm = fma(c.sm, c.cl, c.cm * c.sl);
+  c.cm = c2;
+  c.pmm = c.pmm * (T(sect) * c.s);
+}
+
 template <typename T, int LT>
 __global__ __launch_bounds__(kBlock) void k_sh_synth_chain(int N, const double *__restrict__ chain,
                                                            const double *__restrict__ tabm,
                                                            const double *__restrict__ trig,
                                                            const double *__restrict__ cmaj, const T *base,
                                                            long long stride, T *out, int4 mr4) {
-  __shared__ double part[kBlock];
+  __shared__ double part[2][kBlock];
   constexpr int L = LT;
   const int b = blockIdx.y;
-  const double *cb = cmaj + size_t(b) * (L + 1) * (L + 2);
+  const double *cb = cmaj + size_t(b) * 2 * (L + 1) * (L + 2);
   const int lane = threadIdx.x & 63;
   const int g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int i = blockIdx.x * 64 + lane;
-  const bool ok = i < N;
+  const int i0 = blockIdx.x * 128 + lane;
   const int mlo = g == 0 ? 0 : (g == 1 ? mr4.x : (g == 2 ? mr4.y : mr4.z));
   const int mhi = g == 0 ? mr4.x : (g == 1 ? mr4.y : (g == 2 ? mr4.z : mr4.w));
-  const int ii = ok ? i : 0;
-  const T mu = T(trig[ii]), s = T(trig[N + ii]), cl = T(trig[2 * N + ii]), sl = T(trig[3 * N + ii]);
-  T y = T(0), cm = T(1), sm = T(0), pmm = T(1);
+  SynthCell<T> u[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int ii = min(i0 + 64 * h, N - 1);
+    u[h].mu = T(trig[ii]);
+    u[h].s = T(trig[N + ii]);
+    u[h].cl = T(trig[2 * N + ii]);
+    u[h].sl = T(trig[3 * N + ii]);
+    u[h].cm = T(1);
+    u[h].sm = T(0);
+    u[h].pmm = T(1);
+    u[h].y = T(0);
+  }
   for (int q = 1; q < mlo; ++q) {  // advance to m = mlo - 1
-    const T c2 = fma(cm, cl, -(sm * sl));
-    sm = fma(sm, cl, cm * sl);
-    cm = c2;
-    pmm = pmm * (T(tabm[q]) * s);
+    synth_next_m(u[0], tabm[q]);
+    synth_next_m(u[1], tabm[q]);
   }
 #pragma unroll 1
   for (int m = mlo; m < mhi; ++m) {
     if (m > 0) {
-      const T c2 = fma(cm, cl, -(sm * sl));
-      sm = fma(sm, cl, cm * sl);
-      cm = c2;
-      pmm = pmm * (T(tabm[m]) * s);
+      synth_next_m(u[0], tabm[m]);
+      synth_next_m(u[1], tabm[m]);
     }
     const int off = m * (L + 1) - m * (m - 1) / 2, nk = L - m + 1;
-    const double *ch = chain + 2 * off, *cc = cb + 2 * off;
+    const double *e = cb + 4 * off;  // (beta, S c, S s, 0) per degree
     // degrees m and m + 1, then the recurrence (the sine slots of m = 0 are zero)
-    T A = T(cc[0]) * pmm, B = T(cc[1]) * pmm;
-    T q2 = pmm, q1 = pmm;
+    const T c0 = T(e[1]), s0 = T(e[2]);
+    T A0 = c0 * u[0].pmm, B0 = s0 * u[0].pmm, A1 = c0 * u[1].pmm, B1 = s0 * u[1].pmm;
+    T p2 = u[0].pmm, p1 = u[0].pmm, r2 = u[1].pmm, r1 = u[1].pmm;
     if (nk > 1) {
-      q1 = mu * pmm;
-      A = fma(T(cc[2]), q1, A);
-      B = fma(T(cc[3]), q1, B);
+      const T c1 = T(e[5]), s1 = T(e[6]);
+      p1 = u[0].mu * p2;
+      r1 = u[1].mu * r2;
+      A0 = fma(c1, p1, A0);
+      B0 = fma(s1, p1, B0);
+      A1 = fma(c1, r1, A1);
+      B1 = fma(s1, r1, B1);
     }
+    e += 8;
 #pragma unroll 4
-    for (int k = 2; k < nk; ++k) {
-      const T q = fma(mu, q1, T(ch[2 * k]) * q2);
-      q2 = q1;
-      q1 = q;
-      A = fma(T(cc[2 * k]), q, A);
-      B = fma(T(cc[2 * k + 1]), q, B);
+    for (int k = 2; k < nk; ++k, e += 4) {
+      const T be = T(e[0]), cc = T(e[1]), ss = T(e[2]);
+      const T p = fma(u[0].mu, p1, be * p2);
+      const T r = fma(u[1].mu, r1, be * r2);
+      p2 = p1;
+      p1 = p;
+      r2 = r1;
+      r1 = r;
+      A0 = fma(cc, p, A0);
+      B0 = fma(ss, p, B0);
+      A1 = fma(cc, r, A1);
+      B1 = fma(ss, r, B1);
     }
-    y = fma(A, cm, y);
-    y = fma(B, sm, y);
+    u[0].y = fma(B0, u[0].sm, fma(A0, u[0].cm, u[0].y));
+    u[1].y = fma(B1, u[1].sm, fma(A1, u[1].cm, u[1].y));
   }
-  part[threadIdx.x] = double(y);
+  part[0][threadIdx.x] = double(u[0].y);
+  part[1][threadIdx.x] = double(u[1].y);
   __syncthreads();
-  if (threadIdx.x >= 64 || !ok) return;
-  const double sum = (part[lane] + part[64 + lane]) + (part[128 + lane] + part[192 + lane]);
+  if (threadIdx.x >= 128) return;
+  const int h = threadIdx.x >> 6, i = i0 + 64 * h;
+  if (i >= N) return;
+  const double *pp = part[h];
+  const double sum = (pp[lane] + pp[64 + lane]) + (pp[128 + lane] + pp[192 + lane]);
   const T add = base ? base[b * stride + i] : T(0);
   out[b * stride + i] = add + T(sum);
 }
@@ -1304,7 +1370,7 @@ struct odis_ctx {
   double *sh_Yt = nullptr, *sh_Minv = nullptr, *sh_D = nullptr, *sh_b = nullptr, *sh_c = nullptr;
   unsigned int *sh_Yp = nullptr;
   double *sh_tab = nullptr, *sh_trig = nullptr;  // recurrence table, [4][N] sin/cos lat, cos/sin lon
-  double *sh_tabm = nullptr, *sh_cmaj = nullptr; // m-major table, m-major coefficients [B][(L+1)(L+2)]
+  double *sh_tabm = nullptr, *sh_cmaj = nullptr; // m-major table, m-major coefficients [B][2 (L+1)(L+2)]
   double *sh_chain = nullptr;                    // rescaled recurrence (beta, S) per (l, m), then C_m
   int sh_part = 0;                               // capacity of analysis partials per instance
   int sh_cpt = 0;                                // cells per lane of the on-the-fly analysis (0: auto)
@@ -2111,8 +2177,8 @@ static int sh_apply(odis_ctx *ctx, const T *x, long long stride, int nb, const d
   if (ana) {
     // partial sums, then c = D Minv b; events around both
     const double *bsum = ctx->sh_b + size_t(nb) * n_part * K;
-    hipExtLaunchKernelGGL(k_sh_bsum, dim3((K + 63) / 64, nb), blk, 0, ctx->stream, ev ? ev[2] : nullptr,
-                          nullptr, 0, K, n_part, ctx->sh_b);
+    hipExtLaunchKernelGGL(k_sh_bsum, dim3((K + kBlock - 1) / kBlock, nb), blk, 0, ctx->stream,
+                          ev ? ev[2] : nullptr, nullptr, 0, K, n_part, ctx->sh_b);
     hipExtLaunchKernelGGL(k_sh_coef, dim3((K + 3) / 4, nb), blk, sizeof(double) * K, ctx->stream, nullptr,
                           ev ? ev[3] : nullptr, 0, K, ctx->sh_Minv, D, bsum, c, chain,
                           ctx->sh_mode == ODIS_SH_RECUR ? ctx->sh_cmaj : static_cast<double *>(nullptr));
@@ -2121,17 +2187,17 @@ static int sh_apply(odis_ctx *ctx, const T *x, long long stride, int nb, const d
     if (!ana)  // coefficients from the caller: their m-major copy first
       hipLaunchKernelGGL(k_sh_to_mmajor, dim3((K + kBlock - 1) / kBlock, nb), blk, 0, ctx->stream, K, c, chain,
                          ctx->sh_cmaj);
-    const dim3 gs((N + 63) / 64, nb);
+    const dim3 gs((N + 63) / 64, nb), gs2((N + 127) / 128, nb);
     const int4 mr = synth_split(L);
     hipEvent_t e4 = ev ? ev[4] : nullptr, e5 = ev ? ev[5] : nullptr;
     if (L == 32)
-      hipExtLaunchKernelGGL((k_sh_synth_chain<T, 32>), gs, blk, 0, ctx->stream, e4, e5, 0, N, ctx->sh_chain,
+      hipExtLaunchKernelGGL((k_sh_synth_chain<T, 32>), gs2, blk, 0, ctx->stream, e4, e5, 0, N, ctx->sh_chain,
                             ctx->sh_tabm, ctx->sh_trig, ctx->sh_cmaj, base, stride, out, mr);
     else if (L == 16)
-      hipExtLaunchKernelGGL((k_sh_synth_chain<T, 16>), gs, blk, 0, ctx->stream, e4, e5, 0, N, ctx->sh_chain,
+      hipExtLaunchKernelGGL((k_sh_synth_chain<T, 16>), gs2, blk, 0, ctx->stream, e4, e5, 0, N, ctx->sh_chain,
                             ctx->sh_tabm, ctx->sh_trig, ctx->sh_cmaj, base, stride, out, mr);
     else if (L == 8)
-      hipExtLaunchKernelGGL((k_sh_synth_chain<T, 8>), gs, blk, 0, ctx->stream, e4, e5, 0, N, ctx->sh_chain,
+      hipExtLaunchKernelGGL((k_sh_synth_chain<T, 8>), gs2, blk, 0, ctx->stream, e4, e5, 0, N, ctx->sh_chain,
                             ctx->sh_tabm, ctx->sh_trig, ctx->sh_cmaj, base, stride, out, mr);
     else
       hipExtLaunchKernelGGL((k_sh_synth_rec<T, 0>), gs, blk, 0, ctx->stream, e4, e5, 0, L, N, ctx->sh_tabm,
@@ -2889,9 +2955,17 @@ extern "C" int odis_sh_set_basis(odis_ctx *ctx, int l_max, const double *beta, i
     }
     if ((rc = alloc(reinterpret_cast<void **>(&ctx->sh_chain), chain.size() * sizeof(double)))) return rc;
     CK(hipMemcpy(ctx->sh_chain, chain.data(), chain.size() * sizeof(double), hipMemcpyHostToDevice));
-    if ((rc = alloc(reinterpret_cast<void **>(&ctx->sh_cmaj), size_t(ctx->B) * (l_max + 1) * (l_max + 2) *
-                                                                   sizeof(double))))
+    // coefficients: pairs (runtime degrees) or the 4-double degree table of
+    // the chain synthesis, with beta and the padding set here
+    const size_t nlm = size_t(l_max + 1) * (l_max + 2) / 2;
+    if ((rc = alloc(reinterpret_cast<void **>(&ctx->sh_cmaj), size_t(ctx->B) * 4 * nlm * sizeof(double))))
       return rc;
+    {
+      std::vector<double> ct(size_t(ctx->B) * 4 * nlm, 0.0);
+      for (int b = 0; b < ctx->B; b++)
+        for (size_t q = 0; q < nlm; q++) ct[(size_t(b) * nlm + q) * 4] = chain[2 * q];
+      CK(hipMemcpy(ctx->sh_cmaj, ct.data(), ct.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     if ((rc = alloc(reinterpret_cast<void **>(&ctx->sh_trig), trig.size() * sizeof(double)))) return rc;
     CK(hipMemcpy(ctx->sh_trig, trig.data(), trig.size() * sizeof(double), hipMemcpyHostToDevice));
     // the most partials: 4 cells per lane
