@@ -295,7 +295,7 @@ def main():
     state = ctx.get_state()
     print("bench: headline %.3e cell-steps/s (%.1f us/step)" % (value, 1e3 * ms_per_step),
           file=sys.stderr, flush=True)
-    one = ctx.option(G.OPT_ONE_KERNEL) == 1 and not shell
+    one = ctx.option(G.OPT_ONE_KERNEL) == 1
     tname = ("double" if prec == G.FP64 else "float") + (",compact" if ctx.option(G.OPT_COMPACT) else ",tabulated")
     kname, kbytes, kms, stencil_alg = stencil_roofline(G, ctx, mesh, tm, p_bytes, one)
     kernels = {"stencil": {"kernel": "%s<%s>" % (kname, tname), "ms": kms, "alg_bytes": kbytes}}
@@ -361,11 +361,10 @@ def main():
             "sh_basis": (("on the fly (Legendre recurrence, %s)" % ("fp64" if prec == G.FP64 else "fp32")) if recur
                          else ("stored fp64" if prec == G.FP64 else "stored split-bf16")) if shell else None,
             "trisk": "cell-local (kite fractions)" if ctx.option(G.OPT_COMPACT) else "tabulated",
-            "launch": (("one kernel per step (patches of <= %d cells, %d lanes per cell, %d workgroups)"
+            "launch": ("SH analysis, partial sums, coefficients, synthesis + " if shell else "") +
+                      (("one stencil kernel per step (patches of <= %d cells, %d lanes per cell, %d workgroups)"
                         % (ctx.option(G.OPT_PATCH_CELLS), ctx.option(G.OPT_STEP_LANES),
-                           ctx.option(G.OPT_STEP_BLOCKS))) if one else
-                       ("SH analysis, coefficients, synthesis + face + cell kernel per step" if shell else
-                        "face + cell kernel per step")),
+                           ctx.option(G.OPT_STEP_BLOCKS))) if one else "face + cell kernel per step"),
             "graphs": bool(args.graphs),
             "kernel_ms": {k: r["ms"] for k, r in kernels.items()},
             "step_events_ms": tm["step_ms"],

@@ -533,7 +533,7 @@ __device__ __forceinline__ void put_mmajor(int K, int b, int k, double ck, const
 
 // b_j = sum_q b_part[q][j] over the n_part analysis partials of instance b
 // ([B][n_part][K]), added in q order; written after the partials of every
-// instance.  One lane per j; the loads of 16 partials are issued before
+// instance.  One lane per j; the loads of 32 partials are issued before
 // they are added (the partials come from other XCDs' L2, so latency, not
 // bandwidth, sets the time).
 __global__ __launch_bounds__(kBlock) void k_sh_bsum(int K, int n_part, double *b_part) {
@@ -542,18 +542,18 @@ __global__ __launch_bounds__(kBlock) void k_sh_bsum(int K, int n_part, double *b
   if (j >= K) return;
   const double *p = b_part + size_t(b) * n_part * K + j;
   double t = 0.0;
-  for (int q0 = 0; q0 < n_part; q0 += 16) {
-    double v[16];
+  for (int q0 = 0; q0 < n_part; q0 += 32) {
+    double v[32];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) v[u] = q0 + u < n_part ? p[size_t(q0 + u) * K] : 0.0;
+    for (int u = 0; u < 32; ++u) v[u] = q0 + u < n_part ? p[size_t(q0 + u) * K] : 0.0;
 #pragma unroll
-    for (int u = 0; u < 16; ++u)
+    for (int u = 0; u < 32; ++u)
       if (q0 + u < n_part) t += v[u];
   }
   b_part[(size_t(gridDim.y) * n_part + b) * K + j] = t;
 }
 
-// c_k = D_k sum_j Minv[k][j] b_j: b in LDS, one wavefront per k, eight row
+// c_k = D_k sum_j Minv[k][j] b_j: b in LDS, one wavefront per k, 32 row
 // loads in flight per lane, a fixed-order wave sum.  cmaj (nullable) also
 // receives c in m-major order (put_mmajor).
 __global__ __launch_bounds__(kBlock) void k_sh_coef(int K, const double *__restrict__ Minv, const double *D,
@@ -568,12 +568,12 @@ __global__ __launch_bounds__(kBlock) void k_sh_coef(int K, const double *__restr
   if (k >= K) return;
   const double *row = Minv + size_t(k) * K;
   double acc = 0.0;
-  for (int j0 = lane; j0 < K; j0 += 8 * 64) {
-    double r[8];
+  for (int j0 = lane; j0 < K; j0 += 32 * 64) {
+    double r[32];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) r[u] = j0 + 64 * u < K ? row[j0 + 64 * u] : 0.0;
+    for (int u = 0; u < 32; ++u) r[u] = j0 + 64 * u < K ? row[j0 + 64 * u] : 0.0;
 #pragma unroll
-    for (int u = 0; u < 8; ++u)
+    for (int u = 0; u < 32; ++u)
       if (j0 + 64 * u < K) acc += r[u] * bs[j0 + 64 * u];
   }
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
@@ -1331,7 +1331,7 @@ struct odis_ctx {
   int ts_valid = 1;            // Ops::ts holds the partial sums of v[cur]
   std::string compact_why;     // why compact_ok is 0
   int one_kernel_ok = 0;       // patches for the one-kernel step are built
-  int one_kernel = 0;          // use it (ODIS_OPT_ONE_KERNEL; compact form, SH off)
+  int one_kernel = 0;          // use it (ODIS_OPT_ONE_KERNEL; compact form)
   int n_patch = 0;
   int patch = 64;              // cells per patch (ODIS_OPT_PATCH_CELLS: 64, 128, 256)
   int step_blocks = 0;         // workgroups of the one-kernel step (ODIS_OPT_STEP_BLOCKS; 0: one per patch)
@@ -2114,6 +2114,28 @@ static void launch_analysis_rec(odis_ctx *ctx, int nb, int L, const T *x, long l
 // Degrees with a compiled pair analysis (the others use the per-degree one).
 static bool pair_degree(int L) { return L == 8 || L == 16 || L == 32; }
 
+// Cells per lane of the pair analysis when none is set: the grid
+// (ceil(N / (64 kPairWaves R)) cell blocks x pairs x instances) fits the
+// workgroups the device holds at once (one round, no tail), R even and >= 4.
+template <typename T>
+static int pair_auto_cells(odis_ctx *ctx, int L, int nb) {
+  int per_cu = 0, n_cu = 0, dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev);
+  const void *k = L == 32 ? reinterpret_cast<const void *>(&k_sh_analysis_pair<T, 32>)
+                          : (L == 16 ? reinterpret_cast<const void *>(&k_sh_analysis_pair<T, 16>)
+                                     : reinterpret_cast<const void *>(&k_sh_analysis_pair<T, 8>));
+  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 64 * kPairWaves, 0);
+  if (e != hipSuccess || per_cu <= 0 || n_cu <= 0) return 16;
+  const int n_pairs = L / 2 + 1;
+  const long long slots = (long long)per_cu * n_cu;
+  const long long n_cblk = std::max(1LL, slots / ((long long)n_pairs * nb));
+  const long long span = 64LL * kPairWaves * n_cblk;
+  int R = int((ctx->N + span - 1) / span);
+  R += R & 1;
+  return std::max(4, R);
+}
+
 template <typename T, int LT>
 static void launch_analysis_pair(odis_ctx *ctx, int nb, const T *x, long long stride, int cpt, int n_cblk,
                                  hipEvent_t e0, hipEvent_t e1) {
@@ -2136,8 +2158,9 @@ static int sh_apply(odis_ctx *ctx, const T *x, long long stride, int nb, const d
   const bool split = ctx->sh_mode == ODIS_SH_SPLIT_BF16;
   // on the fly: chunks of 64 CPT cells, partials per workgroup of kShWaves chunks
   const bool by_pair = ctx->sh_mode == ODIS_SH_RECUR && pair_degree(L);
-  // cells per lane: 16 for the pair analysis, 8 for the per-degree one
-  const int cpt = ctx->sh_cpt ? ctx->sh_cpt : (by_pair ? 16 : 8);
+  // cells per lane: the option, else for the pair analysis the fewest that
+  // fit its grid in one round of resident workgroups, 8 for the per-degree one
+  const int cpt = ctx->sh_cpt ? ctx->sh_cpt : (by_pair ? pair_auto_cells<T>(ctx, L, nb) : 8);
   const int n_chunks = (N + 64 * cpt - 1) / (64 * cpt);
   // cmaj holds S_l c for the compiled-degree synthesis
   const double *chain = by_pair ? ctx->sh_chain : nullptr;
@@ -2336,29 +2359,9 @@ static int launch_chunk(odis_ctx *ctx, Ops<T> &o, const double *table, int count
     const int rc = reserve_events(ctx, *ev_used + 10 * sampled);
     if (rc) return rc;
   }
-  if (ctx->compact && ctx->one_kernel && ctx->sh_mode == ODIS_SH_OFF) {
-    if (step_blocks(ctx) > ctx->part_cap) return fail(ctx, ODIS_E_INVALID, "more workgroups than dissipation partials");
-    for (int s = 0; s < count; s++) {
-      StepArgs<T> a = step_args(ctx, o);
-      hipEvent_t *ev = nullptr;
-      if (evs && (n0 + s) % ctx->timing == 0) {
-        evs->mom.push_back(int(*ev_used));
-        ev = &ctx->ev[*ev_used];
-        *ev_used += 2;
-      }
-      if (ctx->patch == 64)
-        launch_step_lanes<T, 64>(ctx, a, table, s, ev);
-      else if (ctx->patch == 256)
-        launch_step_lanes<T, 256>(ctx, a, table, s, ev);
-      else
-        launch_step_lanes<T, 128>(ctx, a, table, s, ev);
-      ctx->cur ^= 1;
-    }
-    hipLaunchKernelGGL(k_series, dim3(ctx->B), dim3(kBlock), 0, ctx->stream, ctx->partials, step_blocks(ctx),
-                       ctx->dconst, ctx->series, ctx->nonfinite, ctx->cap, table, count - 1);
-    CK(hipGetLastError());
-    return ODIS_OK;
-  }
+  const bool one = ctx->compact && ctx->one_kernel;
+  if (one && step_blocks(ctx) > ctx->part_cap)
+    return fail(ctx, ODIS_E_INVALID, "more workgroups than dissipation partials");
   for (int s = 0; s < count; s++) {
     const bool stamp = evs && (n0 + s) % ctx->timing == 0;
     if (ctx->sh_mode != ODIS_SH_OFF) {
@@ -2371,22 +2374,44 @@ static int launch_chunk(odis_ctx *ctx, Ops<T> &o, const double *table, int count
       const int rc = sh_effective_height(ctx, o, evsh);
       if (rc) return rc;
     }
-    FaceArgs<T> fa = face_args(ctx, o);
-    CellArgs<T> ca = cell_args(ctx, o);
-    hipEvent_t *evf = nullptr, *evc = nullptr;
-    if (stamp) {
-      evs->mom.push_back(int(*ev_used));
-      evs->cont.push_back(int(*ev_used + 2));
-      evf = &ctx->ev[*ev_used];
-      evc = &ctx->ev[*ev_used + 2];
-      *ev_used += 4;
+    if (one) {
+      // the face phase reads eta_eff (face_args) when a basis is set
+      StepArgs<T> a = step_args(ctx, o);
+      hipEvent_t *ev = nullptr;
+      if (stamp) {
+        evs->mom.push_back(int(*ev_used));
+        ev = &ctx->ev[*ev_used];
+        *ev_used += 2;
+      }
+      if (ctx->patch == 64)
+        launch_step_lanes<T, 64>(ctx, a, table, s, ev);
+      else if (ctx->patch == 256)
+        launch_step_lanes<T, 256>(ctx, a, table, s, ev);
+      else
+        launch_step_lanes<T, 128>(ctx, a, table, s, ev);
+    } else {
+      FaceArgs<T> fa = face_args(ctx, o);
+      CellArgs<T> ca = cell_args(ctx, o);
+      hipEvent_t *evf = nullptr, *evc = nullptr;
+      if (stamp) {
+        evs->mom.push_back(int(*ev_used));
+        evs->cont.push_back(int(*ev_used + 2));
+        evf = &ctx->ev[*ev_used];
+        evc = &ctx->ev[*ev_used + 2];
+        *ev_used += 4;
+      }
+      if (ctx->compact)
+        launch_pair<T, true>(ctx, fa, ca, table, s, evf, evc);
+      else
+        launch_pair<T, false>(ctx, fa, ca, table, s, evf, evc);
     }
-    if (ctx->compact)
-      launch_pair<T, true>(ctx, fa, ca, table, s, evf, evc);
-    else
-      launch_pair<T, false>(ctx, fa, ca, table, s, evf, evc);
     CK(hipGetLastError());
     ctx->cur ^= 1;
+  }
+  if (one) {
+    hipLaunchKernelGGL(k_series, dim3(ctx->B), dim3(kBlock), 0, ctx->stream, ctx->partials, step_blocks(ctx),
+                       ctx->dconst, ctx->series, ctx->nonfinite, ctx->cap, table, count - 1);
+    CK(hipGetLastError());
   }
   return ODIS_OK;
 }
@@ -2500,13 +2525,14 @@ static int step_t(odis_ctx *ctx, Ops<T> &o, int n_steps) {
         nsh++;
       }
     for (int q = 0; q < 3; q++) ctx->t_sh[q] = nsh ? tsh[q] / nsh : 0.0;
-    if (nsh) {  // a sampled SH step runs from the analysis to the cell kernel
+    if (nsh) {  // a sampled SH step runs from the analysis to the end of the step kernel or cell kernel
       ctx->t_step = 0.0;
       long long k = 0;
       for (const ChunkEvents &c : sampled)
-        for (size_t j = 0; j < c.sh.size() && j < c.cont.size(); j++) {
+        for (size_t j = 0; j < c.sh.size() && j < c.mom.size(); j++) {
           float w = 0;
-          CK(hipEventElapsedTime(&w, ctx->ev[c.sh[j]], ctx->ev[c.cont[j] + 1]));
+          const int end = j < c.cont.size() ? c.cont[j] + 1 : c.mom[j] + 1;
+          CK(hipEventElapsedTime(&w, ctx->ev[c.sh[j]], ctx->ev[end]));
           ctx->t_step += w;
           k++;
         }

@@ -194,9 +194,10 @@ int odis_request_stop(odis_ctx *ctx);
  *                    the reference's operator coefficient by coefficient.
  *                    The two differ by rounding only.  Setting 1 on a context
  *                    without the compact form returns ODIS_E_UNSUPPORTED.
- *  ODIS_OPT_ONE_KERNEL 1: with the compact form and no SH basis, one kernel
- *                    per step over patches of cells (default); 0: a face
- *                    and a cell kernel per step.  The state is the same bit
+ *  ODIS_OPT_ONE_KERNEL 1: with the compact form, one kernel per step over
+ *                    patches of cells (default; with an SH basis it follows
+ *                    the SH kernels of the step); 0: a face and a cell
+ *                    kernel per step.  The state is the same bit
  *                    for bit; dissipation sums differ by rounding (another
  *                    grouping of the per-face terms).
  *  ODIS_OPT_PATCH_CELLS at most 64 (default), 128 or 256 cells per patch of
@@ -208,9 +209,11 @@ int odis_request_stop(odis_ctx *ctx);
  *  ODIS_OPT_STEP_LANES lanes per cell of the one-kernel step: 4 (default,
  *                    about one face per lane) or 2 (about two).
  *  ODIS_OPT_SH_CELLS cells per lane of the on-the-fly SH analysis: 0
- *                    (default: 16 for degrees 8, 16 and 32, which have
- *                    compiled kernels, 8 for the others), 4, 8 or 16; the
- *                    coefficients depend on it by rounding only. */
+ *                    (default: for degrees 8, 16 and 32, which have
+ *                    compiled kernels, the fewest even count >= 4 whose grid
+ *                    fits the workgroups the device holds at once; 8 for the
+ *                    others), 4, 8 or 16; the coefficients depend on it by
+ *                    rounding only. */
 #define ODIS_OPT_GRAPHS     1
 #define ODIS_OPT_COMPACT    2
 #define ODIS_OPT_ONE_KERNEL 3

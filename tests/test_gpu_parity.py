@@ -397,6 +397,44 @@ def _assert_same_state(a, b, insts, where):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
+def test_one_kernel_step_with_sh_matches_face_cell_pair(product, precision):
+    """With an ice shell coupled through the SH projection, the one-kernel
+    step (its face phase reading eta_eff) leaves the same state bit for bit
+    as the face + cell pair: on-the-fly basis at a compiled degree (16) and
+    at a runtime degree (12), and the stored basis; two instances, chunk
+    boundaries and graph replays."""
+    G = product
+    g, _ = generated_grid(5)
+    p, _ = G.make_params(time_step=2.0, radius=232.1e3, g=0.1166, surface=G.LID_MEMBR,
+                         shell_thickness=20e3, love_reduct=0.99)
+    mesh = G.Mesh(g, p.radius)
+    for L, mode in ((16, G.SH_RECUR), (12, G.SH_RECUR), (8, G.SH_FP64)):
+        b1 = np.array(G.surface_beta(G.LID_MEMBR, L, 252.1e3, 0.113, 20e3, 38e3))
+        b2 = np.array(G.surface_beta(G.LID_MEMBR, L, 252.1e3, 0.113, 10e3, 38e3))
+        beta = np.stack([b1, b2])
+        one = G.Context(mesh, [p, p], precision=precision)
+        pair = G.Context(mesh, [p, p], precision=precision)
+        for c in (one, pair):
+            c.set_sh(L, beta, mode=mode)
+        assert one.option(G.OPT_ONE_KERNEL) == 1
+        pair.use_one_kernel(False)
+        for n in (1, 2, 65):
+            one.step(n)
+            pair.step(n)
+            _assert_same_state(one, pair, (0, 1), (L, mode, n))
+        one.use_graphs(True)
+        pair.use_graphs(True)
+        for n in (64, 3):
+            one.step(n)
+            pair.step(n)
+            _assert_same_state(one, pair, (0, 1), (L, mode, "graph", n))
+        st = one.get_state(0)
+        assert np.all(np.isfinite(st["eta"])) and np.abs(st["eta"]).max() > 0
+        one.close()
+        pair.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
 def test_compact_form_matches_tabulated_operator(product, oracle, precision):
     """The cell-local TRiSK form (the default) and the tabulated operator
     differ by rounding only: two instances with different thickness and
