@@ -466,7 +466,7 @@ __global__ __launch_bounds__(kBlock) void k_interp_ll(int n_rows, const int *rp,
 // ---------------------------------------------------------------------------
 // Basis Yt[k][i] (k = harmonic, i = device cell) stored basis-major.  The
 // analysis b_k = sum_i Yt[k][i] x_i runs one workgroup per (harmonic, chunk
-// of kShChunks cell ranges) and writes partial sums, which k_sh_coef adds in
+// of kShChunks cell ranges) and writes partial sums, which k_sh_bsum adds in
 // a fixed order; the synthesis y_i = sum_k c_k Yt[k][i] runs 64 cells per
 // workgroup, each of its 4 waves over a quarter of the harmonics, the
 // quarters added in a fixed order through LDS.  Every basis entry is read
@@ -553,28 +553,30 @@ __global__ __launch_bounds__(kBlock) void k_sh_bsum(int K, int n_part, double *b
   b_part[(size_t(gridDim.y) * n_part + b) * K + j] = t;
 }
 
-// c_k = D_k sum_j Minv[k][j] b_j: b in LDS, one wavefront per k, 32 row
-// loads in flight per lane, a fixed-order wave sum.  cmaj (nullable) also
-// receives c in m-major order (put_mmajor).
+// c_k = D_k sum_j Minv[k][j] b_j: one wavefront per k; each lane loads its
+// row entries and b entries (32 of each in flight) and the wave sums in a
+// fixed order.  cmaj (nullable) also receives c in m-major order
+// (put_mmajor).
 __global__ __launch_bounds__(kBlock) void k_sh_coef(int K, const double *__restrict__ Minv, const double *D,
                                                     const double *__restrict__ bsum, double *c, const double *chain,
                                                     double *cmaj) {
-  extern __shared__ double bs[];
   const int b = blockIdx.y;
-  for (int j = threadIdx.x; j < K; j += kBlock) bs[j] = bsum[size_t(b) * K + j];
-  __syncthreads();
   const int lane = threadIdx.x & 63;
   const int k = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
   if (k >= K) return;
-  const double *row = Minv + size_t(k) * K;
+  const double *row = Minv + size_t(k) * K, *bv = bsum + size_t(b) * K;
   double acc = 0.0;
   for (int j0 = lane; j0 < K; j0 += 32 * 64) {
-    double r[32];
+    double r[32], x[32];
 #pragma unroll
-    for (int u = 0; u < 32; ++u) r[u] = j0 + 64 * u < K ? row[j0 + 64 * u] : 0.0;
+    for (int u = 0; u < 32; ++u) {
+      const bool in = j0 + 64 * u < K;
+      r[u] = in ? row[j0 + 64 * u] : 0.0;
+      x[u] = in ? bv[j0 + 64 * u] : 0.0;
+    }
 #pragma unroll
     for (int u = 0; u < 32; ++u)
-      if (j0 + 64 * u < K) acc += r[u] * bs[j0 + 64 * u];
+      if (j0 + 64 * u < K) acc += r[u] * x[u];
   }
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
   if (lane != 0) return;
@@ -637,10 +639,11 @@ __global__ __launch_bounds__(kBlock) void k_sh_synth(int K, int N, const double 
 // used for the Gram matrix comes from the same table), cos / sin(m lon) by
 // rotation from cos / sin(lon).  No basis is read from memory: per pass the
 // kernels read 4 values per cell and the coefficient table.
-//   analysis: wave (cell chunk, m pair) -- m and L - m, so every wave has
-//     L + 2 degrees -- kShCpt cells per lane, one wave sum per (l, +-m),
-//     partials per chunk added in a fixed order by k_sh_coef;
-//   synthesis: one lane per cell over all (l, m).
+//   analysis, runtime degree: workgroup (cell chunks, m pair) -- m and
+//     L - m, so every wave has L + 2 degrees -- CPT cells per lane, per
+//     (l, +-m) a wave sum, partials per workgroup added in a fixed order by
+//     k_sh_bsum; degrees 8, 16 and 32 have the pair analysis below;
+//   synthesis: one lane per cell over an m-range.
 
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
@@ -812,6 +815,7 @@ __global__ __launch_bounds__(64 * kShWaves) void k_sh_analysis_rec(int Lrt, int 
 // chain (m-major): for m = 0..L and k = 0..L-m the pair (beta, S) of degree
 // l = m + k at 2 (m (L + 1) - m (m - 1) / 2 + k) (k = 0: beta = 1, S = 1;
 // k = 1: beta = 0), then C_0..C_L.
+// Block sums go to b_part ([B][n_cblk][K]), added by k_sh_bsum.
 constexpr int kPairWaves = 4;
 
 template <int LT>
@@ -2202,7 +2206,7 @@ static int sh_apply(odis_ctx *ctx, const T *x, long long stride, int nb, const d
     const double *bsum = ctx->sh_b + size_t(nb) * n_part * K;
     hipExtLaunchKernelGGL(k_sh_bsum, dim3((K + kBlock - 1) / kBlock, nb), blk, 0, ctx->stream,
                           ev ? ev[2] : nullptr, nullptr, 0, K, n_part, ctx->sh_b);
-    hipExtLaunchKernelGGL(k_sh_coef, dim3((K + 3) / 4, nb), blk, sizeof(double) * K, ctx->stream, nullptr,
+    hipExtLaunchKernelGGL(k_sh_coef, dim3((K + 3) / 4, nb), blk, 0, ctx->stream, nullptr,
                           ev ? ev[3] : nullptr, 0, K, ctx->sh_Minv, D, bsum, c, chain,
                           ctx->sh_mode == ODIS_SH_RECUR ? ctx->sh_cmaj : static_cast<double *>(nullptr));
   }
