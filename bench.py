@@ -196,6 +196,120 @@ def stencil_roofline(G, ctx, mesh, tm, p_bytes, one):
     return "k_face", fb, tm["face_ms"], {"face": fb, "cell": cb}
 
 
+SWEEP_SEED = 20261203
+# largest gravity-wave frequency at 40962 cells for a 38 km Enceladus ocean
+# (twice the 0.0193 s^-1 measured at 10242 cells); it scales as sqrt(h)
+OMEGA_MAX_40962_38KM = 0.0386
+
+
+def sweep_params(G, n, seed=SWEEP_SEED):
+    """The batched-sweep instances: thickness 10^U(3, 5) m and linear drag
+    10^U(-11, -7) s^-1 drawn in that order per instance from
+    numpy.random.default_rng(seed); eccentricity tide, free surface; each
+    instance's dt is the quantised step below 0.35 / omega_max(h)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    ps = []
+    for _ in range(n):
+        h = 10.0 ** rng.uniform(3.0, 5.0)
+        alpha = 10.0 ** rng.uniform(-11.0, -7.0)
+        w = OMEGA_MAX_40962_38KM * (h / 38e3) ** 0.5
+        p, _ = G.make_params(h=h, alpha=alpha, time_step=0.35 / w)
+        ps.append(p)
+    return ps
+
+
+def sweep_run(G, args):
+    """Batched sweep: 256 instances on the 40962-cell grid in one FP32
+    context (every kernel takes the instances along its grid's y axis),
+    against one plain FP64 instance of the median thickness on the same
+    grid."""
+    grid = G.Grid.generate(7, 64)
+    ps = sweep_params(G, args.sweep_instances)
+    mesh = G.Mesh(grid, ps[0].radius)
+    n_steps, n_warm = max(20, args.steps // 10), 10
+    ctx = G.Context(mesh, ps, precision=G.FP32)
+    el, _ = timed_run(ctx, n_steps, n_warm, 0)
+    ok = bool(all(abs(x) < float("inf") for x in ctx.dissipation()))
+    ctx.close()
+    hs = sorted(p.h for p in ps)
+    p1 = [p for p in ps if p.h == hs[len(hs) // 2]][0]
+    c1 = G.Context(mesh, p1, precision=G.FP64)
+    el1, _ = timed_run(c1, args.steps, args.warmup, 0)
+    c1.close()
+    batched = len(ps) * mesh.N * n_steps / el
+    single = mesh.N * args.steps / el1
+    return {"instances": len(ps), "cells": mesh.N, "seed": SWEEP_SEED, "precision": "fp32 state, fp64 dissipation",
+            "value": batched, "ms_per_step": 1e3 * el / n_steps, "steps": n_steps,
+            "plain_fp64_single_instance": single, "batched_over_single": batched / single,
+            "finite": ok}
+
+
+def converge_run(G, args):
+    """Time to converged orbit-averaged dissipation for one instance of the
+    plain-method case (Enceladus free surface, 38 km, ECC, linear drag
+    args.alpha) on the 10242-cell grid, plain FP64 and the FP32 form.
+    D_k is the mean per-step dissipation over orbit k; the run stops when
+    |D_k - D_{k-1}| / D_k < 1e-4 for 3 consecutive orbits (D* = D_k) or at
+    args.max_orbits (capped).  Time-to-converged: wall time until
+    |D_k - D*| / D* <= 1 % and it stays so for 3 orbits, D* of plain FP64."""
+    import numpy as np
+    grid = G.Grid.generate(6, 64)
+    # 4200 steps per orbit: the quantised 30 s target (4000) is unstable on this grid
+    p, n_s = G.make_params(alpha=args.alpha, time_step=28.2)
+    mesh = G.Mesh(grid, p.radius)
+    res = {}
+    for name, prec in (("fp64", G.FP64), ("fp32", G.FP32)):
+        ctx = G.Context(mesh, p, precision=prec)
+        D, t_end, stable = [], [], 0
+        ctx.step(1, sync=True)  # state 0 -> 1 outside the clock
+        t0 = time.perf_counter()
+        first = ctx.get_state()["iter"]
+        k = 0
+        while k < args.max_orbits:
+            ctx.step(n_s, sync=True)
+            t_end.append(time.perf_counter() - t0)
+            ser = ctx.dissipation_series(0, first + k * n_s, n_s)
+            D.append(float(np.mean(ser)))
+            if k > 0 and abs(D[-1] - D[-2]) / abs(D[-1]) < 1e-4:
+                stable += 1
+            else:
+                stable = 0
+            k += 1
+            if k % 100 == 0:
+                print("bench: converge %s orbit %d D %.6e (%.1f s)" % (name, k, D[-1], t_end[-1]),
+                      file=sys.stderr, flush=True)
+            if stable >= 3:
+                break
+        ctx.close()
+        res[name] = {"orbits": k, "converged": stable >= 3, "D_star": D[-1], "D": D, "t": t_end,
+                     "wall_s": t_end[-1], "steps_per_s": k * n_s / t_end[-1]}
+    dstar = res["fp64"]["D_star"]
+
+    def t_conv(r):
+        D, t = r["D"], r["t"]
+        for i in range(len(D) - 2):
+            if all(abs(D[j] - dstar) <= 0.01 * abs(dstar) for j in range(i, len(D))):
+                return t[i], i + 1
+        return None, None
+
+    for r in res.values():
+        r["time_to_1pct_s"], r["orbits_to_1pct"] = t_conv(r)
+    out = {"metric": "time-to-converged dissipation (+-1%) vs plain FP64", "value": res["fp32"]["time_to_1pct_s"],
+           "unit": "s", "higher_is_better": False, "n_gpus": 1,
+           "config": {"workload": "plain-method Enceladus free-surface ocean, 38 km, ECC, linear drag %.0e s^-1, "
+                                  "one instance, rest start" % args.alpha,
+                      "cells": mesh.N, "steps_per_orbit": n_s, "dt_s": p.dt, "max_orbits": args.max_orbits}}
+    for name, r in res.items():
+        out["config"][name] = {k: r[k] for k in ("orbits", "converged", "D_star", "wall_s", "steps_per_s",
+                                                  "time_to_1pct_s", "orbits_to_1pct")}
+    out["config"]["fp32_D_star_rel_err"] = abs(res["fp32"]["D_star"] - dstar) / abs(dstar)
+    out["config"]["fp64_time_over_fp32_time"] = (res["fp64"]["time_to_1pct_s"] / res["fp32"]["time_to_1pct_s"]
+                                                 if res["fp64"]["time_to_1pct_s"] and res["fp32"]["time_to_1pct_s"]
+                                                 else None)
+    return out
+
+
 def rel_l2(a, b):
     import numpy as np
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
@@ -206,9 +320,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--workload", choices=["shell", "free"], default="shell",
+    ap.add_argument("--workload", choices=["shell", "free", "converge"], default="shell",
                     help="shell: ocean under a 20 km ice shell with SH coupling to degree 32 (default); "
-                         "free: free-surface ocean, stencils only")
+                         "free: free-surface ocean, stencils only; converge: time to converged dissipation "
+                         "of the plain-method case, FP64 and FP32")
+    ap.add_argument("--alpha", type=float, default=1e-8, help="linear drag of the converge workload (s^-1)")
+    ap.add_argument("--max-orbits", type=int, default=3000, help="orbit cap of the converge workload")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the batched-sweep extra")
+    ap.add_argument("--sweep-instances", type=int, default=256, help="instances of the batched-sweep extra")
     ap.add_argument("--level", type=int, default=8, help="reference grid level (8 = 163842 cells)")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
                     help="precision of the headline value (fp32: FP32 stencils, split-BF16 SH basis)")
@@ -258,6 +377,10 @@ def main():
 
     import numpy as np
     import geodesicodis_amd as G
+
+    if args.workload == "converge":
+        print(json.dumps(converge_run(G, args)), flush=True)
+        return
 
     t0 = time.time()
     grid = G.Grid.generate(args.level, 64)
@@ -420,6 +543,9 @@ def main():
                     "kernel": "%s<%s>" % (kn, tn2), "kernel_ms": km, "alg_bytes": kb,
                     "hbm_frac": (kb / (km * 1e-3) / 1e9 / HBM_PEAK_GBS) if km > 0 else None}
                 c2.close()
+
+    if not args.no_extra and not args.no_sweep:
+        out["config"]["sweep_fp32"] = sweep_run(G, args)
 
     if not args.no_cpu_baseline:
         import oracle as O
