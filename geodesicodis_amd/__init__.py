@@ -37,7 +37,8 @@ EXPORTS = [
     "odis_round_period", "odis_quantise_dt", "odis_config_read", "odis_host_last_error",
     "odis_run_case", "odis_interp_ll_set", "odis_interp_ll", "odis_sh_set_basis",
     "odis_sh_project", "odis_sh_synthesize", "odis_surface_beta", "odis_set_option",
-    "odis_get_option", "odis_mesh_compact_check", "odis_get_kernel_times",
+    "odis_get_option", "odis_mesh_compact_check", "odis_get_kernel_times", "odis_rbf_set",
+    "odis_velocity_xyz",
 ]
 SH_OFF, SH_FP64, SH_SPLIT_BF16, SH_RECUR = 0, 1, 2, 3
 OPT_GRAPHS, OPT_COMPACT, OPT_ONE_KERNEL, OPT_PATCH_CELLS, OPT_STEP_BLOCKS, OPT_STEP_LANES, OPT_SH_CELLS = \
@@ -128,6 +129,8 @@ def lib():
             "odis_run_case": ([ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
             "odis_interp_ll_set": ([vp, ctypes.c_int, ctypes.c_int, ip, ip, dp], ctypes.c_int),
             "odis_interp_ll": ([vp, ctypes.c_int, dp], ctypes.c_int),
+            "odis_rbf_set": ([vp, ctypes.c_int, ctypes.c_int, ip, ip, dp], ctypes.c_int),
+            "odis_velocity_xyz": ([vp, ctypes.c_int, dp], ctypes.c_int),
             "odis_sh_set_basis": ([vp, ctypes.c_int, dp, ctypes.c_int], ctypes.c_int),
             "odis_sh_project": ([vp, dp, dp], ctypes.c_int),
             "odis_sh_synthesize": ([vp, dp, dp], ctypes.c_int),
@@ -208,7 +211,8 @@ class Grid:
             self._h = None
 
 
-_MESH_INT = {"faces", "node_face_dir", "face_nodes", "interp_count", "interp_friends"}
+_MESH_INT = {"faces", "node_face_dir", "face_nodes", "interp_count", "interp_friends", "rbf_row_ptr",
+             "rbf_col"}
 
 
 class Mesh:
@@ -233,6 +237,11 @@ class Mesh:
         d = MeshDesc()
         lib().odis_mesh_describe(self._h, ctypes.byref(d))
         return d
+
+    def rbf_operator(self):
+        """The reference's operatorRBFinterp (src/mesh.cpp:2263-2361) as CSR
+        (row_ptr [3N + 1], col, val): rows 3i + x/y/z, face columns."""
+        return self.get("rbf_row_ptr"), self.get("rbf_col"), self.get("rbf_val")
 
     def compact_check(self, desc=None):
         """(ok, reason, coef_err, div_err, area_err) of the cell-local TRiSK form
@@ -371,6 +380,22 @@ class Context:
         out = np.zeros(self._ll_rows)
         self._check(lib().odis_interp_ll(self._h, int(inst), _dp(out)))
         return out
+
+    def set_rbf(self, row_ptr=None, col=None, val=None):
+        """Attach the RBF Cartesian-velocity operator (3N x E CSR, reference
+        numbering); by default the one the mesh builds (operatorRBFinterp)."""
+        if row_ptr is None:
+            row_ptr, col, val = self.mesh.rbf_operator()
+        rp = np.ascontiguousarray(row_ptr, np.int32)
+        c = np.ascontiguousarray(col, np.int32)
+        v = np.ascontiguousarray(val, np.float64)
+        self._check(lib().odis_rbf_set(self._h, len(rp) - 1, int(self.mesh.E), _ip(rp), _ip(c), _dp(v)))
+
+    def velocity_xyz(self, inst=0):
+        """Cartesian velocity [N, 3] at cell centres (reference numbering)."""
+        out = np.zeros(self.mesh.N * 3)
+        self._check(lib().odis_velocity_xyz(self._h, int(inst), _dp(out)))
+        return out.reshape(-1, 3)
 
     def set_sh(self, l_max, beta=None, mode=SH_FP64):
         """Self-gravity / shell coupling: beta is (n_inst, l_max + 1) or

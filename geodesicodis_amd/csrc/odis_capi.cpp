@@ -183,6 +183,10 @@ extern "C" int odis_mesh_get(const odis_mesh *mm, const char *name, void *dst) {
       {"interp_friends", m.trisk_faces.data(), m.trisk_faces.size(), sizeof(int)},
       {"interp_weights", m.trisk_w.data(), m.trisk_w.size(), sizeof(double)},
       {"trisk_coef", m.trisk_coef.data(), m.trisk_coef.size(), sizeof(double)},
+      {"face_normal_xyz", m.face_normal_xyz.data(), m.face_normal_xyz.size(), sizeof(double)},
+      {"rbf_row_ptr", m.rbf_row_ptr.data(), m.rbf_row_ptr.size(), sizeof(int)},
+      {"rbf_col", m.rbf_col.data(), m.rbf_col.size(), sizeof(int)},
+      {"rbf_val", m.rbf_val.data(), m.rbf_val.size(), sizeof(double)},
       {"cell_lat", m.lat.data(), m.lat.size(), sizeof(double)},
       {"cell_lon", m.lon.data(), m.lon.size(), sizeof(double)},
   };

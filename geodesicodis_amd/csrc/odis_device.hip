@@ -447,6 +447,31 @@ __global__ __launch_bounds__(kBlock) void k_velocity_en(int E, const int *tri_id
   uv[2 * e + 1] = ny * vn + (-nx) * vt;
 }
 
+// RBF reconstruction of the Cartesian velocity at cell centres
+// (interpolateVelocityCartRBF, src/interpolation.cpp:64-174): per device
+// cell the reference operator's 3 x k block (k <= 6 faces, slot-major, the
+// CSR's column order), out [3][N].
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_velocity_xyz(int N, const int *__restrict__ idx,
+                                                         const double *__restrict__ w, const T *__restrict__ v,
+                                                         double *out) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= N) return;
+  double u[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    const int f = idx[j * N + i];
+    u[j] = f >= 0 ? double(v[f]) : 0.0;
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) acc += w[(c * 6 + j) * N + i] * u[j];
+    out[c * N + i] = acc;
+  }
+}
+
 // CSR rows of the lat-lon interpolation: one wavefront per row, the eta
 // columns already mapped to device cell indices on the host.
 template <typename T>
@@ -1386,6 +1411,11 @@ struct odis_ctx {
   int *ll_rp = nullptr, *ll_col = nullptr;
   double *ll_val = nullptr, *ll_out = nullptr;
   std::vector<void *> ll_allocs;
+  // RBF Cartesian velocity operator: [6][N] device face ids (-1 pad),
+  // [3][6][N] coefficients, [3][N] output
+  int *rbf_idx = nullptr;
+  double *rbf_w = nullptr, *rbf_out = nullptr;
+  std::vector<void *> rbf_allocs;
   std::string err;
 };
 
@@ -1628,6 +1658,7 @@ extern "C" int odis_destroy(odis_ctx *ctx) {
     if (ctx->coef_host[k]) (void)hipHostFree(ctx->coef_host[k]);
   }
   for (void *p : ctx->allocs) (void)hipFree(p);
+  for (void *p : ctx->rbf_allocs) (void)hipFree(p);
   for (void *p : ctx->ll_allocs) (void)hipFree(p);
   for (void *p : ctx->sh_allocs) (void)hipFree(p);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -2799,6 +2830,72 @@ extern "C" int odis_interp_ll_set(odis_ctx *ctx, int n_rows, int n_cols, const i
   if ((rc = up(reinterpret_cast<void **>(&ctx->ll_val), vals.data(), vals.size() * sizeof(double)))) return rc;
   if ((rc = up(reinterpret_cast<void **>(&ctx->ll_out), nullptr, size_t(n_rows) * sizeof(double)))) return rc;
   ctx->ll_rows = n_rows;
+  return ODIS_OK;
+}
+
+extern "C" int odis_rbf_set(odis_ctx *ctx, int n_rows, int n_cols, const int *row_ptr, const int *col,
+                            const double *val) {
+  if (!ctx || !row_ptr || !col || !val) return ODIS_E_INVALID;
+  const int N = ctx->N, E = ctx->E;
+  if (n_rows != 3 * N || n_cols != E) return fail(ctx, ODIS_E_INVALID, "RBF operator must be 3N x E");
+  if (row_ptr[0] != 0) return fail(ctx, ODIS_E_INVALID, "row_ptr[0] must be 0");
+  std::vector<int> idx(size_t(6) * N, -1);
+  std::vector<double> w(size_t(18) * N, 0.0);
+  for (int i = 0; i < N; i++) {
+    const int r = ctx->cell_perm[i];
+    const int k0 = row_ptr[3 * i], len = row_ptr[3 * i + 1] - k0;
+    if (len < 0 || len > 6) return fail(ctx, ODIS_E_INVALID, "RBF row must hold 0..6 faces");
+    for (int c = 0; c < 3; c++) {
+      const int kc = row_ptr[3 * i + c];
+      if (row_ptr[3 * i + c + 1] - kc != len)
+        return fail(ctx, ODIS_E_INVALID, "RBF rows of a cell must share their faces");
+      for (int j = 0; j < len; j++) {
+        const int f = col[kc + j];
+        if (f < 0 || f >= E) return fail(ctx, ODIS_E_INVALID, "RBF column out of range");
+        if (f != col[k0 + j]) return fail(ctx, ODIS_E_INVALID, "RBF rows of a cell must share their faces");
+        idx[size_t(j) * N + r] = ctx->face_perm[f];
+        w[(size_t(c) * 6 + j) * N + r] = val[kc + j];
+      }
+    }
+  }
+  CK(hipStreamSynchronize(ctx->stream));
+  for (void *p : ctx->rbf_allocs) (void)hipFree(p);
+  ctx->rbf_allocs.clear();
+  ctx->rbf_idx = nullptr;
+  ctx->rbf_w = ctx->rbf_out = nullptr;
+  auto up = [&](void **dst, const void *src, size_t bytes) -> int {
+    CK(hipMalloc(dst, bytes));
+    ctx->rbf_allocs.push_back(*dst);
+    if (src) CK(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    return ODIS_OK;
+  };
+  int rc;
+  if ((rc = up(reinterpret_cast<void **>(&ctx->rbf_idx), idx.data(), idx.size() * sizeof(int)))) return rc;
+  if ((rc = up(reinterpret_cast<void **>(&ctx->rbf_w), w.data(), w.size() * sizeof(double)))) return rc;
+  if ((rc = up(reinterpret_cast<void **>(&ctx->rbf_out), nullptr, size_t(3) * N * sizeof(double)))) return rc;
+  return ODIS_OK;
+}
+
+extern "C" int odis_velocity_xyz(odis_ctx *ctx, int inst, double *xyz) {
+  if (!ctx || !xyz) return ODIS_E_INVALID;
+  if (inst < 0 || inst >= ctx->B) return fail(ctx, ODIS_E_RANGE, "instance out of range");
+  if (!ctx->rbf_idx) return fail(ctx, ODIS_E_INVALID, "no RBF operator set");
+  const int N = ctx->N;
+  const dim3 grid((N + kBlock - 1) / kBlock), blk(kBlock);
+  if (ctx->precision == ODIS_FP64)
+    hipLaunchKernelGGL((k_velocity_xyz<double>), grid, blk, 0, ctx->stream, N, ctx->rbf_idx, ctx->rbf_w,
+                       ctx->d.v[ctx->cur] + size_t(inst) * ctx->E, ctx->rbf_out);
+  else
+    hipLaunchKernelGGL((k_velocity_xyz<float>), grid, blk, 0, ctx->stream, N, ctx->rbf_idx, ctx->rbf_w,
+                       ctx->f.v[ctx->cur] + size_t(inst) * ctx->E, ctx->rbf_out);
+  CK(hipGetLastError());
+  std::vector<double> h(size_t(3) * N);
+  CK(hipMemcpyAsync(h.data(), ctx->rbf_out, sizeof(double) * 3 * N, hipMemcpyDeviceToHost, ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  for (int r = 0; r < N; r++) {
+    const int i = ctx->cell_inv[r];
+    for (int c = 0; c < 3; c++) xyz[3 * i + c] = h[size_t(c) * N + r];
+  }
   return ODIS_OK;
 }
 

@@ -55,9 +55,15 @@ struct MeshOps {
   std::vector<int> trisk_faces;          // [E*10]
   std::vector<double> trisk_w;           // [E*10] TRiSK weights w_ee'
   std::vector<double> trisk_coef;        // [E*10] w_ee' l_e' / d_e
+  std::vector<double> face_normal_xyz;   // [E*3] Cartesian unit normal (first vertex x second)
+  // RBF velocity reconstruction operator, CSR [3N x E] (rows 3i + x/y/z)
+  std::vector<int> rbf_row_ptr, rbf_col;
+  std::vector<double> rbf_val;
 };
 
 MeshOps build_mesh(const Grid &g, double radius);  // throws std::runtime_error
+// The reference's operatorRBFinterp of a built mesh (fills the rbf_* arrays).
+void build_rbf_operator(MeshOps &M);
 
 // The Globals fields the LTE step and its drivers read (include/globals.h).
 struct Config {

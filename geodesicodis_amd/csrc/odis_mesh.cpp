@@ -94,6 +94,20 @@ inline void edge_normal(LL a, LL b, double *nx, double *ny) {
   *ny = tx;
 }
 
+// normalVectorBetweenXYZ, include/mathRoutines.h:246-283: unit vector along
+// cart(a) x cart(b), the Cartesian normal of the face with vertices a, b
+// (sph2cart on the colatitude, as the reference converts).
+inline void edge_normal_xyz(LL a, LL b, double *n) {
+  const double ca = kPi * 0.5 - a.lat, cb = kPi * 0.5 - b.lat;
+  const double ax = std::sin(ca) * std::cos(a.lon), ay = std::sin(ca) * std::sin(a.lon), az = std::cos(ca);
+  const double bx = std::sin(cb) * std::cos(b.lon), by = std::sin(cb) * std::sin(b.lon), bz = std::cos(cb);
+  double x = ay * bz - az * by, y = -(ax * bz - az * bx), z = ax * by - ay * bx;
+  const double mag = std::sqrt(x * x + y * y + z * z);
+  n[0] = x / mag;
+  n[1] = y / mag;
+  n[2] = z / mag;
+}
+
 // intersectPointSph, include/mathRoutines.h:303-370: crossing of the great
 // circles (a, b) and (c, d), on the side given by (a x b) x (c x d).
 inline LL crossing(LL a, LL b, LL c, LL d) {
@@ -219,6 +233,7 @@ MeshOps build_mesh(const Grid &g, double r) {
   M.face_lon.assign(E, 0.0);
   M.face_m.assign(size_t(E) * 2, 0.0);
   M.face_normal.assign(size_t(E) * 2, 0.0);
+  M.face_normal_xyz.assign(size_t(E) * 3, 0.0);
   M.face_intercept.assign(size_t(E) * 2, 0.0);
   int e = 0;
   for (int i = 0; i < N; i++) {
@@ -239,6 +254,7 @@ MeshOps build_mesh(const Grid &g, double r) {
       M.face_m[e * 2 + 1] = map_factor(c.lat, g.lat[f], c.lon, g.lon[f]);
       M.face_node_dist[e] = arc_atan2(node(i), node(f)) * r;
       edge_normal(a, b, &M.face_normal[e * 2], &M.face_normal[e * 2 + 1]);
+      edge_normal_xyz(a, b, &M.face_normal_xyz[e * 3]);
       const LL x = crossing(a, b, node(i), node(f));
       M.face_intercept[e * 2] = x.lat;
       M.face_intercept[e * 2 + 1] = x.lon;
@@ -324,7 +340,106 @@ MeshOps build_mesh(const Grid &g, double r) {
       const int f = M.faces[i * 6 + j];
       M.div_coef[i * 6 + j] = -M.node_face_dir[i * 6 + j] * M.face_len[f] / M.cv_area[i];  // src/mesh.cpp:3242-3247
     }
+  build_rbf_operator(M);
   return M;
+}
+
+// RBF reconstruction of the Cartesian velocity at cell centres from the
+// face-normal velocities (operatorRBFinterp = RBF Vinv node2faceAdj,
+// src/mesh.cpp:2263-2361, with the per-cell matrices of src/mesh.cpp:360-430
+// and the adjacency of src/mesh.cpp:2725-2763).  For cell i with faces f_j
+// (j in friend order) and Cartesian face normals n_j:
+//     A_i[j1][j2] = phi(arc(f_j1, f_j2)) n_j1 . n_j2,
+//     row 3i + c, column f_j:  sum_m phi(arc(i, f_m)) n_m[c] (A_i^-1)[m][j],
+// with phi(d) = exp(-d^2) (shape parameter 1, phi = 1 below 1e-8 rad,
+// include/mathRoutines.h:432-439) and arcs between face centres and cell
+// nodes on the unit sphere.  A_i^-1 by LU with partial pivoting (the
+// reference's dynamic-size inverse).  Columns ascend within a row.
+void build_rbf_operator(MeshOps &M) {
+  const int N = M.N;
+  M.rbf_row_ptr.assign(size_t(3) * N + 1, 0);
+  M.rbf_col.clear();
+  M.rbf_val.clear();
+  auto rbf = [](double d) { return d < 1e-8 ? 1.0 : std::exp(-(d * d)); };
+  // colatitude Cartesian points, as distanceBetweenSph2 converts them
+  auto colat_xyz = [](double lat, double lon, double *p) {
+    const double c = kPi * 0.5 - lat;
+    p[0] = 1.0 * std::sin(c) * std::cos(lon);
+    p[1] = 1.0 * std::sin(c) * std::sin(lon);
+    p[2] = 1.0 * std::cos(c);
+  };
+  auto arc = [](const double *a, const double *b) {
+    const double c0 = a[1] * b[2] - a[2] * b[1], c1 = -(a[0] * b[2] - a[2] * b[0]), c2 = a[0] * b[1] - a[1] * b[0];
+    double cr = c0 * c0;
+    cr += c1 * c1;
+    cr += c2 * c2;
+    cr = std::sqrt(cr);
+    return std::atan2(cr, a[0] * b[0] + a[1] * b[1] + a[2] * b[2]);
+  };
+  std::vector<double> fc(size_t(M.E) * 3);
+  for (int e = 0; e < M.E; e++) colat_xyz(M.face_lat[e], M.face_lon[e], &fc[size_t(e) * 3]);
+  for (int i = 0; i < N; i++) {
+    int k = 0;
+    while (k < 6 && M.faces[i * 6 + k] >= 0) k++;
+    int f[6];
+    double n[6][3], A[6][6], Ai[6][6], phin[6], pi[3];
+    colat_xyz(M.lat[i], M.lon[i], pi);
+    for (int j = 0; j < k; j++) {
+      f[j] = M.faces[i * 6 + j];
+      for (int c = 0; c < 3; c++) n[j][c] = M.face_normal_xyz[size_t(f[j]) * 3 + c];
+    }
+    for (int j1 = 0; j1 < k; j1++) {
+      const double *a = &fc[size_t(f[j1]) * 3];
+      for (int j2 = 0; j2 < k; j2++) {
+        const double *b = &fc[size_t(f[j2]) * 3];
+        A[j1][j2] = rbf(arc(a, b)) * (n[j1][0] * n[j2][0] + n[j1][1] * n[j2][1] + n[j1][2] * n[j2][2]);
+      }
+      phin[j1] = rbf(arc(pi, a));
+    }
+    // inverse: LU with row pivoting, then one solve per unit column
+    int perm[6];
+    for (int j = 0; j < k; j++) perm[j] = j;
+    for (int c = 0; c < k; c++) {
+      int piv = c;
+      for (int r = c + 1; r < k; r++)
+        if (std::fabs(A[r][c]) > std::fabs(A[piv][c])) piv = r;
+      if (A[piv][c] == 0.0) throw std::runtime_error("RBF matrix of cell " + std::to_string(i) + " is singular");
+      if (piv != c) {
+        for (int q = 0; q < k; q++) std::swap(A[c][q], A[piv][q]);
+        std::swap(perm[c], perm[piv]);
+      }
+      for (int r = c + 1; r < k; r++) {
+        A[r][c] /= A[c][c];
+        for (int q = c + 1; q < k; q++) A[r][q] -= A[r][c] * A[c][q];
+      }
+    }
+    for (int col = 0; col < k; col++) {
+      double y[6];
+      for (int r = 0; r < k; r++) {
+        double t = perm[r] == col ? 1.0 : 0.0;
+        for (int q = 0; q < r; q++) t -= A[r][q] * y[q];
+        y[r] = t;
+      }
+      for (int r = k - 1; r >= 0; r--) {
+        double t = y[r];
+        for (int q = r + 1; q < k; q++) t -= A[r][q] * Ai[q][col];
+        Ai[r][col] = t / A[r][r];
+      }
+    }
+    int order[6];
+    for (int j = 0; j < k; j++) order[j] = j;
+    std::sort(order, order + k, [&](int x, int y) { return f[x] < f[y]; });
+    for (int c = 0; c < 3; c++) {
+      for (int o = 0; o < k; o++) {
+        const int j = order[o];
+        double w = 0.0;
+        for (int m = 0; m < k; m++) w += phin[m] * n[m][c] * Ai[m][j];
+        M.rbf_col.push_back(f[j]);
+        M.rbf_val.push_back(w);
+      }
+      M.rbf_row_ptr[size_t(3) * i + c + 1] = int(M.rbf_col.size());
+    }
+  }
 }
 
 }  // namespace odis

@@ -155,6 +155,17 @@ int odis_get_velocity_en(odis_ctx *ctx, int inst, const double *face_normal, dou
 int odis_interp_ll_set(odis_ctx *ctx, int n_rows, int n_cols, const int *row_ptr,
                        const int *col, const double *val);
 int odis_interp_ll(odis_ctx *ctx, int inst, double *ll);
+/* Cartesian velocity at cell centres by radial basis functions,
+ * interpolateVelocityCartRBF (src/interpolation.cpp:64-174): xyz = R v with
+ * R the reference's operatorRBFinterp (src/mesh.cpp:2263-2361, 360-430), a
+ * CSR matrix of 3N rows (3i + x, y, z in reference cell numbering) and E
+ * columns (reference face numbering) whose three rows of a cell hold the
+ * same <= 6 faces in the same order (odis_mesh_get "rbf_row_ptr", "rbf_col",
+ * "rbf_val" build it).  It is copied to the device as one 3 x 6 block per
+ * cell; xyz[N][3] comes back in reference numbering. */
+int odis_rbf_set(odis_ctx *ctx, int n_rows, int n_cols, const int *row_ptr, const int *col,
+                 const double *val);
+int odis_velocity_xyz(odis_ctx *ctx, int inst, double *xyz);
 
 /* Self-gravity / ice-shell coupling through a spherical-harmonic projection
  * of eta (the reference's commented design: CalcLegendreFuncs,
@@ -272,8 +283,10 @@ int odis_mesh_sizes(const odis_mesh *m, int *n_cells, int *n_faces, int *n_verti
 /* Fill a descriptor whose pointers stay valid while m lives. */
 int odis_mesh_describe(const odis_mesh *m, odis_mesh_desc *desc);
 /* Copy a named array (same names as the Mesh members, e.g. "face_len",
- * "face_node_dist", "face_normal", "interp_weights", "cv_area"); returns the
- * element count, or < 0.  dst may be null to query the count. */
+ * "face_node_dist", "face_normal", "interp_weights", "cv_area"; also
+ * "face_normal_xyz" [E][3] and the RBF velocity operator "rbf_row_ptr"
+ * [3N + 1], "rbf_col", "rbf_val"); returns the element count, or < 0.  dst
+ * may be null to query the count. */
 int odis_mesh_get(const odis_mesh *m, const char *name, void *dst);
 int odis_mesh_free(odis_mesh *m);
 

@@ -299,3 +299,68 @@ def real_sh_basis(L, lat, lon):
             if m > 0:
                 Y[:, l * l + l - m] = p * np.sin(m * lon)
     return Y
+
+
+def rbf_operator(grid, mesh):
+    """operatorRBFinterp restated in numpy, the reference's RBF reconstruction
+    of Cartesian velocity at cell centres from face-normal velocities
+    (src/mesh.cpp:2263-2361 with the per-cell matrices of src/mesh.cpp:360-430
+    and node2faceAdj of src/mesh.cpp:2725-2763).  Face normals are
+    normalVectorBetweenXYZ (include/mathRoutines.h:246-283) of the face's two
+    Voronoi vertices in the order its inner node lists them; arcs are
+    distanceBetweenSph2 (include/mathRoutines.h:139-160); phi(d) = exp(-d^2)
+    (include/mathRoutines.h:432-439); the per-cell inverse is
+    numpy.linalg.inv.  Returns CSR (row_ptr [3N+1], col, val), rows 3i + c,
+    columns ascending."""
+    faces = mesh.get("faces").reshape(-1, 6)
+    fnodes = mesh.get("face_nodes").reshape(-1, 2)
+    fclat, fclon = mesh.get("face_centre_lat"), mesh.get("face_centre_lon")
+    lat, lon = np.asarray(grid["lat"]), np.asarray(grid["lon"])
+    clat, clon = np.asarray(grid["clat"]).reshape(-1, 6), np.asarray(grid["clon"]).reshape(-1, 6)
+    N, E = faces.shape[0], fnodes.shape[0]
+    deg = (faces >= 0).sum(axis=1)
+
+    def xyz(la, lo):
+        c = np.pi * 0.5 - la
+        return np.stack([np.sin(c) * np.cos(lo), np.sin(c) * np.sin(lo), np.cos(c)], axis=-1)
+
+    def arc(a, b):
+        return np.arctan2(np.linalg.norm(np.cross(a, b), axis=-1), np.sum(a * b, axis=-1))
+
+    n = np.zeros((E, 3))
+    for e in range(E):
+        i = fnodes[e, 0]
+        k = deg[i]
+        j = int(np.nonzero(faces[i, :k] == e)[0][0])
+        a = xyz(clat[i, (j + k - 1) % k], clon[i, (j + k - 1) % k])
+        b = xyz(clat[i, j], clon[i, j])
+        c = np.cross(a, b)
+        n[e] = c / np.linalg.norm(c)
+    fc = xyz(fclat, fclon)
+    pc = xyz(lat, lon)
+    phi = lambda d: np.where(d < 1e-8, 1.0, np.exp(-d * d))
+    rp, col, val = [0], [], []
+    for i in range(N):
+        f = faces[i, :deg[i]]
+        A = phi(arc(fc[f][:, None, :], fc[f][None, :, :])) * (n[f] @ n[f].T)
+        W = (phi(arc(pc[i][None, :], fc[f]))[:, None] * n[f]).T @ np.linalg.inv(A)
+        order = np.argsort(f)
+        for c in range(3):
+            col.extend(f[order].tolist())
+            val.extend(W[c, order].tolist())
+            rp.append(len(col))
+    return np.array(rp, np.int32), np.array(col, np.int32), np.array(val)
+
+
+def csr_apply_rowwise(rp, col, val, x):
+    """y = R x with each row summed in column order from 0.0 (the order the
+    device kernel uses)."""
+    rp, col, val = np.asarray(rp), np.asarray(col), np.asarray(val)
+    n = len(rp) - 1
+    y = np.zeros(n)
+    w = np.diff(rp).max() if n else 0
+    for j in range(w):
+        rows = np.nonzero(np.diff(rp) > j)[0]
+        k = rp[rows] + j
+        y[rows] = y[rows] + val[k] * x[col[k]]
+    return y
