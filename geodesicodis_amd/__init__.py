@@ -38,8 +38,9 @@ EXPORTS = [
     "odis_run_case", "odis_interp_ll_set", "odis_interp_ll", "odis_sh_set_basis",
     "odis_sh_project", "odis_sh_synthesize", "odis_surface_beta", "odis_set_option",
     "odis_get_option", "odis_mesh_compact_check", "odis_get_kernel_times", "odis_rbf_set",
-    "odis_velocity_xyz",
+    "odis_velocity_xyz", "odis_h5_open", "odis_h5_write", "odis_h5_close", "odis_h5_last_error",
 ]
+H5_DISPLACEMENT, H5_VELOCITY, H5_VELOCITY_XYZ, H5_DISSIPATION, H5_DISS_AVG = 1, 2, 4, 8, 16
 SH_OFF, SH_FP64, SH_SPLIT_BF16, SH_RECUR = 0, 1, 2, 3
 OPT_GRAPHS, OPT_COMPACT, OPT_ONE_KERNEL, OPT_PATCH_CELLS, OPT_STEP_BLOCKS, OPT_STEP_LANES, OPT_SH_CELLS = \
     1, 2, 3, 4, 5, 6, 7
@@ -131,6 +132,11 @@ def lib():
             "odis_interp_ll": ([vp, ctypes.c_int, dp], ctypes.c_int),
             "odis_rbf_set": ([vp, ctypes.c_int, ctypes.c_int, ip, ip, dp], ctypes.c_int),
             "odis_velocity_xyz": ([vp, ctypes.c_int, dp], ctypes.c_int),
+            "odis_h5_open": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp,
+                              ctypes.POINTER(vp)], ctypes.c_int),
+            "odis_h5_write": ([vp, ctypes.c_int, ctypes.c_int, dp], ctypes.c_int),
+            "odis_h5_close": ([vp], ctypes.c_int),
+            "odis_h5_last_error": ([], ctypes.c_char_p),
             "odis_sh_set_basis": ([vp, ctypes.c_int, dp, ctypes.c_int], ctypes.c_int),
             "odis_sh_project": ([vp, dp, dp], ctypes.c_int),
             "odis_sh_synthesize": ([vp, dp, dp], ctypes.c_int),
@@ -298,6 +304,35 @@ def make_params(radius=252.1e3, g=0.113, ang_vel=5.307e-5, h=38e3, alpha=1e-8, e
                semimajor=semimajor, tide=tide, surface=surface, friction=friction,
                init_load=init_load, quad_drag=quad_drag)
     return p, n
+
+
+class H5File:
+    """DATA/data.h5 writer in the reference's layout (odis_h5_open / write /
+    close; src/outFiles.cpp:138-681)."""
+
+    def __init__(self, path, n_cells, n_faces, n_slices, fields, face_lat=None, face_lon=None):
+        h = ctypes.c_void_p()
+        fl = None if face_lat is None else np.ascontiguousarray(face_lat, np.float64)
+        fo = None if face_lon is None else np.ascontiguousarray(face_lon, np.float64)
+        rc = lib().odis_h5_open(str(path).encode(), int(n_cells), int(n_faces), int(n_slices), int(fields),
+                                None if fl is None else _dp(fl), None if fo is None else _dp(fo), ctypes.byref(h))
+        if rc:
+            raise OdisError(rc, (lib().odis_h5_last_error() or b"").decode())
+        self._h = h
+
+    def write(self, field, slice_, data):
+        d = np.ascontiguousarray(data, np.float64).ravel()
+        rc = lib().odis_h5_write(self._h, int(field), int(slice_), _dp(d))
+        if rc:
+            raise OdisError(rc, "odis_h5_write")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().odis_h5_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 # ---------------------------------------------------------------------------

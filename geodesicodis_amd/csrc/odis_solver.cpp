@@ -6,9 +6,11 @@
 // Kept from the reference: input.in semantics, dt quantisation, initial
 // conditions (NONE / LOAD / ANALYTICAL), output cadence
 // out_freq = totalIter / outputTime with the "DUMPING DATA AT ... AVG DISS:
-// ... GW<k>" line in DATA/OUTPUT.txt, and the end-of-run text checkpoint
-// InitialConditions/{vel,pres}_init.txt in %1.6E.  Not kept: the HDF5
-// DATA/data.h5 writer and the RBF Cartesian velocity output.
+// ... GW<k>" line in DATA/OUTPUT.txt, the DATA/data.h5 fields the input
+// enables (displacement, east / north velocity, RBF Cartesian velocity,
+// per-face dissipation, average dissipation; src/outFiles.cpp:138-681 and
+// src/timeIntegrator.cpp:140-152, 170-200, 280-304), and the end-of-run text
+// checkpoint InitialConditions/{vel,pres}_init.txt in %1.6E.
 #include <sys/stat.h>
 
 #include <cmath>
@@ -217,8 +219,63 @@ static int ab3_explicit_gpu(const Config &c, const MeshOps &M, int precision, co
     log.error(std::string("ERROR: ") + odis_last_error(g.c));
     return rc;
   }
+  // DATA/data.h5 (CreateHDF5Framework at start, DumpData per output)
+  const int fields = (c.out_displacement ? ODIS_H5_DISPLACEMENT : 0) | (c.out_velocity ? ODIS_H5_VELOCITY : 0) |
+                     (c.out_velocity_cart ? ODIS_H5_VELOCITY_XYZ : 0) | (c.out_diss ? ODIS_H5_DISSIPATION : 0) |
+                     (c.diss_avg ? ODIS_H5_DISS_AVG : 0);
+  struct H5Guard {
+    odis_h5 *h = nullptr;
+    ~H5Guard() {
+      if (h) odis_h5_close(h);
+    }
+  } h5;
+  if (fields) {
+    ::mkdir((dir + "/DATA").c_str(), 0775);
+    const int slices = int(std::lround(c.end_time)) * c.output_time + 1;
+    rc = odis_h5_open((dir + "/DATA/data.h5").c_str(), M.N, M.E, std::max(1, slices), fields, M.face_lat.data(),
+                      M.face_lon.data(), &h5.h);
+    if (rc) {
+      log.error(std::string("WARNING: DATA/data.h5 NOT WRITTEN: ") + odis_h5_last_error());
+      h5.h = nullptr;
+    } else if ((fields & ODIS_H5_VELOCITY_XYZ) &&
+               (rc = odis_rbf_set(g.c, 3 * M.N, M.E, M.rbf_row_ptr.data(), M.rbf_col.data(), M.rbf_val.data()))) {
+      log.error(std::string("ERROR: ") + odis_last_error(g.c));
+      return rc;
+    }
+  }
+  auto dump = [&](int k, double total_diss) -> int {
+    if (!h5.h) return ODIS_OK;
+    const int slice = k - 1;
+    std::vector<double> uv(size_t(2) * M.E), xyz(size_t(3) * M.N), s_v(M.E), s_dv(size_t(3) * M.E),
+        s_eta(M.N), s_de(size_t(3) * M.N);
+    int r = ODIS_OK;
+    long long it_now = 0;
+    if ((fields & ODIS_H5_DISPLACEMENT) &&
+        !(r = odis_get_state(g.c, 0, s_v.data(), s_dv.data(), s_eta.data(), s_de.data(), &it_now)))
+      r = odis_h5_write(h5.h, ODIS_H5_DISPLACEMENT, slice, s_eta.data());
+    if (!r && (fields & (ODIS_H5_VELOCITY | ODIS_H5_DISSIPATION)))
+      r = odis_get_velocity_en(g.c, 0, M.face_normal.data(), uv.data());
+    if (!r && (fields & ODIS_H5_VELOCITY)) r = odis_h5_write(h5.h, ODIS_H5_VELOCITY, slice, uv.data());
+    if (!r && (fields & ODIS_H5_DISSIPATION)) {
+      // updateEnergy per-face flux (src/energy.cpp:13-62)
+      std::vector<double> e(M.E);
+      for (int f = 0; f < M.E; f++) {
+        const double q = uv[2 * f] * uv[2 * f] + uv[2 * f + 1] * uv[2 * f + 1];
+        e[f] = c.fric_type == ODIS_LINEAR ? c.alpha * 1000.0 * c.h * q : c.alpha / c.h * std::sqrt(q) * q;
+      }
+      r = odis_h5_write(h5.h, ODIS_H5_DISSIPATION, slice, e.data());
+    }
+    if (!r && (fields & ODIS_H5_VELOCITY_XYZ) && !(r = odis_velocity_xyz(g.c, 0, xyz.data())))
+      r = odis_h5_write(h5.h, ODIS_H5_VELOCITY_XYZ, slice, xyz.data());
+    if (!r && (fields & ODIS_H5_DISS_AVG)) r = odis_h5_write(h5.h, ODIS_H5_DISS_AVG, slice, &total_diss);
+    if (r == ODIS_E_RANGE) return ODIS_OK;  // more outputs than the file's slices (a partial last orbit)
+    if (r) log.error(std::string("ERROR: DATA/data.h5 output ") + std::to_string(k));
+    return r;
+  };
   int out_count = 1;
-  log.msg(dump_line(0.0, diss * to_gw, out_count++));
+  log.msg(dump_line(0.0, diss * to_gw, out_count));
+  if ((rc = dump(out_count, diss))) return rc;
+  out_count++;
   const long long total = (long long)std::ceil(double(n_iter) * c.end_time);
   const int out_freq = std::max(1, n_iter / std::max(1, c.output_time));
   long long iter = 0;
@@ -241,7 +298,9 @@ static int ab3_explicit_gpu(const Config &c, const MeshOps &M, int precision, co
         log.error(std::string("ERROR: ") + odis_last_error(g.c));
         return rc;
       }
-      log.msg(dump_line(dt * double(iter) / P, diss * to_gw, out_count++));
+      log.msg(dump_line(dt * double(iter) / P, diss * to_gw, out_count));
+      if ((rc = dump(out_count, diss))) return rc;
+      out_count++;
     }
   }
   long long it = 0;

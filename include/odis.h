@@ -167,6 +167,30 @@ int odis_rbf_set(odis_ctx *ctx, int n_rows, int n_cols, const int *row_ptr, cons
                  const double *val);
 int odis_velocity_xyz(odis_ctx *ctx, int inst, double *xyz);
 
+/* DATA/data.h5 in the reference's layout (OutFiles::CreateHDF5Framework,
+ * DumpGridData, DumpData, src/outFiles.cpp:138-681): float32 datasets
+ * [n_slices][n] per field, slice k - 1 holding output k, the face centres in
+ * degrees as "face longitude" / "face latitude" (when face_lat / face_lon,
+ * radians, are given).  Fields (a mask at open, one field per write):
+ *   ODIS_H5_DISPLACEMENT  "displacement" [N]: data eta[N]
+ *   ODIS_H5_VELOCITY      "east velocity", "north velocity" [E]: data uv[E][2]
+ *   ODIS_H5_VELOCITY_XYZ  "x velocity", "y velocity", "z velocity" [N]: data xyz[N][3]
+ *   ODIS_H5_DISSIPATION   "dissipated energy" [E]: data e[E]
+ *   ODIS_H5_DISS_AVG      "dissipation avg output" (1-D): data[0]
+ * The HDF5 C library is loaded at the first open (ODIS_E_UNSUPPORTED when
+ * it cannot be; odis_h5_last_error says why).  Host-only. */
+typedef struct odis_h5 odis_h5;
+#define ODIS_H5_DISPLACEMENT 1
+#define ODIS_H5_VELOCITY     2
+#define ODIS_H5_VELOCITY_XYZ 4
+#define ODIS_H5_DISSIPATION  8
+#define ODIS_H5_DISS_AVG     16
+int odis_h5_open(const char *path, int n_cells, int n_faces, int n_slices, int fields,
+                 const double *face_lat, const double *face_lon, odis_h5 **out);
+int odis_h5_write(odis_h5 *h, int field, int slice, const double *data);
+int odis_h5_close(odis_h5 *h);
+const char *odis_h5_last_error(void);
+
 /* Self-gravity / ice-shell coupling through a spherical-harmonic projection
  * of eta (the reference's commented design: CalcLegendreFuncs,
  * src/mesh.cpp:2154-2259, and pressureGradientSH, src/spatialOperators.cpp:

@@ -628,3 +628,71 @@ def test_velocity_xyz_matches_oracle(product, oracle, precision):
     err = np.abs(ctx.velocity_xyz(0) - U).max() / np.abs(U).max()
     assert err < 3e-3, err
     ctx.close()
+
+
+CASE_INPUT = """radius; 252.1e3; top radius;
+k2; 0.0; k2;
+h2; 0.0; h2;
+love reduction factor; 1.0; set by FREE;
+ANGULAR VELOCITY; 5.307e-5; mean motion;
+surface gravity; 0.113; g;
+eccentricity; 0.0047; e;
+obliquity; 0.0; degrees;
+ocean thickness; 38e3; h_o;
+friction coefficient; 1e-7; alpha;
+friction type; LINEAR; friction;
+potential; ECC; potential;
+surface type; FREE; surface;
+advection; false; advection;
+solver type; AB3; solver;
+geodesic grid level; 4; level;
+output time; 4; outputs per orbit;
+initial conditions; NONE; ic;
+simulation end time; 1; orbits;
+time step; 60; dt target;
+displacement output; true; data out opt;
+velocity output; true; data out opt;
+velocity cartesian output; true; data out opt;
+dissipation output; true; data out opt;
+dissipation avg output; true; data out opt;
+"""
+
+
+def test_run_case_writes_reference_outputs(product, tmp_path):
+    """The standalone driver (solveODIS -> ab3Explicit shape) on a 642-cell
+    case: the reference's OUTPUT.txt log lines, DATA/data.h5 with one slice
+    per output (displacement, east/north and Cartesian velocity, per-face and
+    average dissipation), and the end-of-run checkpoint.  The last
+    displacement slice is the checkpoint's eta; every average-dissipation
+    slice matches its log line and the area-weighted per-face dissipation."""
+    from conftest import H5DUMP, h5_read
+    import os
+    if not os.path.isfile(H5DUMP):
+        pytest.skip("h5dump not installed")
+    G = product
+    (tmp_path / "input.in").write_text(CASE_INPUT)
+    assert G.run_case(tmp_path) == 0
+    lines = [l for l in (tmp_path / "DATA" / "OUTPUT.txt").read_text().splitlines() if "DUMPING DATA AT" in l]
+    assert len(lines) == 5
+    gw = np.array([float(l.split("AVG DISS:")[1].split("GW")[0]) for l in lines])
+    h5 = tmp_path / "DATA" / "data.h5"
+    assert h5.is_file()
+    eta = h5_read(h5, "displacement")
+    assert eta.shape == (5, 642)
+    ck = np.loadtxt(tmp_path / "InitialConditions" / "pres_init.txt", delimiter=",")[:, 0]
+    assert np.abs(eta[-1] - ck).max() <= 1e-5 * np.abs(ck).max()
+    avg = h5_read(h5, "dissipation avg output").astype(np.float64)
+    r = 252.1e3
+    assert np.all(np.abs(avg * 4 * np.pi * r * r / 1e9 - gw) <= 1e-5 * np.abs(gw).max())
+    g, _ = generated_grid(4)
+    area = G.Mesh(g, r).get("face_area")
+    ed = h5_read(h5, "dissipated energy").astype(np.float64)
+    assert ed.shape == (5, 1920)
+    assert np.all(np.abs(ed @ area / (4 * np.pi * r * r) - avg) <= 1e-5 * np.abs(avg).max())
+    for nm in ("east velocity", "north velocity"):
+        assert h5_read(h5, nm).shape == (5, 1920)
+    for nm in ("x velocity", "y velocity", "z velocity"):
+        x = h5_read(h5, nm)
+        assert x.shape == (5, 642) and np.all(np.isfinite(x)) and np.abs(x[-1]).max() > 0
+    lat = h5_read(h5, "face latitude")
+    assert lat.shape == (1920,) and np.abs(lat).max() <= 90.0

@@ -283,3 +283,47 @@ def test_rbf_operator_matches_oracle(product, oracle, level, tol):
 def test_rbf_operator_matches_oracle_reference_grids(product, oracle, level, tol):
     path = os.path.join(REFERENCE_INPUTS, "grid_l%d.txt" % level)
     _rbf_check(product, oracle, product.Grid.read(path), oracle.read_reference_grid(path), tol)
+
+
+def test_h5_writer_reference_layout(product, tmp_path):
+    """DATA/data.h5 in the reference's layout (src/outFiles.cpp:138-681):
+    float32 [slices][n] datasets under the reference's names, output k at row
+    k - 1, unwritten rows zero, the face centres in degrees."""
+    from conftest import H5DUMP, h5_read
+    if not os.path.isfile(H5DUMP):
+        pytest.skip("h5dump not installed")
+    G = product
+    N, E, S = 7, 15, 4
+    rng = np.random.default_rng(5)
+    flat, flon = rng.uniform(-1.5, 1.5, E), rng.uniform(0, 6.2, E)
+    path = tmp_path / "data.h5"
+    try:
+        f = G.H5File(path, N, E, S, G.H5_DISPLACEMENT | G.H5_VELOCITY | G.H5_VELOCITY_XYZ | G.H5_DISSIPATION |
+                     G.H5_DISS_AVG, flat, flon)
+    except G.OdisError as e:
+        if e.code == -4:  # ODIS_E_UNSUPPORTED
+            pytest.skip("HDF5 C library not loadable")
+        raise
+    eta, uv, xyz, ed = rng.normal(size=N), rng.normal(size=(E, 2)), rng.normal(size=(N, 3)), rng.random(E)
+    f.write(G.H5_DISPLACEMENT, 1, eta)
+    f.write(G.H5_VELOCITY, 2, uv)
+    f.write(G.H5_VELOCITY_XYZ, 3, xyz)
+    f.write(G.H5_DISSIPATION, 0, ed)
+    f.write(G.H5_DISS_AVG, 3, [2.5e-6])
+    with pytest.raises(G.OdisError):
+        f.write(G.H5_DISPLACEMENT, S, eta)
+    f.close()
+    f32 = lambda a: np.asarray(a, np.float64).astype(np.float32)
+    d = h5_read(path, "displacement")
+    assert d.shape == (S, N) and np.array_equal(d[1], f32(eta)) and not d[[0, 2, 3]].any()
+    assert np.array_equal(h5_read(path, "east velocity")[2], f32(uv[:, 0]))
+    assert np.array_equal(h5_read(path, "north velocity")[2], f32(uv[:, 1]))
+    for c, nm in enumerate(("x velocity", "y velocity", "z velocity")):
+        assert np.array_equal(h5_read(path, nm)[3], f32(xyz[:, c])), nm
+    assert np.array_equal(h5_read(path, "dissipated energy")[0], f32(ed))
+    avg = h5_read(path, "dissipation avg output")
+    assert avg.shape == (S,) and avg[3] == np.float32(2.5e-6)
+    lat = h5_read(path, "face latitude")
+    assert np.array_equal(lat, (flat.astype(np.float32).astype(np.float64) * (180. / np.pi)).astype(np.float32))
+    assert np.array_equal(h5_read(path, "face longitude"),
+                          (flon.astype(np.float32).astype(np.float64) * (180. / np.pi)).astype(np.float32))
