@@ -285,18 +285,20 @@ def test_interp_gg2ll(product, oracle, precision):
 # spherical-harmonic self-gravity / ice-shell coupling
 # ---------------------------------------------------------------------------
 
-@pytest.mark.parametrize("mode,tol", [(1, 1e-11), (2, 2e-5), (3, 1e-11)])
-def test_sh_synthesis_and_projection(product, oracle, mode, tol):
+@pytest.mark.parametrize("mode,tol,L,level", [(1, 1e-11, 12, 5), (2, 2e-5, 12, 5), (3, 1e-11, 12, 5),
+                                              (3, 1e-10, 32, 6)])
+def test_sh_synthesis_and_projection(product, oracle, mode, tol, L, level):
     """Synthesis of unit coefficients reproduces scipy's harmonics on the
     nodes; the GPU least-squares projection recovers band-limited random
     coefficients (stored FP64 basis and on-the-fly recurrence: rounding
-    level; split-BF16: ~1e-6)."""
+    level; split-BF16: ~1e-6).  Degree 12 runs the runtime-degree kernels,
+    degree 32 (10242 nodes) the compiled pair analysis and chain synthesis
+    of the benchmark."""
     G, O = product, oracle
-    g, arrs = generated_grid(5)
+    g, arrs = generated_grid(level)
     p, _ = G.make_params()
     mesh = G.Mesh(g, p.radius)
     ctx = G.Context(mesh, p)
-    L = 12
     ctx.set_sh(L, None, mode=mode)
     Y = O.real_sh_basis(L, arrs["lat"], arrs["lon"])
     K = (L + 1) ** 2
@@ -560,12 +562,12 @@ def test_one_kernel_step_matches_face_cell_pair(product, precision):
             _assert_same_state(one, pair, (0, 1), (level, "patch", cells, lanes, blocks))
 
 
-@pytest.mark.parametrize("precision,L", [(0, 16), (1, 16), (0, 12), (1, 12)])
+@pytest.mark.parametrize("precision,L", [(0, 16), (1, 16), (0, 12), (1, 12), (0, 8)])
 def test_sh_recurrence_matches_stored_basis(product, precision, L):
     """The basis evaluated on the fly and the stored basis give the same
     coupled trajectory: to rounding in FP64, within the FP32 bound in FP32
-    (40962 cells, two instances with different shell factors).  Degree 16
-    runs the compiled pair analysis and chain synthesis, degree 12 the
+    (40962 cells, two instances with different shell factors).  Degrees 16
+    and 8 run the compiled pair analysis and chain synthesis, degree 12 the
     runtime-degree kernels."""
     G = product
     g, _ = generated_grid(6)
