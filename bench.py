@@ -245,6 +245,67 @@ def sweep_run(G, args):
             "finite": ok}
 
 
+UNEVEN_SEED = 20261204
+UNEVEN_SH_DEGREE = 16
+
+
+def uneven_run(G, args):
+    """Uneven batch: args.uneven_instances instances, each drawing (level,
+    obliquity, quadratic-drag coefficient, surface) in that order from
+    numpy.random.default_rng(20261204): grid level from {4, 5, 6, 7} of the
+    reference's numbering (2562 .. 163842 cells), FULL tide (e = 0.0047,
+    obliquity U(0, 1) degrees), quadratic drag c_D = U(1e-3, 5e-3), and a
+    free surface or a 20 km LID_MEMBR shell (SH coupling to degree 16) with
+    probability 1/2.  Instances on one grid and surface share a context
+    (the kernels take them along their grid's y axis); each instance's dt is
+    the quantised step below 0.35 / omega_max, omega_max the larger of the
+    grid-scale gravity-wave frequency (0.0193 s^-1 at 10242 cells, scaling
+    as sqrt(N)) and, under the shell, the degree-16 shell-mode frequency
+    sqrt(g h beta_l l (l + 1)) / R.  FP32 state; the contexts are stepped in
+    turn on one stream."""
+    import numpy as np
+    rng = np.random.default_rng(UNEVEN_SEED)
+    groups = {}
+    for _ in range(args.uneven_instances):
+        level = int(rng.choice([4, 5, 6, 7]))
+        theta = rng.uniform(0.0, 1.0)
+        cd = rng.uniform(1e-3, 5e-3)
+        lid = bool(rng.uniform() < 0.5)
+        groups.setdefault((level, lid), []).append((theta, cd))
+    L = UNEVEN_SH_DEGREE
+    beta = np.array(G.surface_beta(G.LID_MEMBR, L, 252.1e3, 0.113, 20e3, 38e3))
+    ls = np.arange(L + 1)
+    w_shell = float(np.max(np.sqrt(0.1166 * 38e3 * np.maximum(beta, 1.0) * ls * (ls + 1)) / 232.1e3))
+    total_cells, elapsed, n_steps, rows = 0, 0.0, max(10, args.steps // 40), []
+    for (level, lid), inst in sorted(groups.items()):
+        grid = G.Grid.generate(level + 1, 64)
+        n_cells = grid.n
+        w = 0.0193 * (n_cells / 10242.0) ** 0.5
+        if lid:
+            w = max(w, w_shell)
+        ps = []
+        for theta, cd in inst:
+            kw = dict(tide=G.FULL, obliquity=np.radians(theta), friction=G.QUADRATIC, alpha=cd, quad_drag=1,
+                      time_step=0.35 / w)
+            if lid:
+                kw.update(radius=232.1e3, g=0.1166, surface=G.LID_MEMBR, shell_thickness=20e3, love_reduct=0.99)
+            ps.append(G.make_params(**kw)[0])
+        mesh = G.Mesh(grid, ps[0].radius)
+        ctx = G.Context(mesh, ps, precision=G.FP32)
+        if lid:
+            ctx.set_sh(L, np.tile(beta, (len(ps), 1)), mode=G.SH_RECUR)
+        el, _ = timed_run(ctx, n_steps, 3, 0)
+        ok = bool(all(abs(x) < float("inf") for x in ctx.dissipation()))
+        ctx.close()
+        elapsed += el
+        total_cells += n_cells * len(ps)
+        rows.append({"cells": n_cells, "surface": "LID_MEMBR" if lid else "FREE", "instances": len(ps),
+                     "ms_per_step": 1e3 * el / n_steps, "finite": ok})
+    return {"instances": args.uneven_instances, "seed": UNEVEN_SEED, "precision": "fp32 state, fp64 dissipation",
+            "value": total_cells * n_steps / elapsed, "ms_per_step_all_groups": 1e3 * elapsed / n_steps,
+            "steps": n_steps, "groups": rows}
+
+
 def converge_run(G, args):
     """Time to converged orbit-averaged dissipation for one instance of the
     plain-method case (Enceladus free surface, 38 km, ECC, linear drag
@@ -328,6 +389,8 @@ def main():
     ap.add_argument("--max-orbits", type=int, default=3000, help="orbit cap of the converge workload")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched-sweep extra")
     ap.add_argument("--sweep-instances", type=int, default=256, help="instances of the batched-sweep extra")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch extra")
+    ap.add_argument("--uneven-instances", type=int, default=512, help="instances of the uneven-batch extra")
     ap.add_argument("--level", type=int, default=8, help="reference grid level (8 = 163842 cells)")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
                     help="precision of the headline value (fp32: FP32 stencils, split-BF16 SH basis)")
@@ -546,6 +609,8 @@ def main():
 
     if not args.no_extra and not args.no_sweep:
         out["config"]["sweep_fp32"] = sweep_run(G, args)
+    if not args.no_extra and not args.no_uneven:
+        out["config"]["uneven_fp32"] = uneven_run(G, args)
 
     if not args.no_cpu_baseline:
         import oracle as O
