@@ -327,3 +327,30 @@ def test_h5_writer_reference_layout(product, tmp_path):
     assert np.array_equal(lat, (flat.astype(np.float32).astype(np.float64) * (180. / np.pi)).astype(np.float32))
     assert np.array_equal(h5_read(path, "face longitude"),
                           (flon.astype(np.float32).astype(np.float64) * (180. / np.pi)).astype(np.float32))
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on exactly one GPU: --gpus 2 exits with a message and a
+    non-zero status before touching a device."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in r.stderr
+
+
+def test_sweep_draws_are_seeded(product):
+    """The batched sweep's instance draws are the seeded ones: thickness
+    10^U(3,5) m and drag 10^U(-11,-7) s^-1 in draw order, and every dt is
+    below 0.35 / omega_max(h)."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    ps = bench.sweep_params(product, 8)
+    rng = np.random.default_rng(bench.SWEEP_SEED)
+    for p in ps:
+        h, a = 10.0 ** rng.uniform(3, 5), 10.0 ** rng.uniform(-11, -7)
+        assert p.h == h and p.alpha == a
+        assert p.dt * bench.OMEGA_MAX_40962_38KM * (h / 38e3) ** 0.5 <= 0.35
