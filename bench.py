@@ -196,6 +196,17 @@ def stencil_roofline(G, ctx, mesh, tm, p_bytes, one):
     return "k_face", fb, tm["face_ms"], {"face": fb, "cell": cb}
 
 
+_GRIDS = {}
+
+
+def generated_grid(G, level):
+    """The generated grid of a level (icosahedral bisection + 64 Lloyd
+    sweeps), built once per run."""
+    if level not in _GRIDS:
+        _GRIDS[level] = G.Grid.generate(level, 64)
+    return _GRIDS[level]
+
+
 SWEEP_SEED = 20261203
 # largest gravity-wave frequency at 40962 cells for a 38 km Enceladus ocean
 # (twice the 0.0193 s^-1 measured at 10242 cells); it scales as sqrt(h)
@@ -224,7 +235,7 @@ def sweep_run(G, args):
     context (every kernel takes the instances along its grid's y axis),
     against one plain FP64 instance of the median thickness on the same
     grid."""
-    grid = G.Grid.generate(7, 64)
+    grid = generated_grid(G, 7)
     ps = sweep_params(G, args.sweep_instances)
     mesh = G.Mesh(grid, ps[0].radius)
     n_steps, n_warm = max(20, args.steps // 10), 10
@@ -278,7 +289,7 @@ def uneven_run(G, args):
     w_shell = float(np.max(np.sqrt(0.1166 * 38e3 * np.maximum(beta, 1.0) * ls * (ls + 1)) / 232.1e3))
     total_cells, elapsed, n_steps, rows = 0, 0.0, max(10, args.steps // 40), []
     for (level, lid), inst in sorted(groups.items()):
-        grid = G.Grid.generate(level + 1, 64)
+        grid = generated_grid(G, level + 1)
         n_cells = grid.n
         w = 0.0193 * (n_cells / 10242.0) ** 0.5
         if lid:
@@ -315,7 +326,7 @@ def converge_run(G, args):
     args.max_orbits (capped).  Time-to-converged: wall time until
     |D_k - D*| / D* <= 1 % and it stays so for 3 orbits, D* of plain FP64."""
     import numpy as np
-    grid = G.Grid.generate(6, 64)
+    grid = generated_grid(G, 6)
     # 4200 steps per orbit: the quantised 30 s target (4000) is unstable on this grid
     p, n_s = G.make_params(alpha=args.alpha, time_step=28.2)
     mesh = G.Mesh(grid, p.radius)
@@ -446,7 +457,7 @@ def main():
         return
 
     t0 = time.time()
-    grid = G.Grid.generate(args.level, 64)
+    grid = generated_grid(G, args.level)
     shell = args.workload == "shell"
     if shell:
         p, steps_per_orbit, beta = shell_setup(G)
