@@ -23,7 +23,13 @@ stored FP64 basis instead).  The same line carries:
     stored split-BF16 basis with --sh-basis stored), with its error against
     FP64 after the same steps;
   - the free-surface ocean without the shell (workload "free": dt = P/18300,
-    where the stencils alone are timed).
+    where the stencils alone are timed);
+  - a batched FP32 sweep: 256 instances on the 40962-cell grid, seeded
+    thickness and drag draws, against one plain FP64 instance;
+  - an uneven FP32 batch: 512 instances over the 2562- to 163842-cell
+    grids, FULL tide, quadratic drag, half under a shell with SH coupling.
+`--workload converge` instead times the plain-method case (10242 cells) to
+converged orbit-averaged dissipation, plain FP64 and FP32.
 
 Prints one JSON line: cell-steps/s of the whole job; the roofline of the
 dominant kernel (HIP events stamped on the kernel dispatches of every 16th
