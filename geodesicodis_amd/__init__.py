@@ -41,7 +41,7 @@ EXPORTS = [
     "odis_velocity_xyz", "odis_h5_open", "odis_h5_write", "odis_h5_close", "odis_h5_last_error",
 ]
 H5_DISPLACEMENT, H5_VELOCITY, H5_VELOCITY_XYZ, H5_DISSIPATION, H5_DISS_AVG = 1, 2, 4, 8, 16
-SH_OFF, SH_FP64, SH_SPLIT_BF16, SH_RECUR = 0, 1, 2, 3
+SH_OFF, SH_FP64, SH_SPLIT_BF16, SH_RECUR, SH_MFMA = 0, 1, 2, 3, 4
 OPT_GRAPHS, OPT_COMPACT, OPT_ONE_KERNEL, OPT_PATCH_CELLS, OPT_STEP_BLOCKS, OPT_STEP_LANES, OPT_SH_CELLS = \
     1, 2, 3, 4, 5, 6, 7
 
@@ -435,7 +435,8 @@ class Context:
     def set_sh(self, l_max, beta=None, mode=SH_FP64):
         """Self-gravity / shell coupling: beta is (n_inst, l_max + 1) or
         (l_max + 1,) per-degree factors; mode SH_OFF / SH_FP64 / SH_SPLIT_BF16 /
-        SH_RECUR (basis evaluated on the fly)."""
+        SH_RECUR (basis evaluated on the fly) / SH_MFMA (batched split-BF16
+        matrix products)."""
         b = None
         if beta is not None:
             b = np.ascontiguousarray(np.broadcast_to(np.asarray(beta, np.float64),

@@ -556,6 +556,18 @@ __device__ __forceinline__ void put_mmajor(int K, int b, int k, double ck, const
   }
 }
 
+// coefficient k of instance b -> its place in the fragment-ordered copy of
+// the batched products, Cs[b][k / 32][(k / 8) % 4] = 8 hi, then 8 lo bf16
+// of float(c)
+__device__ __forceinline__ void put_split(int KB, int b, int k, double ck, unsigned short *cs) {
+  const float f = float(ck);
+  const __bf16 h = (__bf16)f;
+  const __bf16 l = (__bf16)(f - (float)h);
+  unsigned short *p = cs + ((size_t(b) * KB + (k >> 5)) * 4 + ((k >> 3) & 3)) * 16 + (k & 7);
+  p[0] = __builtin_bit_cast(unsigned short, h);
+  p[8] = __builtin_bit_cast(unsigned short, l);
+}
+
 // b_j = sum_q b_part[q][j] over the n_part analysis partials of instance b
 // ([B][n_part][K]), added in q order; written after the partials of every
 // instance.  One lane per j; the loads of 32 partials are issued before
@@ -581,10 +593,10 @@ __global__ __launch_bounds__(kBlock) void k_sh_bsum(int K, int n_part, double *b
 // c_k = D_k sum_j Minv[k][j] b_j: one wavefront per k; each lane loads its
 // row entries and b entries (32 of each in flight) and the wave sums in a
 // fixed order.  cmaj (nullable) also receives c in m-major order
-// (put_mmajor).
+// (put_mmajor), csplit (nullable) its split copy (put_split).
 __global__ __launch_bounds__(kBlock) void k_sh_coef(int K, const double *__restrict__ Minv, const double *D,
                                                     const double *__restrict__ bsum, double *c, const double *chain,
-                                                    double *cmaj) {
+                                                    double *cmaj, unsigned short *csplit, int KB) {
   const int b = blockIdx.y;
   const int lane = threadIdx.x & 63;
   const int k = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
@@ -608,6 +620,7 @@ __global__ __launch_bounds__(kBlock) void k_sh_coef(int K, const double *__restr
   const double ck = (D ? D[size_t(b) * K + k] : 1.0) * acc;
   c[size_t(b) * K + k] = ck;
   if (cmaj) put_mmajor(K, b, k, ck, chain, cmaj);
+  if (csplit) put_split(KB, b, k, ck, csplit);
 }
 
 // c (index k = l l + l + m) -> its m-major copy, one lane per k
@@ -1246,6 +1259,180 @@ static int4 synth_split(int L) {
   return make_int4(cut[0], std::max(cut[0], cut[1]), std::max(cut[1], cut[2]), L + 1);
 }
 
+// ---------------------------------------------------------------------------
+// the projection of a batch as split-BF16 matrix products (MFMA)
+// ---------------------------------------------------------------------------
+// For nb instances on one grid the analysis B = Y X (K x nb, X = eta of the
+// batch, N x nb) and the synthesis X_eff = X + Y^T C are products with one
+// inner dimension N or Kp.  Each operand is split as x = hi + lo, hi and lo
+// bf16, and a product is hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_bf16
+// with FP32 accumulation (the lo*lo term, below 2^-16 relative, is dropped).
+//
+// Operand fragments of a 16x16x32 product: lane l holds row (or column)
+// l % 16 and eight consecutive entries 8 (l / 16) .. + 7 of the inner
+// dimension, for A and B alike; the result tile holds, in lane l, rows
+// 4 (l / 16) .. + 3 of column l % 16.  The basis is stored twice, each copy
+// in the fragment order of its pass, as 16 bf16 per lane (8 hi, then 8 lo):
+//   Ya[h][q][lane]  analysis B operand: harmonic 16 h + l % 16, cells
+//                   32 q + 8 (l / 16) + i;
+//   Ys[t][kb][lane] synthesis A operand: cell 16 t + l % 16, harmonics
+//                   32 kb + 8 (l / 16) + i;
+// harmonics k >= K (Kp = K rounded up to 32) and cells >= N are zero.  The
+// coefficients go to Cs[j][kb][l / 16] (instance j, same harmonic order),
+// written by k_sh_coef.  Both passes read the basis once per step for the
+// whole batch; instance tiles past nb issue no products.
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ floatx4 mfma3(const bf16x8 &ahi, const bf16x8 &alo, const bf16x8 &bhi,
+                                         const bf16x8 &blo, floatx4 acc) {
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, bhi, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, blo, acc, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bhi, acc, 0, 0, 0);
+}
+
+// hi = bf16(x) and lo = bf16(x - hi), both rounded to nearest even
+__device__ __forceinline__ void split8(const float (&f)[8], bf16x8 &hi, bf16x8 &lo) {
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const __bf16 h = (__bf16)f[i];
+    hi[i] = h;
+    lo[i] = (__bf16)(f[i] - (float)h);
+  }
+}
+
+// analysis: grid (n_rng cell ranges of qpr blocks of 32 cells, harmonic
+// groups of 4 HPW tiles, instance groups of 16 NIT); wave w of a workgroup
+// owns HPW harmonic tiles and every instance tile of the group, and writes
+// its sums over the range as partial n_rng of b_part ([nb][n_part][K]).
+template <typename T, int NIT, int HPW>
+__global__ __launch_bounds__(256) void k_sh_ana_mfma(int N, int K, int KT, int NB32, int qpr, int n_part,
+                                                     const uint4 *__restrict__ Ya, const T *__restrict__ x,
+                                                     long long x_stride, int nb, double *__restrict__ b_part) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, r16 = lane & 15;
+  const int rng = blockIdx.x;
+  const int h0 = (blockIdx.y * 4 + w) * HPW;
+  if (h0 >= KT) return;  // a whole wave without tiles (no barrier below)
+  const int j0 = blockIdx.z * NIT * 16;
+  const int nit = min(NIT, (nb - j0 + 15) / 16);
+  const int q0 = rng * qpr, q1 = min(NB32, q0 + qpr);
+  floatx4 acc[HPW][NIT];
+#pragma unroll
+  for (int t = 0; t < HPW; ++t)
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) acc[t][it] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int q = q0; q < q1; ++q) {
+    bf16x8 ahi[NIT], alo[NIT];
+    const int c0 = q * 32 + 8 * g;
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      if (it >= nit) break;
+      const int j = j0 + it * 16 + r16;
+      float f[8];
+      const T *xr = x + (long long)min(j, nb - 1) * x_stride;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) f[i] = (j < nb && c0 + i < N) ? float(xr[c0 + i]) : 0.f;
+      split8(f, ahi[it], alo[it]);
+    }
+#pragma unroll
+    for (int t = 0; t < HPW; ++t) {
+      const int h = h0 + t;
+      if (h >= KT) break;
+      const uint4 *fp = Ya + ((size_t(h) * NB32 + q) * 64 + lane) * 2;
+      const bf16x8 bhi = __builtin_bit_cast(bf16x8, fp[0]), blo = __builtin_bit_cast(bf16x8, fp[1]);
+#pragma unroll
+      for (int it = 0; it < NIT; ++it) {
+        if (it >= nit) break;
+        acc[t][it] = mfma3(ahi[it], alo[it], bhi, blo, acc[t][it]);
+      }
+    }
+  }
+  // result tile: rows = instances j0 + 16 it + 4 g + r, column = harmonic 16 h + r16
+#pragma unroll
+  for (int t = 0; t < HPW; ++t) {
+    const int h = h0 + t, k = 16 * h + r16;
+    if (h >= KT) break;
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      if (it >= nit) break;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int j = j0 + it * 16 + 4 * g + r;
+        if (j < nb && k < K) b_part[(size_t(j) * n_part + rng) * K + k] = double(acc[t][it][r]);
+      }
+    }
+  }
+}
+
+// synthesis: grid (cell tiles in groups of 4 CT, instance groups of 16 NIT);
+// wave w owns CT tiles of 16 cells and every instance tile of the group:
+// out = base + Y^T C over all Kp harmonics.
+template <typename T, int NIT, int CT>
+__global__ __launch_bounds__(256) void k_sh_syn_mfma(int N, int NT16, int KB, const uint4 *__restrict__ Ys,
+                                                     const uint4 *__restrict__ Cs, int nb, const T *base,
+                                                     long long stride, T *out) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, r16 = lane & 15;
+  const int t0 = (blockIdx.x * 4 + w) * CT;
+  if (t0 >= NT16) return;
+  const int j0 = blockIdx.y * NIT * 16;
+  const int nit = min(NIT, (nb - j0 + 15) / 16);
+  floatx4 acc[CT][NIT];
+#pragma unroll
+  for (int u = 0; u < CT; ++u)
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) acc[u][it] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int kb = 0; kb < KB; ++kb) {
+    bf16x8 bhi[NIT], blo[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      if (it >= nit) break;
+      const int j = min(j0 + it * 16 + r16, nb - 1);  // columns past nb are never stored
+      const uint4 *cp = Cs + ((size_t(j) * KB + kb) * 4 + g) * 2;
+      bhi[it] = __builtin_bit_cast(bf16x8, cp[0]);
+      blo[it] = __builtin_bit_cast(bf16x8, cp[1]);
+    }
+#pragma unroll
+    for (int u = 0; u < CT; ++u) {
+      const int t = t0 + u;
+      if (t >= NT16) break;
+      const uint4 *fp = Ys + ((size_t(t) * KB + kb) * 64 + lane) * 2;
+      const bf16x8 ahi = __builtin_bit_cast(bf16x8, fp[0]), alo = __builtin_bit_cast(bf16x8, fp[1]);
+#pragma unroll
+      for (int it = 0; it < NIT; ++it) {
+        if (it >= nit) break;
+        acc[u][it] = mfma3(ahi, alo, bhi[it], blo[it], acc[u][it]);
+      }
+    }
+  }
+  // result tile: rows = cells 16 t + 4 g + r, column = instance j0 + 16 it + r16
+#pragma unroll
+  for (int u = 0; u < CT; ++u) {
+    const int t = t0 + u;
+    if (t >= NT16) break;
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      if (it >= nit) break;
+      const int j = j0 + it * 16 + r16;
+      if (j >= nb) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int c = 16 * t + 4 * g + r;
+        if (c >= N) continue;
+        const double add = base ? double(base[j * stride + c]) : 0.0;
+        out[j * stride + c] = T(add + double(acc[u][it][r]));
+      }
+    }
+  }
+}
+
+// c (index k = l l + l + m) -> its split copy, one lane per k
+__global__ __launch_bounds__(kBlock) void k_sh_to_split(int K, int KB, const double *c, unsigned short *cs) {
+  const int b = blockIdx.y;
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k < K) put_split(KB, b, k, c[size_t(b) * K + k], cs);
+}
+
 // Gram matrix M = Yt Yt^T (K x K), 32 x 32 output tiles, FP64.
 __global__ __launch_bounds__(kBlock) void k_gram(int K, int N, const double *Yt, double *M) {
   __shared__ double A[32][33], Bm[32][33];
@@ -1401,6 +1588,9 @@ struct odis_ctx {
   double *sh_tab = nullptr, *sh_trig = nullptr;  // recurrence table, [4][N] sin/cos lat, cos/sin lon
   double *sh_tabm = nullptr, *sh_cmaj = nullptr; // m-major table, m-major coefficients [B][2 (L+1)(L+2)]
   double *sh_chain = nullptr;                    // rescaled recurrence (beta, S) per (l, m), then C_m
+  uint4 *sh_Ya = nullptr, *sh_Ys = nullptr;      // batched products: split basis in fragment order
+  unsigned short *sh_Cs = nullptr;               // batched products: split coefficients [B][Kp / 32][4][16]
+  int sh_Kp = 0, sh_qpr = 0, sh_rng = 0;         // batched products: padded K, 32-cell blocks per range, ranges
   int sh_part = 0;                               // capacity of analysis partials per instance
   int sh_cpt = 0;                                // cells per lane of the on-the-fly analysis (0: auto)
   void *eta_eff = nullptr;  // [B][N] of the stencil precision
@@ -2180,6 +2370,53 @@ static void launch_analysis_pair(odis_ctx *ctx, int nb, const T *x, long long st
                         ctx->sh_trig, x, stride, cpt, n_cblk, n_logical, ctx->sh_b);
 }
 
+// Batched products: instance tiles per wave (NIT) and harmonic (HPW) or
+// cell (CT) tiles per wave, from the batch size: at most 64 accumulators
+// per lane.  The ranges of the analysis depend on N only, so an instance's
+// coefficients do not depend on the rest of its batch.
+template <typename T, int NIT, int HPW>
+static void launch_ana_mfma_t(odis_ctx *ctx, int nb, const T *x, long long stride, hipEvent_t e0, hipEvent_t e1) {
+  const int KT = ctx->sh_Kp / 16, NB32 = (ctx->N + 31) / 32;
+  const dim3 grid(ctx->sh_rng, (KT + 4 * HPW - 1) / (4 * HPW), (nb + 16 * NIT - 1) / (16 * NIT));
+  hipExtLaunchKernelGGL((k_sh_ana_mfma<T, NIT, HPW>), grid, dim3(256), 0, ctx->stream, e0, e1, 0, ctx->N,
+                        ctx->sh_K, KT, NB32, ctx->sh_qpr, ctx->sh_rng, ctx->sh_Ya, x, stride, nb, ctx->sh_b);
+}
+
+template <typename T>
+static void launch_ana_mfma(odis_ctx *ctx, int nb, const T *x, long long stride, hipEvent_t e0, hipEvent_t e1) {
+  if (nb <= 16)
+    launch_ana_mfma_t<T, 1, 4>(ctx, nb, x, stride, e0, e1);
+  else if (nb <= 32)
+    launch_ana_mfma_t<T, 2, 4>(ctx, nb, x, stride, e0, e1);
+  else if (nb <= 64)
+    launch_ana_mfma_t<T, 4, 4>(ctx, nb, x, stride, e0, e1);
+  else
+    launch_ana_mfma_t<T, 8, 2>(ctx, nb, x, stride, e0, e1);
+}
+
+template <typename T, int NIT, int CT>
+static void launch_syn_mfma_t(odis_ctx *ctx, int nb, const T *base, long long stride, T *out, hipEvent_t e0,
+                              hipEvent_t e1) {
+  const int NT16 = (ctx->N + 15) / 16;
+  const dim3 grid((NT16 + 4 * CT - 1) / (4 * CT), (nb + 16 * NIT - 1) / (16 * NIT));
+  hipExtLaunchKernelGGL((k_sh_syn_mfma<T, NIT, CT>), grid, dim3(256), 0, ctx->stream, e0, e1, 0, ctx->N, NT16,
+                        ctx->sh_Kp / 32, ctx->sh_Ys, reinterpret_cast<const uint4 *>(ctx->sh_Cs), nb, base, stride,
+                        out);
+}
+
+template <typename T>
+static void launch_syn_mfma(odis_ctx *ctx, int nb, const T *base, long long stride, T *out, hipEvent_t e0,
+                            hipEvent_t e1) {
+  if (nb <= 16)
+    launch_syn_mfma_t<T, 1, 8>(ctx, nb, base, stride, out, e0, e1);
+  else if (nb <= 32)
+    launch_syn_mfma_t<T, 2, 8>(ctx, nb, base, stride, out, e0, e1);
+  else if (nb <= 64)
+    launch_syn_mfma_t<T, 4, 4>(ctx, nb, base, stride, out, e0, e1);
+  else
+    launch_syn_mfma_t<T, 8, 2>(ctx, nb, base, stride, out, e0, e1);
+}
+
 // out = base + Yt^T D Minv Yt x for nb instances (x, base, out with instance
 // stride `stride`; D null: plain LSQ coefficients; base null: synthesis
 // alone).  ana / syn: run the analysis + coefficients / the synthesis.
@@ -2193,6 +2430,7 @@ static int sh_apply(odis_ctx *ctx, const T *x, long long stride, int nb, const d
   const bool split = ctx->sh_mode == ODIS_SH_SPLIT_BF16;
   // on the fly: chunks of 64 CPT cells, partials per workgroup of kShWaves chunks
   const bool by_pair = ctx->sh_mode == ODIS_SH_RECUR && pair_degree(L);
+  const bool mfma = ctx->sh_mode == ODIS_SH_MFMA;
   // cells per lane: the option, else for the pair analysis the fewest that
   // fit its grid in one round of resident workgroups, 8 for the per-degree one
   const int cpt = ctx->sh_cpt ? ctx->sh_cpt : (by_pair ? pair_auto_cells<T>(ctx, L, nb) : 8);
@@ -2200,10 +2438,14 @@ static int sh_apply(odis_ctx *ctx, const T *x, long long stride, int nb, const d
   // cmaj holds S_l c for the compiled-degree synthesis
   const double *chain = by_pair ? ctx->sh_chain : nullptr;
   const int n_cblk = (N + 64 * kPairWaves * cpt - 1) / (64 * kPairWaves * cpt);
-  const int n_part = by_pair ? n_cblk
-                             : (ctx->sh_mode == ODIS_SH_RECUR ? (n_chunks + kShWaves - 1) / kShWaves : kShChunks);
+  const int n_part =
+      mfma ? ctx->sh_rng
+           : (by_pair ? n_cblk
+                      : (ctx->sh_mode == ODIS_SH_RECUR ? (n_chunks + kShWaves - 1) / kShWaves : kShChunks));
   if (n_part > ctx->sh_part) return fail(ctx, ODIS_E_INVALID, "analysis partials exceed their buffer");
-  if (by_pair) {
+  if (mfma) {
+    if (ana) launch_ana_mfma<T>(ctx, nb, x, stride, ev ? ev[0] : nullptr, ev ? ev[1] : nullptr);
+  } else if (by_pair) {
     if (ana) {
       hipEvent_t e0 = ev ? ev[0] : nullptr, e1 = ev ? ev[1] : nullptr;
       if (L == 32)
@@ -2239,9 +2481,15 @@ static int sh_apply(odis_ctx *ctx, const T *x, long long stride, int nb, const d
                           ev ? ev[2] : nullptr, nullptr, 0, K, n_part, ctx->sh_b);
     hipExtLaunchKernelGGL(k_sh_coef, dim3((K + 3) / 4, nb), blk, 0, ctx->stream, nullptr,
                           ev ? ev[3] : nullptr, 0, K, ctx->sh_Minv, D, bsum, c, chain,
-                          ctx->sh_mode == ODIS_SH_RECUR ? ctx->sh_cmaj : static_cast<double *>(nullptr));
+                          ctx->sh_mode == ODIS_SH_RECUR ? ctx->sh_cmaj : static_cast<double *>(nullptr),
+                          mfma ? ctx->sh_Cs : static_cast<unsigned short *>(nullptr), ctx->sh_Kp / 32);
   }
-  if (syn && ctx->sh_mode == ODIS_SH_RECUR) {
+  if (syn && mfma) {
+    if (!ana)  // coefficients from the caller: their split copy first
+      hipLaunchKernelGGL(k_sh_to_split, dim3((K + kBlock - 1) / kBlock, nb), blk, 0, ctx->stream, K,
+                         ctx->sh_Kp / 32, c, ctx->sh_Cs);
+    launch_syn_mfma<T>(ctx, nb, base, stride, out, ev ? ev[4] : nullptr, ev ? ev[5] : nullptr);
+  } else if (syn && ctx->sh_mode == ODIS_SH_RECUR) {
     if (!ana)  // coefficients from the caller: their m-major copy first
       hipLaunchKernelGGL(k_sh_to_mmajor, dim3((K + kBlock - 1) / kBlock, nb), blk, 0, ctx->stream, K, c, chain,
                          ctx->sh_cmaj);
@@ -2972,7 +3220,8 @@ static bool spd_inverse(std::vector<double> &A, int n) {
 
 extern "C" int odis_sh_set_basis(odis_ctx *ctx, int l_max, const double *beta, int mode) {
   if (!ctx) return ODIS_E_INVALID;
-  if (mode != ODIS_SH_OFF && mode != ODIS_SH_FP64 && mode != ODIS_SH_SPLIT_BF16 && mode != ODIS_SH_RECUR)
+  if (mode != ODIS_SH_OFF && mode != ODIS_SH_FP64 && mode != ODIS_SH_SPLIT_BF16 && mode != ODIS_SH_RECUR &&
+      mode != ODIS_SH_MFMA)
     return fail(ctx, ODIS_E_INVALID, "bad sh mode");
   CK(hipStreamSynchronize(ctx->stream));
   drop_graphs(ctx);
@@ -2983,15 +3232,19 @@ extern "C" int odis_sh_set_basis(odis_ctx *ctx, int l_max, const double *beta, i
   ctx->sh_Yt = ctx->sh_Minv = ctx->sh_D = ctx->sh_b = ctx->sh_c = ctx->sh_tab = ctx->sh_trig = nullptr;
   ctx->sh_tabm = ctx->sh_cmaj = ctx->sh_chain = nullptr;
   ctx->sh_Yp = nullptr;
+  ctx->sh_Ya = ctx->sh_Ys = nullptr;
+  ctx->sh_Cs = nullptr;
+  ctx->sh_Kp = ctx->sh_qpr = ctx->sh_rng = 0;
   ctx->eta_eff = nullptr;
   if (mode == ODIS_SH_OFF) return ODIS_OK;
   if (l_max < 0 || l_max > 100) return fail(ctx, ODIS_E_INVALID, "l_max out of range");
   const int K = (l_max + 1) * (l_max + 1), N = ctx->N;
   if (K > N) return fail(ctx, ODIS_E_INVALID, "(l_max+1)^2 exceeds the number of cells");
   // the synthesis keeps the K coefficients and 4 x 64 wave partials (stored
-  // basis) or the recurrence table (on the fly) in LDS
-  const size_t lds_words = mode == ODIS_SH_RECUR ? size_t(K) + size_t(l_max + 1) * (l_max + 2) + 2 * (l_max + 1)
-                                                 : size_t(K) + kBlock;
+  // basis), the recurrence table (on the fly) or nothing (batched products)
+  const size_t lds_words = mode == ODIS_SH_RECUR  ? size_t(K) + size_t(l_max + 1) * (l_max + 2) + 2 * (l_max + 1)
+                           : mode == ODIS_SH_MFMA ? 0
+                                                  : size_t(K) + kBlock;
   if (lds_words * sizeof(double) > 64 * 1024) return fail(ctx, ODIS_E_INVALID, "l_max too large for LDS");
   // basis on the host in reference order, then columns in device order
   std::vector<double> Yref;
@@ -3098,6 +3351,49 @@ extern "C" int odis_sh_set_basis(odis_ctx *ctx, int l_max, const double *beta, i
     // the most partials: 4 cells per lane
     ctx->sh_part = std::max(((N + 64 * 4 - 1) / (64 * 4) + kShWaves - 1) / kShWaves,
                             (N + 64 * kPairWaves * 4 - 1) / (64 * kPairWaves * 4));
+  }
+  if (mode == ODIS_SH_MFMA) {
+    // the FP64 basis was only needed for the Gram matrix
+    (void)hipFree(ctx->sh_Yt);
+    ctx->sh_allocs.erase(std::remove(ctx->sh_allocs.begin(), ctx->sh_allocs.end(), (void *)ctx->sh_Yt),
+                         ctx->sh_allocs.end());
+    ctx->sh_Yt = nullptr;
+    const int Kp = (K + 31) / 32 * 32, KT = Kp / 16, KB = Kp / 32;
+    const int NB32 = (N + 31) / 32, NT16 = (N + 15) / 16;
+    auto yv = [&](int k, int c) { return k < K && c < N ? Y[size_t(k) * N + c] : 0.0; };
+    auto put = [](uint16_t *f, int i, double y) {
+      const uint16_t hi = to_bf16(float(y));
+      f[i] = hi;
+      f[8 + i] = to_bf16(float(y - double(from_bf16(hi))));
+    };
+    std::vector<uint16_t> F(size_t(KT) * NB32 * 64 * 16);
+    for (int h = 0; h < KT; h++)
+      for (int q = 0; q < NB32; q++)
+        for (int l = 0; l < 64; l++) {
+          uint16_t *f = F.data() + ((size_t(h) * NB32 + q) * 64 + l) * 16;
+          for (int i = 0; i < 8; i++) put(f, i, yv(16 * h + (l & 15), 32 * q + 8 * (l >> 4) + i));
+        }
+    if ((rc = alloc(reinterpret_cast<void **>(&ctx->sh_Ya), F.size() * sizeof(uint16_t)))) return rc;
+    CK(hipMemcpy(ctx->sh_Ya, F.data(), F.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    F.assign(size_t(NT16) * KB * 64 * 16, 0);
+    for (int t = 0; t < NT16; t++)
+      for (int kb = 0; kb < KB; kb++)
+        for (int l = 0; l < 64; l++) {
+          uint16_t *f = F.data() + ((size_t(t) * KB + kb) * 64 + l) * 16;
+          for (int i = 0; i < 8; i++) put(f, i, yv(32 * kb + 8 * (l >> 4) + i, 16 * t + (l & 15)));
+        }
+    if ((rc = alloc(reinterpret_cast<void **>(&ctx->sh_Ys), F.size() * sizeof(uint16_t)))) return rc;
+    CK(hipMemcpy(ctx->sh_Ys, F.data(), F.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    std::vector<uint16_t>().swap(F);
+    // split coefficients, their padding zero for good
+    const size_t cs_bytes = size_t(ctx->B) * KB * 4 * 16 * sizeof(uint16_t);
+    if ((rc = alloc(reinterpret_cast<void **>(&ctx->sh_Cs), cs_bytes))) return rc;
+    CK(hipMemset(ctx->sh_Cs, 0, cs_bytes));
+    // analysis ranges: at most 128 of at least 4 blocks of 32 cells
+    ctx->sh_Kp = Kp;
+    ctx->sh_qpr = std::max(4, (NB32 + 127) / 128);
+    ctx->sh_rng = (NB32 + ctx->sh_qpr - 1) / ctx->sh_qpr;
+    ctx->sh_part = ctx->sh_rng;
   }
   if (mode == ODIS_SH_SPLIT_BF16) {
     std::vector<unsigned int> P(Y.size());

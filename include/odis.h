@@ -203,12 +203,18 @@ const char *odis_h5_last_error(void);
  * mode: ODIS_SH_OFF; ODIS_SH_FP64 (basis stored in FP64, FP64 sums);
  * ODIS_SH_SPLIT_BF16 (basis stored as bf16 hi + bf16 lo, FP32 accumulation);
  * ODIS_SH_RECUR (basis evaluated on the fly by the Legendre recurrence in
- * the context's precision; nothing of size N x K is kept).  The Gram
- * matrix and its inverse are FP64 in every mode. */
+ * the context's precision; nothing of size N x K is kept); ODIS_SH_MFMA
+ * (the analysis and synthesis of all instances as matrix products on the
+ * bf16 matrix cores: basis and operands split into bf16 hi + lo, three
+ * products per term, FP32 accumulation, FP64 partial sums; the basis is
+ * read once per step for the whole batch, so it suits batches of
+ * instances on one grid).  The Gram matrix, its inverse and the degree
+ * factors are FP64 in every mode. */
 #define ODIS_SH_OFF 0
 #define ODIS_SH_FP64 1
 #define ODIS_SH_SPLIT_BF16 2
 #define ODIS_SH_RECUR 3
+#define ODIS_SH_MFMA 4
 int odis_sh_set_basis(odis_ctx *ctx, int l_max, const double *beta, int mode);
 /* Coefficients (K = (l_max+1)^2) of a cell field in reference numbering, and
  * the synthesis of a field from coefficients, with the context's basis. */

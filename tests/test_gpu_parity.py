@@ -286,14 +286,15 @@ def test_interp_gg2ll(product, oracle, precision):
 # ---------------------------------------------------------------------------
 
 @pytest.mark.parametrize("mode,tol,L,level", [(1, 1e-11, 12, 5), (2, 2e-5, 12, 5), (3, 1e-11, 12, 5),
-                                              (3, 1e-10, 32, 6)])
+                                              (3, 1e-10, 32, 6), (4, 1e-4, 12, 5), (4, 1e-4, 32, 6)])
 def test_sh_synthesis_and_projection(product, oracle, mode, tol, L, level):
     """Synthesis of unit coefficients reproduces scipy's harmonics on the
     nodes; the GPU least-squares projection recovers band-limited random
     coefficients (stored FP64 basis and on-the-fly recurrence: rounding
-    level; split-BF16: ~1e-6).  Degree 12 runs the runtime-degree kernels,
-    degree 32 (10242 nodes) the compiled pair analysis and chain synthesis
-    of the benchmark."""
+    level; split-BF16: ~1e-6; split-BF16 matrix products, which also split
+    the field: ~1e-5).  Degree 12 runs the runtime-degree kernels, degree 32
+    (10242 nodes) the compiled pair analysis and chain synthesis of the
+    benchmark."""
     G, O = product, oracle
     g, arrs = generated_grid(level)
     p, _ = G.make_params()
@@ -306,11 +307,77 @@ def test_sh_synthesis_and_projection(product, oracle, mode, tol, L, level):
         e = np.zeros(K)
         e[k] = 1.0
         # bf16 hi + lo keeps ~16 significant bits: |Y| <= sqrt(2l+1)-ish
-        bound = 2 ** -16 * np.max(np.abs(Y[:, k])) if mode == 2 else 1e-12
+        bound = 2 ** -16 * np.max(np.abs(Y[:, k])) if mode in (2, 4) else 1e-12
         assert np.max(np.abs(ctx.sh_synthesize(e) - Y[:, k])) < bound
     c_true = np.random.default_rng(11).standard_normal(K)
     c = ctx.sh_project(Y @ c_true)
+    print("mode %d L %d: projection rel err %.2e" % (mode, L, rel_l2(c, c_true)))
     assert rel_l2(c, c_true) < tol
+
+
+def _shell_batch(G, mesh, n, L):
+    """n instances under shells of 5..30 km over oceans of 20..40 km, with
+    their LID_MEMBR degree factors."""
+    ps, betas = [], []
+    for i in range(n):
+        hs, h = 5e3 + 25e3 * i / max(1, n - 1), 20e3 + 20e3 * ((7 * i) % n) / n
+        p, _ = G.make_params(time_step=2.0, radius=232.1e3, g=0.1166, h=h, surface=G.LID_MEMBR,
+                             shell_thickness=hs, love_reduct=0.99)
+        ps.append(p)
+        betas.append(G.surface_beta(G.LID_MEMBR, L, 252.1e3, 0.113, hs, h))
+    return ps, np.array(betas)
+
+
+@pytest.mark.parametrize("level,L,n", [(5, 16, 20), (4, 8, 70)])
+def test_sh_mfma_batch(product, level, L, n):
+    """A batch of shell instances (FP32 stencils) coupled through the
+    split-BF16 matrix products stays within 3e-4 (relative, after 150 steps
+    from rest) of the same batch in FP64 with the FP64 recurrence, as does
+    the FP32 form with the FP32 recurrence; an instance's trajectory is the
+    same bit for bit alone and inside the batch (its products do not depend
+    on the other instances); graph replays equal direct launches.  20 and 70
+    instances run the 2- and 8-tile kernels, one instance the 1-tile ones."""
+    G = product
+    g, _ = generated_grid(level)
+    mesh = G.Mesh(g, 232.1e3)
+    ps, beta = _shell_batch(G, mesh, n, L)
+    mf = G.Context(mesh, ps, precision=1)
+    mf.set_sh(L, beta, mode=G.SH_MFMA)
+    rc = G.Context(mesh, ps, precision=1)
+    rc.set_sh(L, beta, mode=G.SH_RECUR)
+    ref = G.Context(mesh, ps, precision=0)
+    ref.set_sh(L, beta, mode=G.SH_RECUR)
+    j = n // 3
+    one = G.Context(mesh, [ps[j]], precision=1)
+    one.set_sh(L, beta[j:j + 1], mode=G.SH_MFMA)
+    gr = G.Context(mesh, ps, precision=1)
+    gr.set_sh(L, beta, mode=G.SH_MFMA)
+    gr.use_graphs(True)
+    for steps in (1, 2, 147):
+        for c in (mf, rc, one, ref, gr):
+            c.step(steps)
+        a = one.get_state(0)
+        for c in (mf, gr):
+            b = c.get_state(j)
+            for k in ("v", "eta", "dvdt", "detadt"):
+                assert np.array_equal(a[k], b[k]), (steps, k)
+    worst = {"mfma": 0.0, "recurrence": 0.0}
+    for i in range(n):
+        r = ref.get_state(i)
+        for name, c in (("mfma", mf), ("recurrence", rc)):
+            a = c.get_state(i)
+            for k in ("v", "eta"):
+                worst[name] = max(worst[name], rel_l2(a[k], r[k]))
+    print("%d instances, FP32 vs FP64, worst rel err: split-BF16 products %.2e, FP32 recurrence %.2e"
+          % (n, worst["mfma"], worst["recurrence"]))
+    assert worst["mfma"] < 3e-4 and worst["recurrence"] < 3e-4
+    for steps in (64, 3, 64):
+        gr.step(steps)
+        one.step(steps)
+        a, b = one.get_state(0), gr.get_state(j)
+        assert np.array_equal(a["eta"], b["eta"]) and np.array_equal(a["v"], b["v"]), steps
+    for c in (mf, rc, one, gr, ref):
+        c.close()
 
 
 @pytest.mark.parametrize("precision,mode", [(0, 1), (1, 1), (0, 3), (1, 3)])
