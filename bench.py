@@ -274,7 +274,10 @@ def uneven_run(G, args):
     obliquity U(0, 1) degrees), quadratic drag c_D = U(1e-3, 5e-3), and a
     free surface or a 20 km LID_MEMBR shell (SH coupling to degree 16) with
     probability 1/2.  Instances on one grid and surface share a context
-    (the kernels take them along their grid's y axis); each instance's dt is
+    (the kernels take them along their grid's y axis); a shell group of at
+    least 4 instances is coupled through the batched split-BF16 matrix
+    products, a smaller one through the recurrence (args.uneven_sh overrides
+    this); each instance's dt is
     the quantised step below 0.35 / omega_max, omega_max the larger of the
     grid-scale gravity-wave frequency (0.0193 s^-1 at 10242 cells, scaling
     as sqrt(N)) and, under the shell, the degree-16 shell-mode frequency
@@ -309,15 +312,21 @@ def uneven_run(G, args):
             ps.append(G.make_params(**kw)[0])
         mesh = G.Mesh(grid, ps[0].radius)
         ctx = G.Context(mesh, ps, precision=G.FP32)
+        sh = None
         if lid:
-            ctx.set_sh(L, np.tile(beta, (len(ps), 1)), mode=G.SH_RECUR)
-        el, _ = timed_run(ctx, n_steps, 3, 0)
+            sh = args.uneven_sh if args.uneven_sh != "auto" else ("mfma" if len(ps) >= 4 else "recur")
+            ctx.set_sh(L, np.tile(beta, (len(ps), 1)), mode=G.SH_MFMA if sh == "mfma" else G.SH_RECUR)
+        el, tm = timed_run(ctx, n_steps, 3, 4 if lid else 0)
         ok = bool(all(abs(x) < float("inf") for x in ctx.dissipation()))
         ctx.close()
         elapsed += el
         total_cells += n_cells * len(ps)
-        rows.append({"cells": n_cells, "surface": "LID_MEMBR" if lid else "FREE", "instances": len(ps),
-                     "ms_per_step": 1e3 * el / n_steps, "finite": ok})
+        row = {"cells": n_cells, "surface": "LID_MEMBR" if lid else "FREE", "instances": len(ps),
+               "ms_per_step": 1e3 * el / n_steps, "finite": ok}
+        if sh:
+            row["sh"] = {"mfma": "split-BF16 matrix products", "recur": "on-the-fly recurrence"}[sh]
+            row["kernel_ms"] = {k[:-3]: tm[k] for k in ("step_ms", "sh_analysis_ms", "sh_coef_ms", "sh_synth_ms")}
+        rows.append(row)
     return {"instances": args.uneven_instances, "seed": UNEVEN_SEED, "precision": "fp32 state, fp64 dissipation",
             "value": total_cells * n_steps / elapsed, "ms_per_step_all_groups": 1e3 * elapsed / n_steps,
             "steps": n_steps, "groups": rows}
@@ -408,6 +417,9 @@ def main():
     ap.add_argument("--sweep-instances", type=int, default=256, help="instances of the batched-sweep extra")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch extra")
     ap.add_argument("--uneven-instances", type=int, default=512, help="instances of the uneven-batch extra")
+    ap.add_argument("--uneven-sh", choices=("auto", "recur", "mfma"), default="auto",
+                    help="SH coupling of the uneven batch's shell groups: split-BF16 matrix products, the "
+                         "on-the-fly recurrence, or the products for groups of at least 4 instances (auto)")
     ap.add_argument("--level", type=int, default=8, help="reference grid level (8 = 163842 cells)")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
                     help="precision of the headline value (fp32: FP32 stencils, split-BF16 SH basis)")

@@ -86,7 +86,7 @@ __device__ __forceinline__ FaceOut<T> face_update(const FaceArgs<T> &a, int e, i
   out.f0 = T(0);
   if (!DISS_ONLY) {
     const long long n = (long long)table[0] + s;
-    const double *coef = table + 8 + size_t(s) * gridDim.y * 8;
+    const double *coef = table + 8 + size_t(s) * a.B * 8;
     const T g = T(ip[0]), alpha = T(ip[1]), dt = T(ip[2]), qd = T(ip[4]);
     const bool ab3 = (n >= 2) || (ip[3] != 0.0);
     const T *eta = a.eta + b * a.eta_stride;
@@ -306,6 +306,8 @@ struct StepArgs {
   const unsigned short *cyc_loc;  // [6][N] cycle faces, patch-local index
   int n_patch;
   int n_blocks;                 // workgroups per instance (= gridDim.x); each runs a contiguous run of patches
+  int n_inst;                   // instances
+  int ipb;                      // instances per workgroup, one after the other (gridDim.y = ceil(n_inst / ipb))
 };
 
 // One patch of the one-kernel step; returns this lane's dissipation
@@ -380,32 +382,37 @@ __global__ __launch_bounds__(LPC * P) void k_step(StepArgs<T> a, int energy_quad
   __shared__ T lv[6 * P];
   __shared__ int gid[6 * P];
   __shared__ double sh[NT / 64];
-  const int b = blockIdx.y;
   const long long n = (long long)table[0] + s;
   const int t = xcd_tile(blockIdx.x, a.n_blocks);
-  double local = 0.0;
-  if (MULTI) {
-    const int iters = (a.n_patch + a.n_blocks - 1) / a.n_blocks;
-    for (int it = 0; it < iters; ++it) {
-      const int j = t * iters + it;
-      if (j >= a.n_patch) break;    // uniform in the workgroup
-      if (it > 0) __syncthreads();  // the previous patch's cell phase has read lv / gid
-      local += step_patch<T, P, LPC>(a, j, b, n, energy_quad, table, s, lv, gid);
+  // the workgroup's instances one after the other: the operators of its
+  // patches come from the MALL once and from L1 / L2 afterwards
+  const int b0 = blockIdx.y * a.ipb, b1 = min(a.n_inst, b0 + a.ipb);
+  for (int b = b0; b < b1; ++b) {
+    if (b > b0) __syncthreads();  // the previous instance's phases are done with lv / gid / sh
+    double local = 0.0;
+    if (MULTI) {
+      const int iters = (a.n_patch + a.n_blocks - 1) / a.n_blocks;
+      for (int it = 0; it < iters; ++it) {
+        const int j = t * iters + it;
+        if (j >= a.n_patch) break;    // uniform in the workgroup
+        if (it > 0) __syncthreads();  // the previous patch's cell phase has read lv / gid
+        local += step_patch<T, P, LPC>(a, j, b, n, energy_quad, table, s, lv, gid);
+      }
+    } else {
+      local = step_patch<T, P, LPC>(a, t, b, n, energy_quad, table, s, lv, gid);
     }
-  } else {
-    local = step_patch<T, P, LPC>(a, t, b, n, energy_quad, table, s, lv, gid);
-  }
-  const double part = block_sum<NT / 64>(local, sh);
-  if (threadIdx.x == 0) a.f.partials[((n & 1) * gridDim.y + b) * a.n_blocks + t] = part;
-  if (!PENDING || blockIdx.x != 0) return;
-  // dissipation of step n - 1: partials of the previous launch
-  __syncthreads();
-  const double tot = sum_partials<NT / 64>(
-      a.f.partials + (((n - 1) & 1) * gridDim.y + b) * size_t(a.n_blocks), a.n_blocks, sh);
-  if (threadIdx.x == 0) {
-    const double d = tot * a.c.dconst[b];
-    a.c.series[size_t(b) * a.c.cap + size_t((n - 1) % a.c.cap)] = d;
-    if (!isfinite(d)) a.c.nonfinite[b] = 1;
+    const double part = block_sum<NT / 64>(local, sh);
+    if (threadIdx.x == 0) a.f.partials[((n & 1) * a.n_inst + b) * a.n_blocks + t] = part;
+    if (!PENDING || blockIdx.x != 0) continue;  // uniform in the workgroup
+    // dissipation of step n - 1: partials of the previous launch
+    __syncthreads();
+    const double tot = sum_partials<NT / 64>(
+        a.f.partials + (((n - 1) & 1) * a.n_inst + b) * size_t(a.n_blocks), a.n_blocks, sh);
+    if (threadIdx.x == 0) {
+      const double d = tot * a.c.dconst[b];
+      a.c.series[size_t(b) * a.c.cap + size_t((n - 1) % a.c.cap)] = d;
+      if (!isfinite(d)) a.c.nonfinite[b] = 1;
+    }
   }
 }
 
@@ -1275,7 +1282,7 @@ static int4 synth_split(int L) {
 // in the fragment order of its pass, as 16 bf16 per lane (8 hi, then 8 lo):
 //   Ya[h][q][lane]  analysis B operand: harmonic 16 h + l % 16, cells
 //                   32 q + 8 (l / 16) + i;
-//   Ys[t][kb][lane] synthesis A operand: cell 16 t + l % 16, harmonics
+//   Ys[t][kb][lane] synthesis B operand: cell 16 t + l % 16, harmonics
 //                   32 kb + 8 (l / 16) + i;
 // harmonics k >= K (Kp = K rounded up to 32) and cells >= N are zero.  The
 // coefficients go to Cs[j][kb][l / 16] (instance j, same harmonic order),
@@ -1302,72 +1309,122 @@ __device__ __forceinline__ void split8(const float (&f)[8], bf16x8 &hi, bf16x8 &
   }
 }
 
+// eta of the batch -> the analysis A operands, Xa[it][q][lane] = 8 hi, then
+// 8 lo bf16 of instance 16 it + l % 16 at cells 32 q + 8 (l / 16) .. + 7
+// (zero past nb and N).  One thread per (instance, run of 8 cells), threads
+// in cell order, so the loads of a wave are one contiguous stretch.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sh_pack_mfma(int N, int NB32, int nb, const T *__restrict__ x,
+                                                         long long x_stride, uint4 *__restrict__ Xa) {
+  const int n8 = NB32 * 4;
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  const int j = int(t / n8), c8 = int(t % n8);
+  if (j >= (nb + 15) / 16 * 16) return;
+  const int c0 = 8 * c8;
+  const T *xr = x + (long long)min(j, nb - 1) * x_stride;
+  float f[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) f[i] = (j < nb && c0 + i < N) ? float(xr[c0 + i]) : 0.f;
+  bf16x8 hi, lo;
+  split8(f, hi, lo);
+  uint4 *dst = Xa + ((size_t(j >> 4) * NB32 + (c8 >> 2)) * 64 + (j & 15) + 16 * (c8 & 3)) * 2;
+  dst[0] = __builtin_bit_cast(uint4, hi);
+  dst[1] = __builtin_bit_cast(uint4, lo);
+}
+
 // analysis: grid (n_rng cell ranges of qpr blocks of 32 cells, harmonic
-// groups of 4 HPW tiles, instance groups of 16 NIT); wave w of a workgroup
-// owns HPW harmonic tiles and every instance tile of the group, and writes
-// its sums over the range as partial n_rng of b_part ([nb][n_part][K]).
-template <typename T, int NIT, int HPW>
-__global__ __launch_bounds__(256) void k_sh_ana_mfma(int N, int K, int KT, int NB32, int qpr, int n_part,
-                                                     const uint4 *__restrict__ Ya, const T *__restrict__ x,
-                                                     long long x_stride, int nb, double *__restrict__ b_part) {
+// groups of 4 HPW tiles, instance groups of NIT tiles); wave w of a
+// workgroup owns HPW harmonic tiles and the group's instance tiles, and
+// writes its sums over the range as partial `rng` of b_part
+// ([nb][n_part][K]).  The operands of block q + 1 are loaded while block q
+// is multiplied.  Tiles past the last (harmonic or instance) repeat the
+// last one's operands and are not stored, so the loop has no branch.
+template <int NIT, int HPW>
+__global__ __launch_bounds__(256) void k_sh_ana_mfma(int K, int KT, int NB32, int qpr, int n_part,
+                                                     const uint4 *__restrict__ Ya, const uint4 *__restrict__ Xa,
+                                                     int nb, double *__restrict__ b_part) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, r16 = lane & 15;
   const int rng = blockIdx.x;
   const int h0 = (blockIdx.y * 4 + w) * HPW;
   if (h0 >= KT) return;  // a whole wave without tiles (no barrier below)
-  const int j0 = blockIdx.z * NIT * 16;
-  const int nit = min(NIT, (nb - j0 + 15) / 16);
+  const int it0 = blockIdx.z * NIT, n_it = (nb + 15) / 16;
   const int q0 = rng * qpr, q1 = min(NB32, q0 + qpr);
+  const uint4 *xp[NIT], *yp[HPW];
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) xp[it] = Xa + (size_t(min(it0 + it, n_it - 1)) * NB32 * 64 + lane) * 2;
+#pragma unroll
+  for (int t = 0; t < HPW; ++t) yp[t] = Ya + (size_t(min(h0 + t, KT - 1)) * NB32 * 64 + lane) * 2;
   floatx4 acc[HPW][NIT];
 #pragma unroll
   for (int t = 0; t < HPW; ++t)
 #pragma unroll
     for (int it = 0; it < NIT; ++it) acc[t][it] = floatx4{0.f, 0.f, 0.f, 0.f};
+  uint4 xa[NIT][2], ya[HPW][2];
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    xa[it][0] = xp[it][size_t(q0) * 128];
+    xa[it][1] = xp[it][size_t(q0) * 128 + 1];
+  }
+#pragma unroll
+  for (int t = 0; t < HPW; ++t) {
+    ya[t][0] = yp[t][size_t(q0) * 128];
+    ya[t][1] = yp[t][size_t(q0) * 128 + 1];
+  }
   for (int q = q0; q < q1; ++q) {
-    bf16x8 ahi[NIT], alo[NIT];
-    const int c0 = q * 32 + 8 * g;
+    uint4 xn[NIT][2], yn[HPW][2];
+    const size_t on = size_t(min(q + 1, q1 - 1)) * 128;  // the last block loads itself again
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
-      if (it >= nit) break;
-      const int j = j0 + it * 16 + r16;
-      float f[8];
-      const T *xr = x + (long long)min(j, nb - 1) * x_stride;
-#pragma unroll
-      for (int i = 0; i < 8; ++i) f[i] = (j < nb && c0 + i < N) ? float(xr[c0 + i]) : 0.f;
-      split8(f, ahi[it], alo[it]);
+      xn[it][0] = xp[it][on];
+      xn[it][1] = xp[it][on + 1];
     }
 #pragma unroll
     for (int t = 0; t < HPW; ++t) {
-      const int h = h0 + t;
-      if (h >= KT) break;
-      const uint4 *fp = Ya + ((size_t(h) * NB32 + q) * 64 + lane) * 2;
-      const bf16x8 bhi = __builtin_bit_cast(bf16x8, fp[0]), blo = __builtin_bit_cast(bf16x8, fp[1]);
+      yn[t][0] = yp[t][on];
+      yn[t][1] = yp[t][on + 1];
+    }
 #pragma unroll
-      for (int it = 0; it < NIT; ++it) {
-        if (it >= nit) break;
-        acc[t][it] = mfma3(ahi[it], alo[it], bhi, blo, acc[t][it]);
-      }
+    for (int t = 0; t < HPW; ++t) {
+      const bf16x8 bhi = __builtin_bit_cast(bf16x8, ya[t][0]), blo = __builtin_bit_cast(bf16x8, ya[t][1]);
+#pragma unroll
+      for (int it = 0; it < NIT; ++it)
+        acc[t][it] = mfma3(__builtin_bit_cast(bf16x8, xa[it][0]), __builtin_bit_cast(bf16x8, xa[it][1]), bhi,
+                           blo, acc[t][it]);
+    }
+    // issue every load of block q + 1, then the products of block q
+
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      xa[it][0] = xn[it][0];
+      xa[it][1] = xn[it][1];
+    }
+#pragma unroll
+    for (int t = 0; t < HPW; ++t) {
+      ya[t][0] = yn[t][0];
+      ya[t][1] = yn[t][1];
     }
   }
-  // result tile: rows = instances j0 + 16 it + 4 g + r, column = harmonic 16 h + r16
+  // result tile: rows = instances 16 (it0 + it) + 4 g + r, column = harmonic 16 h + r16
 #pragma unroll
   for (int t = 0; t < HPW; ++t) {
     const int h = h0 + t, k = 16 * h + r16;
-    if (h >= KT) break;
 #pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-      if (it >= nit) break;
+    for (int it = 0; it < NIT; ++it)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int j = j0 + it * 16 + 4 * g + r;
-        if (j < nb && k < K) b_part[(size_t(j) * n_part + rng) * K + k] = double(acc[t][it][r]);
+        const int j = 16 * (it0 + it) + 4 * g + r;
+        if (h < KT && j < nb && k < K) b_part[(size_t(j) * n_part + rng) * K + k] = double(acc[t][it][r]);
       }
-    }
   }
 }
 
-// synthesis: grid (cell tiles in groups of 4 CT, instance groups of 16 NIT);
-// wave w owns CT tiles of 16 cells and every instance tile of the group:
-// out = base + Y^T C over all Kp harmonics.
+// synthesis: grid (cell tiles in groups of 4 CT, instance groups of NIT
+// tiles); wave w owns CT tiles of 16 cells and the group's instance tiles:
+// out = base + (C^T Y)^T over all Kp harmonics (the fragment of Cs is the A
+// operand, that of Ys the B operand: both hold 8 harmonics of one row or
+// column), the operands of harmonic block kb + 1 loaded while block kb is
+// multiplied.  Tiles past the last repeat the last one's operands and are
+// not stored (no branch in the loop).
 template <typename T, int NIT, int CT>
 __global__ __launch_bounds__(256) void k_sh_syn_mfma(int N, int NT16, int KB, const uint4 *__restrict__ Ys,
                                                      const uint4 *__restrict__ Cs, int nb, const T *base,
@@ -1376,53 +1433,85 @@ __global__ __launch_bounds__(256) void k_sh_syn_mfma(int N, int NT16, int KB, co
   const int t0 = (blockIdx.x * 4 + w) * CT;
   if (t0 >= NT16) return;
   const int j0 = blockIdx.y * NIT * 16;
-  const int nit = min(NIT, (nb - j0 + 15) / 16);
+  const uint4 *cp[NIT], *yp[CT];
+#pragma unroll
+  for (int it = 0; it < NIT; ++it)  // rows past nb read the last instance's coefficients
+    cp[it] = Cs + (size_t(min(j0 + it * 16 + r16, nb - 1)) * KB * 4 + g) * 2;
+#pragma unroll
+  for (int u = 0; u < CT; ++u) yp[u] = Ys + (size_t(min(t0 + u, NT16 - 1)) * KB * 64 + lane) * 2;
   floatx4 acc[CT][NIT];
 #pragma unroll
   for (int u = 0; u < CT; ++u)
 #pragma unroll
     for (int it = 0; it < NIT; ++it) acc[u][it] = floatx4{0.f, 0.f, 0.f, 0.f};
+  uint4 cb[NIT][2], ab[CT][2];
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    cb[it][0] = cp[it][0];
+    cb[it][1] = cp[it][1];
+  }
+#pragma unroll
+  for (int u = 0; u < CT; ++u) {
+    ab[u][0] = yp[u][0];
+    ab[u][1] = yp[u][1];
+  }
   for (int kb = 0; kb < KB; ++kb) {
-    bf16x8 bhi[NIT], blo[NIT];
+    uint4 cn[NIT][2], an[CT][2];
+    const int kn = min(kb + 1, KB - 1);  // the last block loads itself again
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
-      if (it >= nit) break;
-      const int j = min(j0 + it * 16 + r16, nb - 1);  // columns past nb are never stored
-      const uint4 *cp = Cs + ((size_t(j) * KB + kb) * 4 + g) * 2;
-      bhi[it] = __builtin_bit_cast(bf16x8, cp[0]);
-      blo[it] = __builtin_bit_cast(bf16x8, cp[1]);
+      cn[it][0] = cp[it][size_t(kn) * 8];
+      cn[it][1] = cp[it][size_t(kn) * 8 + 1];
     }
 #pragma unroll
     for (int u = 0; u < CT; ++u) {
-      const int t = t0 + u;
-      if (t >= NT16) break;
-      const uint4 *fp = Ys + ((size_t(t) * KB + kb) * 64 + lane) * 2;
-      const bf16x8 ahi = __builtin_bit_cast(bf16x8, fp[0]), alo = __builtin_bit_cast(bf16x8, fp[1]);
-#pragma unroll
-      for (int it = 0; it < NIT; ++it) {
-        if (it >= nit) break;
-        acc[u][it] = mfma3(ahi, alo, bhi[it], blo[it], acc[u][it]);
-      }
+      an[u][0] = yp[u][size_t(kn) * 128];
+      an[u][1] = yp[u][size_t(kn) * 128 + 1];
     }
-  }
-  // result tile: rows = cells 16 t + 4 g + r, column = instance j0 + 16 it + r16
 #pragma unroll
-  for (int u = 0; u < CT; ++u) {
-    const int t = t0 + u;
-    if (t >= NT16) break;
+    for (int u = 0; u < CT; ++u) {
+      const bf16x8 yhi = __builtin_bit_cast(bf16x8, ab[u][0]), ylo = __builtin_bit_cast(bf16x8, ab[u][1]);
+#pragma unroll
+      for (int it = 0; it < NIT; ++it)
+        acc[u][it] = mfma3(__builtin_bit_cast(bf16x8, cb[it][0]), __builtin_bit_cast(bf16x8, cb[it][1]), yhi, ylo,
+                           acc[u][it]);
+    }
+    // issue every load of block kb + 1, then the products of block kb
+
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
-      if (it >= nit) break;
-      const int j = j0 + it * 16 + r16;
-      if (j >= nb) continue;
+      cb[it][0] = cn[it][0];
+      cb[it][1] = cn[it][1];
+    }
+#pragma unroll
+    for (int u = 0; u < CT; ++u) {
+      ab[u][0] = an[u][0];
+      ab[u][1] = an[u][1];
+    }
+  }
+  // result tile (C^T Y): rows = instances j0 + 16 it + 4 g + r, column =
+  // cell 16 t + r16, so 16 lanes store 16 consecutive cells of a row; the
+  // base values are loaded together first (clamped into range)
+  T add[CT][NIT][4];
+#pragma unroll
+  for (int u = 0; u < CT; ++u)
+#pragma unroll
+    for (int it = 0; it < NIT; ++it)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int c = 16 * t + 4 * g + r;
-        if (c >= N) continue;
-        const double add = base ? double(base[j * stride + c]) : 0.0;
-        out[j * stride + c] = T(add + double(acc[u][it][r]));
+        const int c = min(16 * (t0 + u) + r16, N - 1), j = min(j0 + it * 16 + 4 * g + r, nb - 1);
+        add[u][it][r] = base ? base[j * stride + c] : T(0);
       }
-    }
+#pragma unroll
+  for (int u = 0; u < CT; ++u) {
+    const int t = t0 + u, c = 16 * t + r16;
+#pragma unroll
+    for (int it = 0; it < NIT; ++it)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int j = j0 + it * 16 + 4 * g + r;
+        if (t < NT16 && c < N && j < nb) out[j * stride + c] = T(double(add[u][it][r]) + double(acc[u][it][r]));
+      }
   }
 }
 
@@ -1552,6 +1641,7 @@ struct odis_ctx {
   int patch = 64;              // cells per patch (ODIS_OPT_PATCH_CELLS: 64, 128, 256)
   int step_blocks = 0;         // workgroups of the one-kernel step (ODIS_OPT_STEP_BLOCKS; 0: one per patch)
   int step_lanes = 4;          // lanes per cell of the one-kernel step (ODIS_OPT_STEP_LANES: 2 or 4)
+  int step_inst = 0;           // instances per workgroup of the one-kernel step (ODIS_OPT_STEP_INSTANCES; 0: auto)
   std::string one_kernel_why;
   std::vector<int> gi_host, cyc_host;  // device-order face cells and cell cycles, to rebuild patches
   long long step = 0;          // next step index n, shared by all instances
@@ -1589,6 +1679,7 @@ struct odis_ctx {
   double *sh_tabm = nullptr, *sh_cmaj = nullptr; // m-major table, m-major coefficients [B][2 (L+1)(L+2)]
   double *sh_chain = nullptr;                    // rescaled recurrence (beta, S) per (l, m), then C_m
   uint4 *sh_Ya = nullptr, *sh_Ys = nullptr;      // batched products: split basis in fragment order
+  uint4 *sh_Xa = nullptr;                        // batched products: split eta of the batch (analysis operand)
   unsigned short *sh_Cs = nullptr;               // batched products: split coefficients [B][Kp / 32][4][16]
   int sh_Kp = 0, sh_qpr = 0, sh_rng = 0;         // batched products: padded K, 32-cell blocks per range, ranges
   int sh_part = 0;                               // capacity of analysis partials per instance
@@ -2235,6 +2326,7 @@ static FaceArgs<T> face_args(odis_ctx *ctx, Ops<T> &o) {
   FaceArgs<T> a;
   a.E = ctx->E;
   a.nb = ctx->nb_face;
+  a.B = ctx->B;
   a.tri_idx = o.tri_idx;
   a.tri_coef = o.tri_coef;
   a.g_idx = o.g_idx;
@@ -2370,51 +2462,47 @@ static void launch_analysis_pair(odis_ctx *ctx, int nb, const T *x, long long st
                         ctx->sh_trig, x, stride, cpt, n_cblk, n_logical, ctx->sh_b);
 }
 
-// Batched products: instance tiles per wave (NIT) and harmonic (HPW) or
-// cell (CT) tiles per wave, from the batch size: at most 64 accumulators
-// per lane.  The ranges of the analysis depend on N only, so an instance's
-// coefficients do not depend on the rest of its batch.
-template <typename T, int NIT, int HPW>
-static void launch_ana_mfma_t(odis_ctx *ctx, int nb, const T *x, long long stride, hipEvent_t e0, hipEvent_t e1) {
-  const int KT = ctx->sh_Kp / 16, NB32 = (ctx->N + 31) / 32;
-  const dim3 grid(ctx->sh_rng, (KT + 4 * HPW - 1) / (4 * HPW), (nb + 16 * NIT - 1) / (16 * NIT));
-  hipExtLaunchKernelGGL((k_sh_ana_mfma<T, NIT, HPW>), grid, dim3(256), 0, ctx->stream, e0, e1, 0, ctx->N,
-                        ctx->sh_K, KT, NB32, ctx->sh_qpr, ctx->sh_rng, ctx->sh_Ya, x, stride, nb, ctx->sh_b);
+// Batched products: NIT instance tiles per wave (all of the batch up to
+// 128 instances) and HPW harmonic or CT cell tiles per wave, fewer as NIT
+// grows, so that operands of two blocks and the accumulators stay in
+// registers.  The analysis ranges depend on N only, and every kernel sums
+// an instance's products in the same order whatever NIT, so an instance's
+// results do not depend on the rest of its batch.
+template <typename T, int NIT>
+static void launch_mfma_t(odis_ctx *ctx, bool ana, int nb, const T *x, const T *base, long long stride, T *out,
+                          hipEvent_t e0, hipEvent_t e1) {
+  constexpr int W = NIT <= 3 ? 4 : (NIT == 4 ? 3 : (NIT <= 6 ? 2 : 1));  // HPW of the analysis
+  constexpr int CT = NIT == 1 ? 8 : (NIT == 2 ? 6 : (NIT == 3 ? 4 : (NIT == 4 ? 3 : (NIT <= 6 ? 2 : 1))));
+  const int N = ctx->N, NB32 = (N + 31) / 32, NT16 = (N + 15) / 16, n_it = (nb + 15) / 16;
+  if (ana) {
+    const int KT = ctx->sh_Kp / 16;
+    const long long threads = (long long)n_it * 16 * NB32 * 4;
+    hipLaunchKernelGGL((k_sh_pack_mfma<T>), dim3(unsigned((threads + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       ctx->stream, N, NB32, nb, x, stride, ctx->sh_Xa);
+    const dim3 grid(ctx->sh_rng, (KT + 4 * W - 1) / (4 * W), (n_it + NIT - 1) / NIT);
+    hipExtLaunchKernelGGL((k_sh_ana_mfma<NIT, W>), grid, dim3(256), 0, ctx->stream, e0, e1, 0, ctx->sh_K, KT, NB32,
+                          ctx->sh_qpr, ctx->sh_rng, ctx->sh_Ya, ctx->sh_Xa, nb, ctx->sh_b);
+  } else {
+    const dim3 grid((NT16 + 4 * CT - 1) / (4 * CT), (n_it + NIT - 1) / NIT);
+    hipExtLaunchKernelGGL((k_sh_syn_mfma<T, NIT, CT>), grid, dim3(256), 0, ctx->stream, e0, e1, 0, N, NT16,
+                          ctx->sh_Kp / 32, ctx->sh_Ys, reinterpret_cast<const uint4 *>(ctx->sh_Cs), nb, base,
+                          stride, out);
+  }
 }
 
 template <typename T>
-static void launch_ana_mfma(odis_ctx *ctx, int nb, const T *x, long long stride, hipEvent_t e0, hipEvent_t e1) {
-  if (nb <= 16)
-    launch_ana_mfma_t<T, 1, 4>(ctx, nb, x, stride, e0, e1);
-  else if (nb <= 32)
-    launch_ana_mfma_t<T, 2, 4>(ctx, nb, x, stride, e0, e1);
-  else if (nb <= 64)
-    launch_ana_mfma_t<T, 4, 4>(ctx, nb, x, stride, e0, e1);
-  else
-    launch_ana_mfma_t<T, 8, 2>(ctx, nb, x, stride, e0, e1);
-}
-
-template <typename T, int NIT, int CT>
-static void launch_syn_mfma_t(odis_ctx *ctx, int nb, const T *base, long long stride, T *out, hipEvent_t e0,
-                              hipEvent_t e1) {
-  const int NT16 = (ctx->N + 15) / 16;
-  const dim3 grid((NT16 + 4 * CT - 1) / (4 * CT), (nb + 16 * NIT - 1) / (16 * NIT));
-  hipExtLaunchKernelGGL((k_sh_syn_mfma<T, NIT, CT>), grid, dim3(256), 0, ctx->stream, e0, e1, 0, ctx->N, NT16,
-                        ctx->sh_Kp / 32, ctx->sh_Ys, reinterpret_cast<const uint4 *>(ctx->sh_Cs), nb, base, stride,
-                        out);
-}
-
-template <typename T>
-static void launch_syn_mfma(odis_ctx *ctx, int nb, const T *base, long long stride, T *out, hipEvent_t e0,
-                            hipEvent_t e1) {
-  if (nb <= 16)
-    launch_syn_mfma_t<T, 1, 8>(ctx, nb, base, stride, out, e0, e1);
-  else if (nb <= 32)
-    launch_syn_mfma_t<T, 2, 8>(ctx, nb, base, stride, out, e0, e1);
-  else if (nb <= 64)
-    launch_syn_mfma_t<T, 4, 4>(ctx, nb, base, stride, out, e0, e1);
-  else
-    launch_syn_mfma_t<T, 8, 2>(ctx, nb, base, stride, out, e0, e1);
+static void launch_mfma(odis_ctx *ctx, bool ana, int nb, const T *x, const T *base, long long stride, T *out,
+                        hipEvent_t e0, hipEvent_t e1) {
+  switch (std::min(8, (nb + 15) / 16)) {
+    case 1: launch_mfma_t<T, 1>(ctx, ana, nb, x, base, stride, out, e0, e1); break;
+    case 2: launch_mfma_t<T, 2>(ctx, ana, nb, x, base, stride, out, e0, e1); break;
+    case 3: launch_mfma_t<T, 3>(ctx, ana, nb, x, base, stride, out, e0, e1); break;
+    case 4: launch_mfma_t<T, 4>(ctx, ana, nb, x, base, stride, out, e0, e1); break;
+    case 5: launch_mfma_t<T, 5>(ctx, ana, nb, x, base, stride, out, e0, e1); break;
+    case 6: launch_mfma_t<T, 6>(ctx, ana, nb, x, base, stride, out, e0, e1); break;
+    case 7: launch_mfma_t<T, 7>(ctx, ana, nb, x, base, stride, out, e0, e1); break;
+    default: launch_mfma_t<T, 8>(ctx, ana, nb, x, base, stride, out, e0, e1); break;
+  }
 }
 
 // out = base + Yt^T D Minv Yt x for nb instances (x, base, out with instance
@@ -2444,7 +2532,7 @@ static int sh_apply(odis_ctx *ctx, const T *x, long long stride, int nb, const d
                       : (ctx->sh_mode == ODIS_SH_RECUR ? (n_chunks + kShWaves - 1) / kShWaves : kShChunks));
   if (n_part > ctx->sh_part) return fail(ctx, ODIS_E_INVALID, "analysis partials exceed their buffer");
   if (mfma) {
-    if (ana) launch_ana_mfma<T>(ctx, nb, x, stride, ev ? ev[0] : nullptr, ev ? ev[1] : nullptr);
+    if (ana) launch_mfma<T>(ctx, true, nb, x, nullptr, stride, nullptr, ev ? ev[0] : nullptr, ev ? ev[1] : nullptr);
   } else if (by_pair) {
     if (ana) {
       hipEvent_t e0 = ev ? ev[0] : nullptr, e1 = ev ? ev[1] : nullptr;
@@ -2488,7 +2576,7 @@ static int sh_apply(odis_ctx *ctx, const T *x, long long stride, int nb, const d
     if (!ana)  // coefficients from the caller: their split copy first
       hipLaunchKernelGGL(k_sh_to_split, dim3((K + kBlock - 1) / kBlock, nb), blk, 0, ctx->stream, K,
                          ctx->sh_Kp / 32, c, ctx->sh_Cs);
-    launch_syn_mfma<T>(ctx, nb, base, stride, out, ev ? ev[4] : nullptr, ev ? ev[5] : nullptr);
+    launch_mfma<T>(ctx, false, nb, nullptr, base, stride, out, ev ? ev[4] : nullptr, ev ? ev[5] : nullptr);
   } else if (syn && ctx->sh_mode == ODIS_SH_RECUR) {
     if (!ana)  // coefficients from the caller: their m-major copy first
       hipLaunchKernelGGL(k_sh_to_mmajor, dim3((K + kBlock - 1) / kBlock, nb), blk, 0, ctx->stream, K, c, chain,
@@ -2565,6 +2653,13 @@ static int step_blocks(const odis_ctx *ctx) {
   return ctx->step_blocks > 0 ? std::min(ctx->step_blocks, ctx->n_patch) : ctx->n_patch;
 }
 
+// Instances per workgroup of the one-kernel step: the option, else 1 for a
+// single instance and kStepInstances for a batch.
+constexpr int kStepInstances = 4;
+static int step_instances(const odis_ctx *ctx) {
+  return std::min(ctx->B, ctx->step_inst > 0 ? ctx->step_inst : (ctx->B > 1 ? kStepInstances : 1));
+}
+
 template <typename T>
 static StepArgs<T> step_args(odis_ctx *ctx, Ops<T> &o) {
   StepArgs<T> a;
@@ -2577,12 +2672,14 @@ static StepArgs<T> step_args(odis_ctx *ctx, Ops<T> &o) {
   a.cyc_loc = o.cyc_loc;
   a.n_patch = ctx->n_patch;
   a.n_blocks = step_blocks(ctx);
+  a.n_inst = ctx->B;
+  a.ipb = step_instances(ctx);
   return a;
 }
 
 template <typename T, int P, int LPC, bool MULTI>
 static void launch_step_kernel(odis_ctx *ctx, const StepArgs<T> &a, const double *table, int s, hipEvent_t *ev) {
-  const dim3 gp(a.n_blocks, ctx->B), bp(LPC * P);
+  const dim3 gp(a.n_blocks, (ctx->B + a.ipb - 1) / a.ipb), bp(LPC * P);
   if (s == 0) {
     if (ev)
       hipExtLaunchKernelGGL((k_step<T, false, P, LPC, MULTI>), gp, bp, 0, ctx->stream, ev[0], ev[1], 0, a,
@@ -2981,6 +3078,13 @@ extern "C" int odis_set_option(odis_ctx *ctx, int option, int value) {
     ctx->sh_cpt = value;
     return ODIS_OK;
   }
+  if (option == ODIS_OPT_STEP_INSTANCES) {
+    if (value < 0) return fail(ctx, ODIS_E_INVALID, "step instances < 0");
+    CK(hipStreamSynchronize(ctx->stream));
+    drop_graphs(ctx);
+    ctx->step_inst = value;
+    return ODIS_OK;
+  }
   if (option == ODIS_OPT_STEP_LANES) {
     if (value != 2 && value != 4) return fail(ctx, ODIS_E_INVALID, "step lanes must be 2 or 4");
     CK(hipStreamSynchronize(ctx->stream));
@@ -3018,6 +3122,7 @@ extern "C" int odis_get_option(odis_ctx *ctx, int option, int *value) {
   else if (option == ODIS_OPT_STEP_BLOCKS) *value = step_blocks(ctx);
   else if (option == ODIS_OPT_STEP_LANES) *value = ctx->step_lanes;
   else if (option == ODIS_OPT_SH_CELLS) *value = ctx->sh_cpt;
+  else if (option == ODIS_OPT_STEP_INSTANCES) *value = step_instances(ctx);
   else return fail(ctx, ODIS_E_INVALID, "unknown option");
   return ODIS_OK;
 }
@@ -3232,7 +3337,7 @@ extern "C" int odis_sh_set_basis(odis_ctx *ctx, int l_max, const double *beta, i
   ctx->sh_Yt = ctx->sh_Minv = ctx->sh_D = ctx->sh_b = ctx->sh_c = ctx->sh_tab = ctx->sh_trig = nullptr;
   ctx->sh_tabm = ctx->sh_cmaj = ctx->sh_chain = nullptr;
   ctx->sh_Yp = nullptr;
-  ctx->sh_Ya = ctx->sh_Ys = nullptr;
+  ctx->sh_Ya = ctx->sh_Ys = ctx->sh_Xa = nullptr;
   ctx->sh_Cs = nullptr;
   ctx->sh_Kp = ctx->sh_qpr = ctx->sh_rng = 0;
   ctx->eta_eff = nullptr;
@@ -3389,6 +3494,9 @@ extern "C" int odis_sh_set_basis(odis_ctx *ctx, int l_max, const double *beta, i
     const size_t cs_bytes = size_t(ctx->B) * KB * 4 * 16 * sizeof(uint16_t);
     if ((rc = alloc(reinterpret_cast<void **>(&ctx->sh_Cs), cs_bytes))) return rc;
     CK(hipMemset(ctx->sh_Cs, 0, cs_bytes));
+    // split eta of every instance tile
+    const size_t xa_bytes = size_t((ctx->B + 15) / 16) * NB32 * 64 * 32;
+    if ((rc = alloc(reinterpret_cast<void **>(&ctx->sh_Xa), xa_bytes))) return rc;
     // analysis ranges: at most 128 of at least 4 blocks of 32 cells
     ctx->sh_Kp = Kp;
     ctx->sh_qpr = std::max(4, (NB32 + 127) / 128);

@@ -51,6 +51,7 @@ struct TideDev {
 template <typename T>
 struct FaceArgs {
   int E, nb;                 // faces, workgroups per instance
+  int B;                     // instances
   const int *tri_idx;        // [10][E]
   const T *tri_coef;         // [10][E]
   const int *g_idx;          // [2][E]
