@@ -48,6 +48,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0  # dense bf16 matrix cores
 SH_DEGREE = 32
 SHELL_INPUT = """radius; 252.1e3; top radius;
 k2; 0.0; k2;
@@ -262,6 +263,32 @@ def sweep_run(G, args):
             "finite": ok}
 
 
+def mfma_flops(n_cells, l_max, n_inst):
+    """Dense bf16 flops one batched split-BF16 pass issues on the matrix
+    cores (three products per term, harmonics padded to 32, cells to 32 or
+    16, instances to 16), and the useful flops 2 K N n of the same product."""
+    K = (l_max + 1) ** 2
+    Kp = (K + 31) // 32 * 32
+    it = (n_inst + 15) // 16 * 16
+    ana = 3 * 2 * Kp * ((n_cells + 31) // 32 * 32) * it
+    syn = 3 * 2 * Kp * ((n_cells + 15) // 16 * 16) * it
+    return {"sh_analysis": ana, "sh_synth": syn, "useful": 2 * K * n_cells * n_inst}
+
+
+def mfma_utilisation(tm, n_cells, l_max, n_inst):
+    """Matrix-core utilisation of the two product passes against the dense
+    bf16 peak (2.5 PFLOP/s)."""
+    fl = mfma_flops(n_cells, l_max, n_inst)
+    out = {}
+    for k in ("sh_analysis", "sh_synth"):
+        ms = tm[k + "_ms"]
+        if ms > 0:
+            out[k] = {"issued_tflops": fl[k] / (ms * 1e-3) / 1e12,
+                      "useful_tflops": fl["useful"] / (ms * 1e-3) / 1e12,
+                      "frac_of_dense_bf16_peak": fl[k] / (ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS}
+    return out
+
+
 UNEVEN_SEED = 20261204
 UNEVEN_SH_DEGREE = 16
 
@@ -326,6 +353,8 @@ def uneven_run(G, args):
         if sh:
             row["sh"] = {"mfma": "split-BF16 matrix products", "recur": "on-the-fly recurrence"}[sh]
             row["kernel_ms"] = {k[:-3]: tm[k] for k in ("step_ms", "sh_analysis_ms", "sh_coef_ms", "sh_synth_ms")}
+            if sh == "mfma":
+                row["mfma"] = mfma_utilisation(tm, n_cells, L, len(ps))
         rows.append(row)
     return {"instances": args.uneven_instances, "seed": UNEVEN_SEED, "precision": "fp32 state, fp64 dissipation",
             "value": total_cells * n_steps / elapsed, "ms_per_step_all_groups": 1e3 * elapsed / n_steps,
@@ -407,10 +436,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--workload", choices=["shell", "free", "converge"], default="shell",
+    ap.add_argument("--workload", choices=["shell", "free", "converge", "sweep", "uneven"], default="shell",
                     help="shell: ocean under a 20 km ice shell with SH coupling to degree 32 (default); "
                          "free: free-surface ocean, stencils only; converge: time to converged dissipation "
-                         "of the plain-method case, FP64 and FP32")
+                         "of the plain-method case, FP64 and FP32; sweep / uneven: the batched extras alone")
     ap.add_argument("--alpha", type=float, default=1e-8, help="linear drag of the converge workload (s^-1)")
     ap.add_argument("--max-orbits", type=int, default=3000, help="orbit cap of the converge workload")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched-sweep extra")
@@ -472,6 +501,13 @@ def main():
 
     if args.workload == "converge":
         print(json.dumps(converge_run(G, args)), flush=True)
+        return
+    if args.workload in ("sweep", "uneven"):
+        r = sweep_run(G, args) if args.workload == "sweep" else uneven_run(G, args)
+        print(json.dumps({"metric": "cell-steps/s, batched FP32 (%s)" % args.workload, "value": r["value"],
+                          "unit": "cell-steps/s", "n_gpus": 1, "higher_is_better": True, "scaling": "weak",
+                          "vs_baseline": None, "dtype": "f32", "data": "synthetic (generated SCVT grids, rest start)",
+                          "config": r}), flush=True)
         return
 
     t0 = time.time()
@@ -619,6 +655,28 @@ def main():
                     "dissipation": abs(float(fd[0]) - float(diss[0])) / abs(float(diss[0]))},
             }
             fctx.close()
+            # FP32 stencils with the SH passes as split-BF16 products on the
+            # matrix cores (one instance: each pass reads the stored basis once)
+            mctx = G.Context(mesh, p, precision=G.FP32)
+            apply_launch_options(G, mctx, args)
+            mctx.set_sh(SH_DEGREE, beta, mode=G.SH_MFMA)
+            m_steps = args.steps
+            mel, mtm = timed_run(mctx, m_steps, args.warmup, args.timing_stride)
+            ms_ = mctx.get_state()
+            md = mctx.dissipation()
+            out["config"]["fp32_mfma_form"] = {
+                "sh": "split-bf16 matrix products (stored basis in fragment order, three products per term, fp32 "
+                      "accumulation, fp64 partial sums and coupling)",
+                "value": mesh.N * m_steps / mel, "ms_per_step": 1e3 * mel / m_steps, "steps": m_steps,
+                "kernel_ms": {"step": mtm["step_ms"], "sh_analysis": mtm["sh_analysis_ms"],
+                              "sh_coef": mtm["sh_coef_ms"], "sh_synth": mtm["sh_synth_ms"]},
+                "mfma": mfma_utilisation(mtm, mesh.N, SH_DEGREE, 1),
+                "basis_bytes_per_pass": ((K + 31) // 32 * 32) * ((mesh.N + 31) // 32 * 32) * 4,
+                "rel_err_vs_fp64_after_%d_steps" % (args.steps + args.warmup): {
+                    "eta": rel_l2(ms_["eta"], state["eta"]), "v": rel_l2(ms_["v"], state["v"]),
+                    "dissipation": abs(float(md[0]) - float(diss[0])) / abs(float(diss[0]))},
+            }
+            mctx.close()
         if shell:
             # the stencils alone: free-surface ocean at its own stable step
             pf, _ = G.make_params(time_step=6.5)

@@ -306,8 +306,6 @@ struct StepArgs {
   const unsigned short *cyc_loc;  // [6][N] cycle faces, patch-local index
   int n_patch;
   int n_blocks;                 // workgroups per instance (= gridDim.x); each runs a contiguous run of patches
-  int n_inst;                   // instances
-  int ipb;                      // instances per workgroup, one after the other (gridDim.y = ceil(n_inst / ipb))
 };
 
 // One patch of the one-kernel step; returns this lane's dissipation
@@ -382,37 +380,32 @@ __global__ __launch_bounds__(LPC * P) void k_step(StepArgs<T> a, int energy_quad
   __shared__ T lv[6 * P];
   __shared__ int gid[6 * P];
   __shared__ double sh[NT / 64];
+  const int b = blockIdx.y;
   const long long n = (long long)table[0] + s;
   const int t = xcd_tile(blockIdx.x, a.n_blocks);
-  // the workgroup's instances one after the other: the operators of its
-  // patches come from the MALL once and from L1 / L2 afterwards
-  const int b0 = blockIdx.y * a.ipb, b1 = min(a.n_inst, b0 + a.ipb);
-  for (int b = b0; b < b1; ++b) {
-    if (b > b0) __syncthreads();  // the previous instance's phases are done with lv / gid / sh
-    double local = 0.0;
-    if (MULTI) {
-      const int iters = (a.n_patch + a.n_blocks - 1) / a.n_blocks;
-      for (int it = 0; it < iters; ++it) {
-        const int j = t * iters + it;
-        if (j >= a.n_patch) break;    // uniform in the workgroup
-        if (it > 0) __syncthreads();  // the previous patch's cell phase has read lv / gid
-        local += step_patch<T, P, LPC>(a, j, b, n, energy_quad, table, s, lv, gid);
-      }
-    } else {
-      local = step_patch<T, P, LPC>(a, t, b, n, energy_quad, table, s, lv, gid);
+  double local = 0.0;
+  if (MULTI) {
+    const int iters = (a.n_patch + a.n_blocks - 1) / a.n_blocks;
+    for (int it = 0; it < iters; ++it) {
+      const int j = t * iters + it;
+      if (j >= a.n_patch) break;    // uniform in the workgroup
+      if (it > 0) __syncthreads();  // the previous patch's cell phase has read lv / gid
+      local += step_patch<T, P, LPC>(a, j, b, n, energy_quad, table, s, lv, gid);
     }
-    const double part = block_sum<NT / 64>(local, sh);
-    if (threadIdx.x == 0) a.f.partials[((n & 1) * a.n_inst + b) * a.n_blocks + t] = part;
-    if (!PENDING || blockIdx.x != 0) continue;  // uniform in the workgroup
-    // dissipation of step n - 1: partials of the previous launch
-    __syncthreads();
-    const double tot = sum_partials<NT / 64>(
-        a.f.partials + (((n - 1) & 1) * a.n_inst + b) * size_t(a.n_blocks), a.n_blocks, sh);
-    if (threadIdx.x == 0) {
-      const double d = tot * a.c.dconst[b];
-      a.c.series[size_t(b) * a.c.cap + size_t((n - 1) % a.c.cap)] = d;
-      if (!isfinite(d)) a.c.nonfinite[b] = 1;
-    }
+  } else {
+    local = step_patch<T, P, LPC>(a, t, b, n, energy_quad, table, s, lv, gid);
+  }
+  const double part = block_sum<NT / 64>(local, sh);
+  if (threadIdx.x == 0) a.f.partials[((n & 1) * gridDim.y + b) * a.n_blocks + t] = part;
+  if (!PENDING || blockIdx.x != 0) return;
+  // dissipation of step n - 1: partials of the previous launch
+  __syncthreads();
+  const double tot = sum_partials<NT / 64>(
+      a.f.partials + (((n - 1) & 1) * gridDim.y + b) * size_t(a.n_blocks), a.n_blocks, sh);
+  if (threadIdx.x == 0) {
+    const double d = tot * a.c.dconst[b];
+    a.c.series[size_t(b) * a.c.cap + size_t((n - 1) % a.c.cap)] = d;
+    if (!isfinite(d)) a.c.nonfinite[b] = 1;
   }
 }
 
@@ -1641,7 +1634,6 @@ struct odis_ctx {
   int patch = 64;              // cells per patch (ODIS_OPT_PATCH_CELLS: 64, 128, 256)
   int step_blocks = 0;         // workgroups of the one-kernel step (ODIS_OPT_STEP_BLOCKS; 0: one per patch)
   int step_lanes = 4;          // lanes per cell of the one-kernel step (ODIS_OPT_STEP_LANES: 2 or 4)
-  int step_inst = 0;           // instances per workgroup of the one-kernel step (ODIS_OPT_STEP_INSTANCES; 0: auto)
   std::string one_kernel_why;
   std::vector<int> gi_host, cyc_host;  // device-order face cells and cell cycles, to rebuild patches
   long long step = 0;          // next step index n, shared by all instances
@@ -2653,13 +2645,6 @@ static int step_blocks(const odis_ctx *ctx) {
   return ctx->step_blocks > 0 ? std::min(ctx->step_blocks, ctx->n_patch) : ctx->n_patch;
 }
 
-// Instances per workgroup of the one-kernel step: the option, else 1 for a
-// single instance and kStepInstances for a batch.
-constexpr int kStepInstances = 4;
-static int step_instances(const odis_ctx *ctx) {
-  return std::min(ctx->B, ctx->step_inst > 0 ? ctx->step_inst : (ctx->B > 1 ? kStepInstances : 1));
-}
-
 template <typename T>
 static StepArgs<T> step_args(odis_ctx *ctx, Ops<T> &o) {
   StepArgs<T> a;
@@ -2672,14 +2657,12 @@ static StepArgs<T> step_args(odis_ctx *ctx, Ops<T> &o) {
   a.cyc_loc = o.cyc_loc;
   a.n_patch = ctx->n_patch;
   a.n_blocks = step_blocks(ctx);
-  a.n_inst = ctx->B;
-  a.ipb = step_instances(ctx);
   return a;
 }
 
 template <typename T, int P, int LPC, bool MULTI>
 static void launch_step_kernel(odis_ctx *ctx, const StepArgs<T> &a, const double *table, int s, hipEvent_t *ev) {
-  const dim3 gp(a.n_blocks, (ctx->B + a.ipb - 1) / a.ipb), bp(LPC * P);
+  const dim3 gp(a.n_blocks, ctx->B), bp(LPC * P);
   if (s == 0) {
     if (ev)
       hipExtLaunchKernelGGL((k_step<T, false, P, LPC, MULTI>), gp, bp, 0, ctx->stream, ev[0], ev[1], 0, a,
@@ -3078,13 +3061,6 @@ extern "C" int odis_set_option(odis_ctx *ctx, int option, int value) {
     ctx->sh_cpt = value;
     return ODIS_OK;
   }
-  if (option == ODIS_OPT_STEP_INSTANCES) {
-    if (value < 0) return fail(ctx, ODIS_E_INVALID, "step instances < 0");
-    CK(hipStreamSynchronize(ctx->stream));
-    drop_graphs(ctx);
-    ctx->step_inst = value;
-    return ODIS_OK;
-  }
   if (option == ODIS_OPT_STEP_LANES) {
     if (value != 2 && value != 4) return fail(ctx, ODIS_E_INVALID, "step lanes must be 2 or 4");
     CK(hipStreamSynchronize(ctx->stream));
@@ -3122,7 +3098,6 @@ extern "C" int odis_get_option(odis_ctx *ctx, int option, int *value) {
   else if (option == ODIS_OPT_STEP_BLOCKS) *value = step_blocks(ctx);
   else if (option == ODIS_OPT_STEP_LANES) *value = ctx->step_lanes;
   else if (option == ODIS_OPT_SH_CELLS) *value = ctx->sh_cpt;
-  else if (option == ODIS_OPT_STEP_INSTANCES) *value = step_instances(ctx);
   else return fail(ctx, ODIS_E_INVALID, "unknown option");
   return ODIS_OK;
 }

@@ -254,13 +254,7 @@ int odis_request_stop(odis_ctx *ctx);
  *                    compiled kernels, the fewest even count >= 4 whose grid
  *                    fits the workgroups the device holds at once; 8 for the
  *                    others), 4, 8 or 16; the coefficients depend on it by
- *                    rounding only.
- *  ODIS_OPT_STEP_INSTANCES instances each workgroup of the one-kernel step
- *                    runs one after the other, so that the operators of its
- *                    patches are read from memory once for all of them: 0
- *                    (default: 1 for one instance, 4 for a batch) or any
- *                    count >= 1 (reading it returns the count in use).  The
- *                    results are the same bit for bit. */
+ *                    rounding only. */
 #define ODIS_OPT_GRAPHS     1
 #define ODIS_OPT_COMPACT    2
 #define ODIS_OPT_ONE_KERNEL 3
@@ -268,7 +262,6 @@ int odis_request_stop(odis_ctx *ctx);
 #define ODIS_OPT_STEP_BLOCKS 5
 #define ODIS_OPT_STEP_LANES  6
 #define ODIS_OPT_SH_CELLS    7
-#define ODIS_OPT_STEP_INSTANCES 8
 int odis_set_option(odis_ctx *ctx, int option, int value);
 int odis_get_option(odis_ctx *ctx, int option, int *value);
 /* Timing for benchmarks: with stride T > 0, the kernels of every step n of

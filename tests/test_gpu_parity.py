@@ -629,36 +629,6 @@ def test_one_kernel_step_matches_face_cell_pair(product, precision):
             _assert_same_state(one, pair, (0, 1), (level, "patch", cells, lanes, blocks))
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_step_instances_per_workgroup(product, precision):
-    """The one-kernel step runs each workgroup's instances one after the
-    other (ODIS_OPT_STEP_INSTANCES); any count per workgroup leaves the same
-    state and dissipation series, bit for bit, for a batch of 10 instances of
-    different thickness and drag (graph replays included)."""
-    G = product
-    g, _ = generated_grid(5)
-    ps = [G.make_params(time_step=30.0, h=10e3 + 4e3 * i, alpha=10.0 ** (-8 - 0.3 * i))[0] for i in range(10)]
-    mesh = G.Mesh(g, ps[0].radius)
-    ctxs = []
-    for ipb in (1, 3, 4, 7):
-        c = G.Context(mesh, ps, precision=precision)
-        c.set_option(G.OPT_STEP_INSTANCES, ipb)
-        assert c.option(G.OPT_STEP_INSTANCES) == ipb
-        ctxs.append(c)
-    ctxs[-1].use_graphs(True)
-    for n in (1, 2, 97, 64):
-        for c in ctxs:
-            c.step(n)
-        for c in ctxs[1:]:
-            _assert_same_state(ctxs[0], c, range(10), ("ipb", n))
-            assert np.array_equal(ctxs[0].dissipation(), c.dissipation())
-    first = ctxs[0].get_state(3)["iter"] - 150
-    for c in ctxs[1:]:
-        assert np.array_equal(ctxs[0].dissipation_series(3, first, 150), c.dissipation_series(3, first, 150))
-    for c in ctxs:
-        c.close()
-
-
 @pytest.mark.parametrize("precision,L", [(0, 16), (1, 16), (0, 12), (1, 12), (0, 8)])
 def test_sh_recurrence_matches_stored_basis(product, precision, L):
     """The basis evaluated on the fly and the stored basis give the same
