@@ -1304,23 +1304,24 @@ __device__ __forceinline__ void split8(const float (&f)[8], bf16x8 &hi, bf16x8 &
 
 // eta of the batch -> the analysis A operands, Xa[it][q][lane] = 8 hi, then
 // 8 lo bf16 of instance 16 it + l % 16 at cells 32 q + 8 (l / 16) .. + 7
-// (zero past nb and N).  One thread per (instance, run of 8 cells), threads
-// in cell order, so the loads of a wave are one contiguous stretch.
+// (zero past nb and N).  One wave per fragment (it, q): lane l reads its 8
+// cells of one instance (16 rows of 128 contiguous bytes per wave) and the
+// wave writes the fragment as 2 KB in one piece.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_sh_pack_mfma(int N, int NB32, int nb, const T *__restrict__ x,
-                                                         long long x_stride, uint4 *__restrict__ Xa) {
-  const int n8 = NB32 * 4;
-  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
-  const int j = int(t / n8), c8 = int(t % n8);
-  if (j >= (nb + 15) / 16 * 16) return;
-  const int c0 = 8 * c8;
+__global__ __launch_bounds__(kBlock) void k_sh_pack_mfma(int N, int NB32, int n_frag, int nb,
+                                                         const T *__restrict__ x, long long x_stride,
+                                                         uint4 *__restrict__ Xa) {
+  const int f = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (f >= n_frag) return;
+  const int it = f / NB32, q = f % NB32;
+  const int j = 16 * it + (lane & 15), c0 = 32 * q + 8 * (lane >> 4);
   const T *xr = x + (long long)min(j, nb - 1) * x_stride;
-  float f[8];
+  float v[8];
 #pragma unroll
-  for (int i = 0; i < 8; ++i) f[i] = (j < nb && c0 + i < N) ? float(xr[c0 + i]) : 0.f;
+  for (int i = 0; i < 8; ++i) v[i] = (j < nb && c0 + i < N) ? float(xr[c0 + i]) : 0.f;
   bf16x8 hi, lo;
-  split8(f, hi, lo);
-  uint4 *dst = Xa + ((size_t(j >> 4) * NB32 + (c8 >> 2)) * 64 + (j & 15) + 16 * (c8 & 3)) * 2;
+  split8(v, hi, lo);
+  uint4 *dst = Xa + (size_t(f) * 64 + lane) * 2;
   dst[0] = __builtin_bit_cast(uint4, hi);
   dst[1] = __builtin_bit_cast(uint4, lo);
 }
@@ -2468,9 +2469,9 @@ static void launch_mfma_t(odis_ctx *ctx, bool ana, int nb, const T *x, const T *
   const int N = ctx->N, NB32 = (N + 31) / 32, NT16 = (N + 15) / 16, n_it = (nb + 15) / 16;
   if (ana) {
     const int KT = ctx->sh_Kp / 16;
-    const long long threads = (long long)n_it * 16 * NB32 * 4;
-    hipLaunchKernelGGL((k_sh_pack_mfma<T>), dim3(unsigned((threads + kBlock - 1) / kBlock)), dim3(kBlock), 0,
-                       ctx->stream, N, NB32, nb, x, stride, ctx->sh_Xa);
+    const int n_frag = n_it * NB32;
+    hipLaunchKernelGGL((k_sh_pack_mfma<T>), dim3((n_frag + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), 0,
+                       ctx->stream, N, NB32, n_frag, nb, x, stride, ctx->sh_Xa);
     const dim3 grid(ctx->sh_rng, (KT + 4 * W - 1) / (4 * W), (n_it + NIT - 1) / NIT);
     hipExtLaunchKernelGGL((k_sh_ana_mfma<NIT, W>), grid, dim3(256), 0, ctx->stream, e0, e1, 0, ctx->sh_K, KT, NB32,
                           ctx->sh_qpr, ctx->sh_rng, ctx->sh_Ya, ctx->sh_Xa, nb, ctx->sh_b);
@@ -2559,8 +2560,8 @@ static int sh_apply(odis_ctx *ctx, const T *x, long long stride, int nb, const d
     const double *bsum = ctx->sh_b + size_t(nb) * n_part * K;
     hipExtLaunchKernelGGL(k_sh_bsum, dim3((K + kBlock - 1) / kBlock, nb), blk, 0, ctx->stream,
                           ev ? ev[2] : nullptr, nullptr, 0, K, n_part, ctx->sh_b);
-    hipExtLaunchKernelGGL(k_sh_coef, dim3((K + 3) / 4, nb), blk, 0, ctx->stream, nullptr,
-                          ev ? ev[3] : nullptr, 0, K, ctx->sh_Minv, D, bsum, c, chain,
+    hipExtLaunchKernelGGL(k_sh_coef, dim3((K + 3) / 4, nb), blk, 0, ctx->stream, nullptr, ev ? ev[3] : nullptr,
+                          0, K, ctx->sh_Minv, D, bsum, c, chain,
                           ctx->sh_mode == ODIS_SH_RECUR ? ctx->sh_cmaj : static_cast<double *>(nullptr),
                           mfma ? ctx->sh_Cs : static_cast<unsigned short *>(nullptr), ctx->sh_Kp / 32);
   }
