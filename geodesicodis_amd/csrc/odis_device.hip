@@ -48,6 +48,14 @@ __device__ __forceinline__ double sum_partials(const double *p, int count, doubl
   return block_sum<NW>(acc, sh);
 }
 
+// Index of element i of instance b in a per-instance array: instance-major
+// (b stride + i) or, for a batch held interleaved (IL), element-major
+// (i stride + b, stride = the padded instance count).
+template <bool IL>
+__device__ __forceinline__ long long sidx(int b, long long stride, long long i) {
+  return IL ? i * stride + b : b * stride + i;
+}
+
 // Result of the momentum update of one face.
 template <typename T>
 struct FaceOut {
@@ -62,23 +70,26 @@ struct FaceOut {
 //   v' = v + dt AB3(f0) + dt (-alpha v_e + (G U)_e)   (src/timeIntegrator.cpp:207-232)
 // COMPACT: v_t from the TRiSK partial sums and G U from the forcing fields at
 // the cells; else the tabulated stencil and per-face forcing gradients.
-template <typename T, bool DISS_ONLY, bool COMPACT>
+// IL (compact form only): the state is held interleaved; instances past
+// a.B (padding of the last group) take the parameters of the last one.
+template <typename T, bool DISS_ONLY, bool COMPACT, bool IL = false>
 __device__ __forceinline__ FaceOut<T> face_update(const FaceArgs<T> &a, int e, int b, int energy_quad,
                                                   const double *table, int s) {
+  static_assert(COMPACT || !IL, "the interleaved layout has the compact form only");
   const int E = a.E;
-  const double *ip = a.inst + b * 8;
-  const T *v = a.v_cur + b * a.v_stride;
+  const int bp = IL ? min(b, a.B - 1) : b;
+  const double *ip = a.inst + bp * 8;
   FaceOut<T> out;
   T vt;
   if (COMPACT) {
-    const T *ts = a.ts + b * a.ts_stride;
-    vt = (ts[e] - ts[E + e]) / a.fdist[e];
+    vt = (a.ts[sidx<IL>(b, a.ts_stride, e)] - a.ts[sidx<IL>(b, a.ts_stride, E + e)]) / a.fdist[e];
   } else {
+    const T *v = a.v_cur + b * a.v_stride;
     vt = T(0);
 #pragma unroll
     for (int j = 0; j < 10; ++j) vt += a.tri_coef[j * E + e] * v[a.tri_idx[j * E + e]];
   }
-  const T ve = v[e];
+  const T ve = a.v_cur[sidx<IL>(b, a.v_stride, e)];
   const T sp = ve * ve + vt * vt;
   const T en = energy_quad ? sqrt(sp) * sp : sp;
   out.energy = double(a.farea[e]) * double(en);
@@ -89,10 +100,9 @@ __device__ __forceinline__ FaceOut<T> face_update(const FaceArgs<T> &a, int e, i
     const double *coef = table + 8 + size_t(s) * a.B * 8;
     const T g = T(ip[0]), alpha = T(ip[1]), dt = T(ip[2]), qd = T(ip[4]);
     const bool ab3 = (n >= 2) || (ip[3] != 0.0);
-    const T *eta = a.eta + b * a.eta_stride;
     const int c0 = a.g_idx[e], c1 = a.g_idx[E + e];
     const T gc0 = a.g_coef[e], gc1 = a.g_coef[E + e];
-    const T ge = gc0 * eta[c0] + gc1 * eta[c1];
+    const T ge = gc0 * a.eta[sidx<IL>(b, a.eta_stride, c0)] + gc1 * a.eta[sidx<IL>(b, a.eta_stride, c1)];
     T f0 = -g * ge + a.fcor[e] * vt;
     if (a.quad_drag) f0 -= qd * sqrt(sp) * ve;
     // G U, U = sum_k c_k(t + dt) S_k (src/timeIntegrator.cpp:219)
@@ -100,7 +110,7 @@ __device__ __forceinline__ FaceOut<T> face_update(const FaceArgs<T> &a, int e, i
     if (COMPACT) {
       T u0 = T(0), u1 = T(0);
       for (int k = 0; k < a.nfields; ++k) {
-        const T ck = T(coef[b * 8 + k]);
+        const T ck = T(coef[bp * 8 + k]);
         u0 += ck * a.cellS[k * a.N + c0];
         u1 += ck * a.cellS[k * a.N + c1];
       }
@@ -110,11 +120,12 @@ __device__ __forceinline__ FaceOut<T> face_update(const FaceArgs<T> &a, int e, i
       for (int k = 0; k < a.nfields; ++k) fu += T(coef[b * 8 + k]) * a.fgs[k * E + e];
     }
     const T drag = -alpha * ve + fu;
-    const T *ring = a.ring + b * a.ring_stride;
     const int s1 = int((n + 2) % 3), s2 = int((n + 1) % 3);
     T upd;
     if (ab3)
-      upd = (T(23. / 12.) * f0 + T(-16. / 12.) * ring[s1 * E + e] + T(5. / 12.) * ring[s2 * E + e]) * dt;
+      upd = (T(23. / 12.) * f0 + T(-16. / 12.) * a.ring[sidx<IL>(b, a.ring_stride, (long long)s1 * E + e)] +
+             T(5. / 12.) * a.ring[sidx<IL>(b, a.ring_stride, (long long)s2 * E + e)]) *
+            dt;
     else
       upd = f0 * dt;
     out.f0 = f0;
@@ -155,8 +166,9 @@ __global__ __launch_bounds__(kBlock) void k_face(FaceArgs<T> a, int energy_quad,
 // T_p = sum_{j=1}^{K-1} (R_p + ... + R_{p+j-1} - 1/2) F_{p+j}, F_q = dir_q l_q v_q
 // (indices mod K).  The running sum of kite fractions starts at the face
 // itself, in the order of the reference's weight loop (src/mesh.cpp:677-910).
-template <typename T, int K>
-__device__ __forceinline__ void trisk_partials(const T *F, const T *R, const int *f, int db, T *ts, int E) {
+template <typename T, int K, bool IL>
+__device__ __forceinline__ void trisk_partials(const T *F, const T *R, const int *f, int db, T *ts, int E, int b,
+                                               long long stride) {
 #pragma unroll
   for (int p = 0; p < K; ++p) {
     T S = R[p];
@@ -166,26 +178,25 @@ __device__ __forceinline__ void trisk_partials(const T *F, const T *R, const int
       S += R[(p + j - 1) % K];
       acc += (S - T(0.5)) * F[(p + j) % K];
     }
-    ts[((db >> p) & 1) * E + f[p]] = acc;
+    ts[sidx<IL>(b, stride, (long long)((db >> p) & 1) * E + f[p])] = acc;
   }
 }
 
 // TRiSK partial sums of a cell with k faces.
-template <typename T>
+template <typename T, bool IL = false>
 __device__ __forceinline__ void cell_outputs(const CellArgs<T> &a, int b, int k, int db, const int *f, const T *F,
                                              const T *R) {
-  T *ts = a.ts + b * a.ts_stride;
   if (k == 6)
-    trisk_partials<T, 6>(F, R, f, db, ts, a.E);
+    trisk_partials<T, 6, IL>(F, R, f, db, a.ts, a.E, b, a.ts_stride);
   else
-    trisk_partials<T, 5>(F, R, f, db, ts, a.E);
+    trisk_partials<T, 5, IL>(F, R, f, db, a.ts, a.E, b, a.ts_stride);
 }
 
 // Continuity + AB3 of cell i: eta^{n+1} = eta^n + dt AB3(h Div v^{n+1})
 // (src/updateEta.cpp, src/temporalOperators.cpp:36-47), history slot n % 3.
 // eta_old = eta^n, r1 / r2 = history slots (n+2) % 3 and (n+1) % 3, already
 // loaded.
-template <typename T>
+template <typename T, bool IL = false>
 __device__ __forceinline__ T continuity_loaded(const CellArgs<T> &a, int i, int b, long long n, T div,
                                                T eta_old, T r1, T r2) {
   const double *ip = a.inst + b * 8;
@@ -197,9 +208,9 @@ __device__ __forceinline__ T continuity_loaded(const CellArgs<T> &a, int i, int 
     upd = (T(23. / 12.) * f0 + T(-16. / 12.) * r1 + T(5. / 12.) * r2) * dt;
   else
     upd = f0 * dt;
-  a.ring[b * a.ring_stride + int(n % 3) * a.N + i] = f0;
+  a.ring[sidx<IL>(b, a.ring_stride, (long long)int(n % 3) * a.N + i)] = f0;
   const T eta_new = eta_old + upd;
-  a.eta_nxt[b * a.eta_stride + i] = eta_new;
+  a.eta_nxt[sidx<IL>(b, a.eta_stride, i)] = eta_new;
   return eta_new;
 }
 
@@ -407,6 +418,138 @@ __global__ __launch_bounds__(LPC * P) void k_step(StepArgs<T> a, int energy_quad
     a.c.series[size_t(b) * a.c.cap + size_t((n - 1) % a.c.cap)] = d;
     if (!isfinite(d)) a.c.nonfinite[b] = 1;
   }
+}
+
+// One-kernel step of a batch held interleaved: every per-instance array is
+// element-major with the instances innermost ([element][Bp], Bp = the
+// instance count padded to IB).  Workgroup (patch j, group of IB
+// instances); lane l serves instance l % IB of the group.  The patch's faces
+// go NT / IB at a time, so each face's operators are loaded once for IB
+// instances and its state is IB consecutive values; then its (cell,
+// instance) pairs.  The arithmetic of a face and a cell is step_patch's, so
+// the state is the same bit for bit; an instance's dissipation partial adds
+// its faces in another grouping.  Padding instances (b >= B) compute with
+// the last instance's parameters and store nothing.
+template <typename T, bool PENDING, int P, int IB, int FU, int CU>
+__global__ __launch_bounds__(256) void k_step_il(StepArgs<T> a, int energy_quad, const double *table, int s) {
+  constexpr int NT = 256, SL = NT / IB;
+  __shared__ T lv[6 * P * IB];
+  __shared__ int gid[6 * P];
+  __shared__ double red[SL * IB];
+  __shared__ double sh[NT / 64];
+  const int B = a.f.B, E = a.f.E, N = a.f.N;
+  const int ib = threadIdx.x % IB, sl = threadIdx.x / IB;
+  const int b = blockIdx.y * IB + ib;
+  const bool live = b < B;
+  const long long n = (long long)table[0] + s;
+  const int j = xcd_tile(blockIdx.x, a.n_blocks);
+  const int lo = a.own_lo[j], n_own = a.own_lo[j + 1] - lo;
+  const int flo = a.for_ptr[j], n_loc = n_own + (a.for_ptr[j + 1] - flo);
+  double local = 0.0;
+  // FU faces per lane are updated before any of them is stored, so that
+  // their loads are in flight together (the stores could alias them)
+  for (int q0 = sl; q0 < n_loc; q0 += SL * FU) {
+    FaceOut<T> r[FU];
+    int e[FU];
+#pragma unroll
+    for (int u = 0; u < FU; ++u) {
+      const int q = min(q0 + u * SL, n_loc - 1);  // past the end: the last face again, not stored
+      e[u] = q < n_own ? lo + q : a.for_idx[flo + q - n_own];
+      r[u] = face_update<T, false, true, true>(a.f, e[u], b, energy_quad, table, s);
+    }
+#pragma unroll
+    for (int u = 0; u < FU; ++u) {
+      const int q = q0 + u * SL;
+      if (q >= n_loc) break;
+      if (q < n_own && live) {
+        local += r[u].energy;
+        a.f.ring[sidx<true>(b, a.f.ring_stride, (long long)int(n % 3) * E + e[u])] = r[u].f0;
+        a.f.v_nxt[sidx<true>(b, a.f.v_stride, e[u])] = r[u].vn;
+      }
+      lv[q * IB + ib] = a.c.flen[e[u]] * r[u].vn;
+      if (ib == 0) gid[q] = e[u];
+    }
+  }
+  __syncthreads();
+  const int c0 = a.cell_lo[j], nc = a.cell_lo[j + 1] - c0;
+  // CU (cell, instance) pairs per lane loaded together, then updated
+  if (live)
+    for (int p0 = threadIdx.x; p0 < nc * IB; p0 += NT * CU) {  // p % IB == ib
+      int info[CU], lq[CU][6];
+      T R[CU][6], eta_old[CU], r1[CU], r2[CU], area[CU];
+#pragma unroll
+      for (int u = 0; u < CU; ++u) {
+        const int i = c0 + min(p0 + u * NT, nc * IB - 1) / IB;
+        info[u] = a.c.info[i];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+          lq[u][q] = a.cyc_loc[q * N + i];
+          R[u][q] = a.c.kite[q * N + i];
+        }
+        area[u] = a.c.area[i];
+        eta_old[u] = a.c.eta[sidx<true>(b, a.c.eta_stride, i)];
+        r1[u] = a.c.ring[sidx<true>(b, a.c.ring_stride, (long long)int((n + 2) % 3) * N + i)];
+        r2[u] = a.c.ring[sidx<true>(b, a.c.ring_stride, (long long)int((n + 1) % 3) * N + i)];
+      }
+#pragma unroll
+      for (int u = 0; u < CU; ++u) {
+        const int p = p0 + u * NT;
+        if (p >= nc * IB) break;
+        const int i = c0 + p / IB;
+        int f[6];
+        T F[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+          const int l = lq[u][q];
+          f[q] = gid[l];
+          const T x = lv[l * IB + ib];
+          F[q] = ((info[u] >> (3 + q)) & 1) ? -x : x;
+        }
+        const int k = info[u] & 7, db = info[u] >> 3;
+        continuity_loaded<T, true>(a.c, i, b, n, divergence(F, k, area[u]), eta_old[u], r1[u], r2[u]);
+        cell_outputs<T, true>(a.c, b, k, db, f, F, R[u]);
+      }
+    }
+  // the patch's dissipation partial of each instance: its face slots in order
+  red[sl * IB + ib] = local;
+  __syncthreads();
+  if (threadIdx.x < IB) {
+    double sum = 0.0;
+    for (int u = 0; u < SL; ++u) sum += red[u * IB + threadIdx.x];
+    const int bb = blockIdx.y * IB + threadIdx.x;
+    if (bb < B) a.f.partials[((n & 1) * B + bb) * a.n_blocks + j] = sum;
+  }
+  if (!PENDING || blockIdx.x != 0) return;
+  // dissipation of step n - 1 of the group's instances: partials of the previous launch
+  for (int u = 0; u < IB; ++u) {
+    const int bb = blockIdx.y * IB + u;
+    if (bb >= B) break;  // uniform in the workgroup
+    __syncthreads();     // sh is free
+    const double tot = sum_partials<NT / 64>(a.f.partials + (((n - 1) & 1) * B + bb) * size_t(a.n_blocks),
+                                             a.n_blocks, sh);
+    if (threadIdx.x == 0) {
+      const double d = tot * a.c.dconst[bb];
+      a.c.series[size_t(bb) * a.c.cap + size_t((n - 1) % a.c.cap)] = d;
+      if (!isfinite(d)) a.c.nonfinite[bb] = 1;
+    }
+  }
+}
+
+// Per-instance array of n elements between the instance-major layout
+// (std, [B][stride]) and the interleaved one (il, [n][Bp]): TO_IL copies std
+// into il, else il into std.  Padding instances are not touched.
+// Consecutive threads take consecutive instances of one element.
+template <typename T, bool TO_IL>
+__global__ __launch_bounds__(kBlock) void k_interleave(T *__restrict__ stdp, T *__restrict__ ilp, long long n,
+                                                       long long stride, int B, int Bp) {
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  const long long i = t / B;
+  const int b = int(t % B);
+  if (i >= n) return;
+  if (TO_IL)
+    ilp[i * Bp + b] = stdp[b * stride + i];
+  else
+    stdp[b * stride + i] = ilp[i * Bp + b];
 }
 
 // Dissipation of the last step of a chunk of k_step launches.
@@ -1635,6 +1778,11 @@ struct odis_ctx {
   int patch = 64;              // cells per patch (ODIS_OPT_PATCH_CELLS: 64, 128, 256)
   int step_blocks = 0;         // workgroups of the one-kernel step (ODIS_OPT_STEP_BLOCKS; 0: one per patch)
   int step_lanes = 4;          // lanes per cell of the one-kernel step (ODIS_OPT_STEP_LANES: 2 or 4)
+  int il_opt = 2;              // ODIS_OPT_INTERLEAVE: 0 off, 1 on, 2 auto (batches of >= kInterleaveMin)
+  int il_Bp = 0;               // instances of the interleaved copy, padded to kInterleaveGroup (0: none yet)
+  void *il_buf = nullptr;      // interleaved v[2], eta[2], ring_v, ring_e, ts[2]
+  int il_fresh = 0;            // the interleaved copy holds the current state
+  int std_fresh = 1;           // the instance-major arrays hold the current state
   std::string one_kernel_why;
   std::vector<int> gi_host, cyc_host;  // device-order face cells and cell cycles, to rebuild patches
   long long step = 0;          // next step index n, shared by all instances
@@ -2204,6 +2352,8 @@ extern "C" const char *odis_last_error(const odis_ctx *ctx) {
 // 36-65); after step m >= 1 it holds (f_m, f_m, f_{m-1}), except after an
 // Euler step 0 (f_0, ds1 as set, f_0).  set/get translate between the two.
 
+static int il_out_any(odis_ctx *ctx);  // the instance-major arrays up to date (batches held interleaved)
+
 template <typename T>
 static int put_state(odis_ctx *ctx, Ops<T> &o, int b, const double *v, const double *dv, const double *eta,
                      const double *de, long long k) {
@@ -2250,8 +2400,11 @@ extern "C" int odis_set_state(odis_ctx *ctx, int inst, const double *v, const do
   if (!ctx) return ODIS_E_INVALID;
   if (inst < 0 || inst >= ctx->B) return fail(ctx, ODIS_E_RANGE, "instance out of range");
   if (iter < 0) return fail(ctx, ODIS_E_INVALID, "iter < 0");
+  int rc = il_out_any(ctx);  // the other instances' state, when the batch stepped interleaved
+  if (rc) return rc;
   CK(hipStreamSynchronize(ctx->stream));
   ctx->step = iter;  // the step counter is shared by all instances of a context
+  ctx->il_fresh = 0;
   return ctx->precision == ODIS_FP64 ? put_state(ctx, ctx->d, inst, v, dvdt, eta, detadt, iter)
                                      : put_state(ctx, ctx->f, inst, v, dvdt, eta, detadt, iter);
 }
@@ -2304,6 +2457,8 @@ extern "C" int odis_get_state(odis_ctx *ctx, int inst, double *v, double *dvdt, 
                               long long *iter) {
   if (!ctx) return ODIS_E_INVALID;
   if (inst < 0 || inst >= ctx->B) return fail(ctx, ODIS_E_RANGE, "instance out of range");
+  const int rc = il_out_any(ctx);
+  if (rc) return rc;
   CK(hipStreamSynchronize(ctx->stream));
   if (iter) *iter = ctx->step;
   return ctx->precision == ODIS_FP64 ? get_state_t(ctx, ctx->d, inst, v, dvdt, eta, detadt, ctx->step)
@@ -2661,6 +2816,127 @@ static StepArgs<T> step_args(odis_ctx *ctx, Ops<T> &o) {
   return a;
 }
 
+// ---------------------------------------------------------------------------
+// batches held interleaved (k_step_il)
+// ---------------------------------------------------------------------------
+// A batch may step with its state interleaved ([element][Bp]); the
+// instance-major arrays are then brought up to date only when something
+// reads or writes them (state in / out, dissipation, outputs, the other
+// step kernels).  std_fresh / il_fresh say which copy holds the current
+// state (at least one always does).
+constexpr int kInterleaveGroup = 16;  // instances per workgroup of k_step_il
+// faces and (cell, instance) pairs per lane in flight in k_step_il: FP64
+// gains from 4 and 2 (0.287 against 0.303 ms for 64 instances at 40962
+// cells), FP32 loses occupancy by them (0.72 against 0.63 ms for 256)
+template <typename T>
+struct IlUnroll {
+  static constexpr int faces = sizeof(T) == 8 ? 4 : 1, cells = sizeof(T) == 8 ? 2 : 1;
+};
+constexpr int kInterleaveMin = 16;    // batch size from which the automatic choice interleaves
+constexpr int kInterleaveCells = 20000;  // ... on grids of at least this many cells (smaller: slower)
+
+static bool il_wanted(const odis_ctx *ctx) {
+  const bool on =
+      ctx->il_opt == 1 || (ctx->il_opt == 2 && ctx->B >= kInterleaveMin && ctx->N >= kInterleaveCells);
+  return on && ctx->compact && ctx->one_kernel && ctx->patch == 64 && ctx->sh_mode == ODIS_SH_OFF &&
+         step_blocks(ctx) == ctx->n_patch;
+}
+
+template <typename T>
+struct IlOps {
+  T *v[2], *eta[2], *ring_v, *ring_e, *ts[2];
+};
+
+template <typename T>
+static IlOps<T> il_ops(odis_ctx *ctx) {
+  const size_t E = ctx->E, N = ctx->N, Bp = ctx->il_Bp;
+  T *p = static_cast<T *>(ctx->il_buf);
+  IlOps<T> q;
+  q.v[0] = p;
+  q.v[1] = q.v[0] + E * Bp;
+  q.eta[0] = q.v[1] + E * Bp;
+  q.eta[1] = q.eta[0] + N * Bp;
+  q.ring_v = q.eta[1] + N * Bp;
+  q.ring_e = q.ring_v + 3 * E * Bp;
+  q.ts[0] = q.ring_e + 3 * N * Bp;
+  q.ts[1] = q.ts[0] + 2 * E * Bp;
+  return q;
+}
+
+template <typename T>
+static int il_alloc(odis_ctx *ctx) {
+  if (ctx->il_buf) return ODIS_OK;
+  const int Bp = (ctx->B + kInterleaveGroup - 1) / kInterleaveGroup * kInterleaveGroup;
+  const size_t bytes = (size_t(9) * ctx->E + size_t(5) * ctx->N) * Bp * sizeof(T);
+  CK(hipMalloc(&ctx->il_buf, bytes));
+  ctx->allocs.push_back(ctx->il_buf);
+  CK(hipMemset(ctx->il_buf, 0, bytes));  // padding instances read zeros
+  ctx->il_Bp = Bp;
+  return ODIS_OK;
+}
+
+// the current state between the layouts (v, eta and ts of parity cur, both rings)
+template <typename T>
+static int il_convert(odis_ctx *ctx, Ops<T> &o, bool to_il) {
+  const IlOps<T> q = il_ops<T>(ctx);
+  const int c = ctx->cur, B = ctx->B, Bp = ctx->il_Bp;
+  const long long E = ctx->E, N = ctx->N;
+  // (instance-major array [B][n], interleaved array [n][Bp], n)
+  struct {
+    T *stdp, *ilp;
+    long long n;
+  } arr[5] = {{o.v[c], q.v[c], E}, {o.eta[c], q.eta[c], N}, {o.ring_v, q.ring_v, 3 * E},
+              {o.ring_e, q.ring_e, 3 * N}, {o.ts[c], q.ts[c], 2 * E}};
+  for (const auto &x : arr) {
+    if (!x.stdp) continue;  // no partial sums without the compact form
+    const long long threads = x.n * B;
+    const dim3 g(unsigned((threads + kBlock - 1) / kBlock));
+    if (to_il)
+      hipLaunchKernelGGL((k_interleave<T, true>), g, dim3(kBlock), 0, ctx->stream, x.stdp, x.ilp, x.n, x.n, B, Bp);
+    else
+      hipLaunchKernelGGL((k_interleave<T, false>), g, dim3(kBlock), 0, ctx->stream, x.stdp, x.ilp, x.n, x.n, B,
+                         Bp);
+  }
+  CK(hipGetLastError());
+  return ODIS_OK;
+}
+
+// bring the instance-major arrays up to date (before anything uses them)
+template <typename T>
+static int il_out(odis_ctx *ctx, Ops<T> &o) {
+  if (ctx->std_fresh) return ODIS_OK;
+  const int rc = il_convert(ctx, o, false);
+  if (rc) return rc;
+  ctx->std_fresh = 1;
+  return ODIS_OK;
+}
+
+static int il_out_any(odis_ctx *ctx) {
+  return ctx->precision == ODIS_FP64 ? il_out(ctx, ctx->d) : il_out(ctx, ctx->f);
+}
+
+// step arguments on the interleaved copy
+template <typename T>
+static StepArgs<T> step_args_il(odis_ctx *ctx, Ops<T> &o) {
+  StepArgs<T> a = step_args(ctx, o);
+  const IlOps<T> q = il_ops<T>(ctx);
+  const int c = ctx->cur;
+  const long long Bp = ctx->il_Bp;
+  a.f.v_cur = q.v[c];
+  a.f.v_nxt = q.v[c ^ 1];
+  a.f.ring = q.ring_v;
+  a.f.eta = q.eta[c];
+  a.f.ts = q.ts[c];
+  a.f.v_stride = a.f.ring_stride = a.f.eta_stride = a.f.ts_stride = Bp;
+  a.c.v_new = q.v[c ^ 1];
+  a.c.eta = q.eta[c];
+  a.c.eta_nxt = q.eta[c ^ 1];
+  a.c.ring = q.ring_e;
+  a.c.ts = q.ts[c ^ 1];
+  a.c.v_stride = a.c.eta_stride = a.c.ring_stride = a.c.ts_stride = Bp;
+  return a;
+}
+
 template <typename T, int P, int LPC, bool MULTI>
 static void launch_step_kernel(odis_ctx *ctx, const StepArgs<T> &a, const double *table, int s, hipEvent_t *ev) {
   const dim3 gp(a.n_blocks, ctx->B), bp(LPC * P);
@@ -2724,6 +3000,7 @@ static int launch_chunk(odis_ctx *ctx, Ops<T> &o, const double *table, int count
     if (rc) return rc;
   }
   const bool one = ctx->compact && ctx->one_kernel;
+  const bool il = il_wanted(ctx);
   if (one && step_blocks(ctx) > ctx->part_cap)
     return fail(ctx, ODIS_E_INVALID, "more workgroups than dissipation partials");
   for (int s = 0; s < count; s++) {
@@ -2738,7 +3015,32 @@ static int launch_chunk(odis_ctx *ctx, Ops<T> &o, const double *table, int count
       const int rc = sh_effective_height(ctx, o, evsh);
       if (rc) return rc;
     }
-    if (one) {
+    if (one && il) {
+      constexpr int IB = kInterleaveGroup, FU = IlUnroll<T>::faces, CU = IlUnroll<T>::cells;
+      StepArgs<T> a = step_args_il(ctx, o);
+      hipEvent_t *ev = nullptr;
+      if (stamp) {
+        evs->mom.push_back(int(*ev_used));
+        ev = &ctx->ev[*ev_used];
+        *ev_used += 2;
+      }
+      const dim3 gp(a.n_blocks, ctx->il_Bp / IB), bp(256);
+      if (s == 0) {
+        if (ev)
+          hipExtLaunchKernelGGL((k_step_il<T, false, 64, IB, FU, CU>), gp, bp, 0, ctx->stream, ev[0], ev[1], 0, a,
+                                ctx->energy_quad, table, s);
+        else
+          hipLaunchKernelGGL((k_step_il<T, false, 64, IB, FU, CU>), gp, bp, 0, ctx->stream, a, ctx->energy_quad,
+                             table, s);
+      } else {
+        if (ev)
+          hipExtLaunchKernelGGL((k_step_il<T, true, 64, IB, FU, CU>), gp, bp, 0, ctx->stream, ev[0], ev[1], 0, a,
+                                ctx->energy_quad, table, s);
+        else
+          hipLaunchKernelGGL((k_step_il<T, true, 64, IB, FU, CU>), gp, bp, 0, ctx->stream, a, ctx->energy_quad,
+                             table, s);
+      }
+    } else if (one) {
       // the face phase reads eta_eff (face_args) when a basis is set
       StepArgs<T> a = step_args(ctx, o);
       hipEvent_t *ev = nullptr;
@@ -2824,8 +3126,18 @@ static int step_t(odis_ctx *ctx, Ops<T> &o, int n_steps) {
   std::vector<ChunkEvents> sampled;
   size_t ev_used = 0;
   int done = 0;
-  int rc = ensure_ts(ctx, o);
-  if (rc) return rc;
+  int rc = ODIS_OK;
+  const bool il = il_wanted(ctx);
+  if (il) {
+    if (!ctx->il_fresh) {  // the instance-major arrays are current: copy them in
+      if ((rc = ensure_ts(ctx, o)) || (rc = il_alloc<T>(ctx)) || (rc = il_convert(ctx, o, true))) return rc;
+      ctx->il_fresh = 1;
+    }
+    ctx->std_fresh = 0;
+  } else {
+    if ((rc = il_out(ctx, o)) || (rc = ensure_ts(ctx, o))) return rc;
+    ctx->il_fresh = 0;
+  }
   while (done < n_steps) {
     if (ctx->stop_flag) {
       ctx->stop_flag = 0;
@@ -2923,8 +3235,8 @@ extern "C" int odis_sync(odis_ctx *ctx) {
 
 template <typename T>
 static int diss_t(odis_ctx *ctx, Ops<T> &o) {
-  const int rc = ensure_ts(ctx, o);
-  if (rc) return rc;
+  int rc = il_out(ctx, o);
+  if (rc || (rc = ensure_ts(ctx, o))) return rc;
   FaceArgs<T> fa = face_args(ctx, o);
   const dim3 gf(ctx->nb_face, ctx->B), blk(kBlock);
   if (ctx->compact)
@@ -2969,6 +3281,8 @@ extern "C" int odis_get_dissipation_series(odis_ctx *ctx, int inst, long long fi
 template <typename T>
 static int vel_t(odis_ctx *ctx, Ops<T> &o, int b, const double *normal, double *uv) {
   const int E = ctx->E;
+  const int rc = il_out(ctx, o);
+  if (rc) return rc;
   std::vector<double> nd(size_t(2) * E);
   for (int r = 0; r < E; r++) {
     nd[2 * r] = normal[2 * ctx->face_inv[r]];
@@ -3062,6 +3376,13 @@ extern "C" int odis_set_option(odis_ctx *ctx, int option, int value) {
     ctx->sh_cpt = value;
     return ODIS_OK;
   }
+  if (option == ODIS_OPT_INTERLEAVE) {
+    if (value < 0 || value > 2) return fail(ctx, ODIS_E_INVALID, "interleave: 0 (off), 1 (on) or 2 (automatic)");
+    CK(hipStreamSynchronize(ctx->stream));
+    drop_graphs(ctx);
+    ctx->il_opt = value;
+    return ODIS_OK;
+  }
   if (option == ODIS_OPT_STEP_LANES) {
     if (value != 2 && value != 4) return fail(ctx, ODIS_E_INVALID, "step lanes must be 2 or 4");
     CK(hipStreamSynchronize(ctx->stream));
@@ -3099,6 +3420,8 @@ extern "C" int odis_get_option(odis_ctx *ctx, int option, int *value) {
   else if (option == ODIS_OPT_STEP_BLOCKS) *value = step_blocks(ctx);
   else if (option == ODIS_OPT_STEP_LANES) *value = ctx->step_lanes;
   else if (option == ODIS_OPT_SH_CELLS) *value = ctx->sh_cpt;
+  else if (option == ODIS_OPT_INTERLEAVE) *value = ctx->il_opt;
+  else if (option == ODIS_OPT_INTERLEAVE_ACTIVE) *value = il_wanted(ctx) ? 1 : 0;
   else return fail(ctx, ODIS_E_INVALID, "unknown option");
   return ODIS_OK;
 }
@@ -3209,6 +3532,8 @@ extern "C" int odis_velocity_xyz(odis_ctx *ctx, int inst, double *xyz) {
   if (!ctx || !xyz) return ODIS_E_INVALID;
   if (inst < 0 || inst >= ctx->B) return fail(ctx, ODIS_E_RANGE, "instance out of range");
   if (!ctx->rbf_idx) return fail(ctx, ODIS_E_INVALID, "no RBF operator set");
+  const int rc = il_out_any(ctx);
+  if (rc) return rc;
   const int N = ctx->N;
   const dim3 grid((N + kBlock - 1) / kBlock), blk(kBlock);
   if (ctx->precision == ODIS_FP64)
@@ -3232,6 +3557,8 @@ extern "C" int odis_interp_ll(odis_ctx *ctx, int inst, double *ll) {
   if (!ctx || !ll) return ODIS_E_INVALID;
   if (inst < 0 || inst >= ctx->B) return fail(ctx, ODIS_E_RANGE, "instance out of range");
   if (!ctx->ll_rows) return fail(ctx, ODIS_E_INVALID, "no interpolation matrix set");
+  const int rc = il_out_any(ctx);
+  if (rc) return rc;
   const dim3 grid((ctx->ll_rows + kBlock / 64 - 1) / (kBlock / 64)), blk(kBlock);
   if (ctx->precision == ODIS_FP64)
     hipLaunchKernelGGL((k_interp_ll<double>), grid, blk, 0, ctx->stream, ctx->ll_rows, ctx->ll_rp, ctx->ll_col,

@@ -254,7 +254,19 @@ int odis_request_stop(odis_ctx *ctx);
  *                    compiled kernels, the fewest even count >= 4 whose grid
  *                    fits the workgroups the device holds at once; 8 for the
  *                    others), 4, 8 or 16; the coefficients depend on it by
- *                    rounding only. */
+ *                    rounding only.
+ *  ODIS_OPT_INTERLEAVE a batch steps with its state held interleaved
+ *                    (instances innermost), one workgroup per patch and 16
+ *                    instances, so that the operators are read once per 16
+ *                    instances and each state access is 16 consecutive
+ *                    values: 0 off, 1 on, 2 (default) on for 16 instances
+ *                    or more on grids of 20000 cells or more.  It applies to the one-kernel step with 64-cell
+ *                    patches and no SH basis; the instance-major arrays are
+ *                    brought up to date when the state is read or set.  The
+ *                    state is the same bit for bit; dissipation sums differ
+ *                    by rounding (another grouping of the per-face terms).
+ *  ODIS_OPT_INTERLEAVE_ACTIVE (read only) 1 when the next step would run
+ *                    interleaved. */
 #define ODIS_OPT_GRAPHS     1
 #define ODIS_OPT_COMPACT    2
 #define ODIS_OPT_ONE_KERNEL 3
@@ -262,6 +274,8 @@ int odis_request_stop(odis_ctx *ctx);
 #define ODIS_OPT_STEP_BLOCKS 5
 #define ODIS_OPT_STEP_LANES  6
 #define ODIS_OPT_SH_CELLS    7
+#define ODIS_OPT_INTERLEAVE  8
+#define ODIS_OPT_INTERLEAVE_ACTIVE 9
 int odis_set_option(odis_ctx *ctx, int option, int value);
 int odis_get_option(odis_ctx *ctx, int option, int *value);
 /* Timing for benchmarks: with stride T > 0, the kernels of every step n of

@@ -629,6 +629,54 @@ def test_one_kernel_step_matches_face_cell_pair(product, precision):
             _assert_same_state(one, pair, (0, 1), (level, "patch", cells, lanes, blocks))
 
 
+@pytest.mark.parametrize("precision,n", [(0, 20), (1, 33)])
+def test_interleaved_batch_step(product, precision, n):
+    """A batch stepping with its state held interleaved (16 instances per
+    workgroup, the last group padded) leaves the same state bit for bit as
+    the instance-major step, across state reads between calls, a set_state
+    of one instance in mid-run, and graph replays; dissipation and its
+    series agree to 1e-12 (another grouping of the face terms).  A context
+    with an SH basis does not interleave."""
+    G = product
+    g, _ = generated_grid(5)
+    rng = np.random.default_rng(7)
+    ps = [G.make_params(time_step=30.0, h=float(10 ** rng.uniform(3.5, 4.6)), alpha=float(10 ** rng.uniform(-10, -7)),
+                        tide=G.FULL, obliquity=float(np.radians(rng.uniform(0, 1))))[0] for _ in range(n)]
+    mesh = G.Mesh(g, ps[0].radius)
+    il = G.Context(mesh, ps, precision=precision)
+    im = G.Context(mesh, ps, precision=precision)
+    il.set_option(G.OPT_INTERLEAVE, 1)
+    im.set_option(G.OPT_INTERLEAVE, 0)
+    assert il.option(G.OPT_INTERLEAVE_ACTIVE) == 1 and im.option(G.OPT_INTERLEAVE_ACTIVE) == 0
+    insts = sorted({0, 1, n // 2, n - 1})
+    for steps in (1, 2, 97):
+        il.step(steps)
+        im.step(steps)
+        _assert_same_state(il, im, insts, ("il", steps))
+    saved = im.get_state(3)
+    for steps in (50,):
+        il.step(steps)
+        im.step(steps)
+    for c in (il, im):
+        c.set_state(3, saved["v"], saved["dvdt"], saved["eta"], saved["detadt"], c.get_state(0)["iter"])
+    il.use_graphs(True)
+    for steps in (64, 5, 64):
+        il.step(steps)
+        im.step(steps)
+        _assert_same_state(il, im, insts + [3], ("il graphs", steps))
+    da, db = il.dissipation(), im.dissipation()
+    assert np.max(np.abs(da - db) / np.abs(db)) < 1e-12
+    first = im.get_state(0)["iter"] - 100
+    for i in (0, n - 1):
+        sa, sb = il.dissipation_series(i, first, 100), im.dissipation_series(i, first, 100)
+        assert np.max(np.abs(sa - sb) / np.abs(sb)) < 1e-12
+    sh = G.Context(mesh, ps, precision=precision)
+    sh.set_sh(4, None, mode=G.SH_RECUR)
+    assert sh.option(G.OPT_INTERLEAVE_ACTIVE) == 0
+    for c in (il, im, sh):
+        c.close()
+
+
 @pytest.mark.parametrize("precision,L", [(0, 16), (1, 16), (0, 12), (1, 12), (0, 8)])
 def test_sh_recurrence_matches_stored_basis(product, precision, L):
     """The basis evaluated on the fly and the stored basis give the same
