@@ -328,7 +328,7 @@ def _shell_batch(G, mesh, n, L):
     return ps, np.array(betas)
 
 
-@pytest.mark.parametrize("level,L,n", [(5, 16, 20), (4, 8, 70)])
+@pytest.mark.parametrize("level,L,n", [(5, 16, 20), (4, 8, 70), (3, 6, 150)])
 def test_sh_mfma_batch(product, level, L, n):
     """A batch of shell instances (FP32 stencils) coupled through the
     split-BF16 matrix products stays within 3e-4 (relative, after 150 steps
@@ -336,7 +336,8 @@ def test_sh_mfma_batch(product, level, L, n):
     the FP32 form with the FP32 recurrence; an instance's trajectory is the
     same bit for bit alone and inside the batch (its products do not depend
     on the other instances); graph replays equal direct launches.  20 and 70
-    instances run the 2- and 8-tile kernels, one instance the 1-tile ones."""
+    instances run the 2- and 5-tile kernels, 150 the 8-tile ones over two
+    instance groups, one instance the 1-tile ones."""
     G = product
     g, _ = generated_grid(level)
     mesh = G.Mesh(g, 232.1e3)
