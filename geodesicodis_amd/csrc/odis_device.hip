@@ -48,12 +48,16 @@ __device__ __forceinline__ double sum_partials(const double *p, int count, doubl
   return block_sum<NW>(acc, sh);
 }
 
-// Index of element i of instance b in a per-instance array: instance-major
-// (b stride + i) or, for a batch held interleaved (IL), element-major
-// (i stride + b, stride = the padded instance count).
+// Index of element i of instance b in a per-instance array of `stride`
+// elements per instance: instance-major (b stride + i) or, for a batch held
+// interleaved (IL), in groups of kIlGroup instances, each group
+// element-major with its instances innermost
+// (((b / kIlGroup) stride + i) kIlGroup + b % kIlGroup), so that a group's
+// values of consecutive elements are contiguous.
+constexpr int kIlGroup = 16;
 template <bool IL>
-__device__ __forceinline__ long long sidx(int b, long long stride, long long i) {
-  return IL ? i * stride + b : b * stride + i;
+__device__ __host__ __forceinline__ long long sidx(int b, long long stride, long long i) {
+  return IL ? ((long long)(b / kIlGroup) * stride + i) * kIlGroup + b % kIlGroup : b * stride + i;
 }
 
 // Result of the momentum update of one face.
@@ -421,8 +425,8 @@ __global__ __launch_bounds__(LPC * P) void k_step(StepArgs<T> a, int energy_quad
 }
 
 // One-kernel step of a batch held interleaved: every per-instance array is
-// element-major with the instances innermost ([element][Bp], Bp = the
-// instance count padded to IB).  Workgroup (patch j, group of IB
+// held in groups of IB instances, each group element-major with its
+// instances innermost (sidx<true>).  Workgroup (patch j, group of IB
 // instances); lane l serves instance l % IB of the group.  The patch's faces
 // go NT / IB at a time, so each face's operators are loaded once for IB
 // instances and its state is IB consecutive values; then its (cell,
@@ -535,21 +539,21 @@ __global__ __launch_bounds__(256) void k_step_il(StepArgs<T> a, int energy_quad,
   }
 }
 
-// Per-instance array of n elements between the instance-major layout
-// (std, [B][stride]) and the interleaved one (il, [n][Bp]): TO_IL copies std
-// into il, else il into std.  Padding instances are not touched.
-// Consecutive threads take consecutive instances of one element.
+// Per-instance array of n elements between the instance-major layout (std,
+// [B][n]) and the interleaved one (il, sidx<true>): TO_IL copies std into
+// il, else il into std.  Padding instances are not touched.  Thread t takes
+// instance t / n, element t % n, so the instance-major side is contiguous.
 template <typename T, bool TO_IL>
 __global__ __launch_bounds__(kBlock) void k_interleave(T *__restrict__ stdp, T *__restrict__ ilp, long long n,
-                                                       long long stride, int B, int Bp) {
+                                                       int B) {
   const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
-  const long long i = t / B;
-  const int b = int(t % B);
-  if (i >= n) return;
+  const int b = int(t / n);
+  const long long i = t % n;
+  if (b >= B) return;
   if (TO_IL)
-    ilp[i * Bp + b] = stdp[b * stride + i];
+    ilp[sidx<true>(b, n, i)] = stdp[b * n + i];
   else
-    stdp[b * stride + i] = ilp[i * Bp + b];
+    stdp[b * n + i] = ilp[sidx<true>(b, n, i)];
 }
 
 // Dissipation of the last step of a chunk of k_step launches.
@@ -2824,13 +2828,13 @@ static StepArgs<T> step_args(odis_ctx *ctx, Ops<T> &o) {
 // reads or writes them (state in / out, dissipation, outputs, the other
 // step kernels).  std_fresh / il_fresh say which copy holds the current
 // state (at least one always does).
-constexpr int kInterleaveGroup = 16;  // instances per workgroup of k_step_il
-// faces and (cell, instance) pairs per lane in flight in k_step_il: FP64
-// gains from 4 and 2 (0.287 against 0.303 ms for 64 instances at 40962
-// cells), FP32 loses occupancy by them (0.72 against 0.63 ms for 256)
+constexpr int kInterleaveGroup = kIlGroup;  // instances per workgroup of k_step_il (one group of the layout)
+// faces and (cell, instance) pairs per lane in flight in k_step_il, the
+// fastest measured: FP64 4 and 2, FP32 2 and 1 (256 instances at 40962
+// cells: 0.548 ms per step against 0.556 with 1 and 1, 0.62 with 4 and 2)
 template <typename T>
 struct IlUnroll {
-  static constexpr int faces = sizeof(T) == 8 ? 4 : 1, cells = sizeof(T) == 8 ? 2 : 1;
+  static constexpr int faces = sizeof(T) == 8 ? 4 : 2, cells = sizeof(T) == 8 ? 2 : 1;
 };
 constexpr int kInterleaveMin = 16;    // batch size from which the automatic choice interleaves
 constexpr int kInterleaveCells = 20000;  // ... on grids of at least this many cells (smaller: slower)
@@ -2879,7 +2883,7 @@ static int il_alloc(odis_ctx *ctx) {
 template <typename T>
 static int il_convert(odis_ctx *ctx, Ops<T> &o, bool to_il) {
   const IlOps<T> q = il_ops<T>(ctx);
-  const int c = ctx->cur, B = ctx->B, Bp = ctx->il_Bp;
+  const int c = ctx->cur, B = ctx->B;
   const long long E = ctx->E, N = ctx->N;
   // (instance-major array [B][n], interleaved array [n][Bp], n)
   struct {
@@ -2892,10 +2896,9 @@ static int il_convert(odis_ctx *ctx, Ops<T> &o, bool to_il) {
     const long long threads = x.n * B;
     const dim3 g(unsigned((threads + kBlock - 1) / kBlock));
     if (to_il)
-      hipLaunchKernelGGL((k_interleave<T, true>), g, dim3(kBlock), 0, ctx->stream, x.stdp, x.ilp, x.n, x.n, B, Bp);
+      hipLaunchKernelGGL((k_interleave<T, true>), g, dim3(kBlock), 0, ctx->stream, x.stdp, x.ilp, x.n, B);
     else
-      hipLaunchKernelGGL((k_interleave<T, false>), g, dim3(kBlock), 0, ctx->stream, x.stdp, x.ilp, x.n, x.n, B,
-                         Bp);
+      hipLaunchKernelGGL((k_interleave<T, false>), g, dim3(kBlock), 0, ctx->stream, x.stdp, x.ilp, x.n, B);
   }
   CK(hipGetLastError());
   return ODIS_OK;
@@ -2921,19 +2924,17 @@ static StepArgs<T> step_args_il(odis_ctx *ctx, Ops<T> &o) {
   StepArgs<T> a = step_args(ctx, o);
   const IlOps<T> q = il_ops<T>(ctx);
   const int c = ctx->cur;
-  const long long Bp = ctx->il_Bp;
+  // the strides stay the elements per instance (sidx<true> groups the instances)
   a.f.v_cur = q.v[c];
   a.f.v_nxt = q.v[c ^ 1];
   a.f.ring = q.ring_v;
   a.f.eta = q.eta[c];
   a.f.ts = q.ts[c];
-  a.f.v_stride = a.f.ring_stride = a.f.eta_stride = a.f.ts_stride = Bp;
   a.c.v_new = q.v[c ^ 1];
   a.c.eta = q.eta[c];
   a.c.eta_nxt = q.eta[c ^ 1];
   a.c.ring = q.ring_e;
   a.c.ts = q.ts[c ^ 1];
-  a.c.v_stride = a.c.eta_stride = a.c.ring_stride = a.c.ts_stride = Bp;
   return a;
 }
 
