@@ -1454,7 +1454,8 @@ __device__ __forceinline__ void split8(const float (&f)[8], bf16x8 &hi, bf16x8 &
 // (zero past nb and N).  One wave per fragment (it, q): lane l reads its 8
 // cells of one instance (16 rows of 128 contiguous bytes per wave) and the
 // wave writes the fragment as 2 KB in one piece.
-template <typename T>
+// IL: x held interleaved (sidx<true>).
+template <typename T, bool IL>
 __global__ __launch_bounds__(kBlock) void k_sh_pack_mfma(int N, int NB32, int n_frag, int nb,
                                                          const T *__restrict__ x, long long x_stride,
                                                          uint4 *__restrict__ Xa) {
@@ -1462,10 +1463,10 @@ __global__ __launch_bounds__(kBlock) void k_sh_pack_mfma(int N, int NB32, int n_
   if (f >= n_frag) return;
   const int it = f / NB32, q = f % NB32;
   const int j = 16 * it + (lane & 15), c0 = 32 * q + 8 * (lane >> 4);
-  const T *xr = x + (long long)min(j, nb - 1) * x_stride;
+  const int jc = min(j, nb - 1);
   float v[8];
 #pragma unroll
-  for (int i = 0; i < 8; ++i) v[i] = (j < nb && c0 + i < N) ? float(xr[c0 + i]) : 0.f;
+  for (int i = 0; i < 8; ++i) v[i] = (j < nb && c0 + i < N) ? float(x[sidx<IL>(jc, x_stride, c0 + i)]) : 0.f;
   bf16x8 hi, lo;
   split8(v, hi, lo);
   uint4 *dst = Xa + (size_t(f) * 64 + lane) * 2;
@@ -1565,8 +1566,8 @@ __global__ __launch_bounds__(256) void k_sh_ana_mfma(int K, int KT, int NB32, in
 // operand, that of Ys the B operand: both hold 8 harmonics of one row or
 // column), the operands of harmonic block kb + 1 loaded while block kb is
 // multiplied.  Tiles past the last repeat the last one's operands and are
-// not stored (no branch in the loop).
-template <typename T, int NIT, int CT>
+// not stored (no branch in the loop).  IL: base and out held interleaved.
+template <typename T, int NIT, int CT, bool IL>
 __global__ __launch_bounds__(256) void k_sh_syn_mfma(int N, int NT16, int KB, const uint4 *__restrict__ Ys,
                                                      const uint4 *__restrict__ Cs, int nb, const T *base,
                                                      long long stride, T *out) {
@@ -1641,7 +1642,7 @@ __global__ __launch_bounds__(256) void k_sh_syn_mfma(int N, int NT16, int KB, co
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int c = min(16 * (t0 + u) + r16, N - 1), j = min(j0 + it * 16 + 4 * g + r, nb - 1);
-        add[u][it][r] = base ? base[j * stride + c] : T(0);
+        add[u][it][r] = base ? base[sidx<IL>(j, stride, c)] : T(0);
       }
 #pragma unroll
   for (int u = 0; u < CT; ++u) {
@@ -1651,7 +1652,7 @@ __global__ __launch_bounds__(256) void k_sh_syn_mfma(int N, int NT16, int KB, co
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int j = j0 + it * 16 + 4 * g + r;
-        if (t < NT16 && c < N && j < nb) out[j * stride + c] = T(double(add[u][it][r]) + double(acc[u][it][r]));
+        if (t < NT16 && c < N && j < nb) out[sidx<IL>(j, stride, c)] = T(double(add[u][it][r]) + double(acc[u][it][r]));
       }
   }
 }
@@ -2620,7 +2621,7 @@ static void launch_analysis_pair(odis_ctx *ctx, int nb, const T *x, long long st
 // registers.  The analysis ranges depend on N only, and every kernel sums
 // an instance's products in the same order whatever NIT, so an instance's
 // results do not depend on the rest of its batch.
-template <typename T, int NIT>
+template <typename T, int NIT, bool IL>
 static void launch_mfma_t(odis_ctx *ctx, bool ana, int nb, const T *x, const T *base, long long stride, T *out,
                           hipEvent_t e0, hipEvent_t e1) {
   constexpr int W = NIT <= 3 ? 4 : (NIT == 4 ? 3 : (NIT <= 6 ? 2 : 1));  // HPW of the analysis
@@ -2629,32 +2630,42 @@ static void launch_mfma_t(odis_ctx *ctx, bool ana, int nb, const T *x, const T *
   if (ana) {
     const int KT = ctx->sh_Kp / 16;
     const int n_frag = n_it * NB32;
-    hipLaunchKernelGGL((k_sh_pack_mfma<T>), dim3((n_frag + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), 0,
+    hipLaunchKernelGGL((k_sh_pack_mfma<T, IL>), dim3((n_frag + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), 0,
                        ctx->stream, N, NB32, n_frag, nb, x, stride, ctx->sh_Xa);
     const dim3 grid(ctx->sh_rng, (KT + 4 * W - 1) / (4 * W), (n_it + NIT - 1) / NIT);
     hipExtLaunchKernelGGL((k_sh_ana_mfma<NIT, W>), grid, dim3(256), 0, ctx->stream, e0, e1, 0, ctx->sh_K, KT, NB32,
                           ctx->sh_qpr, ctx->sh_rng, ctx->sh_Ya, ctx->sh_Xa, nb, ctx->sh_b);
   } else {
     const dim3 grid((NT16 + 4 * CT - 1) / (4 * CT), (n_it + NIT - 1) / NIT);
-    hipExtLaunchKernelGGL((k_sh_syn_mfma<T, NIT, CT>), grid, dim3(256), 0, ctx->stream, e0, e1, 0, N, NT16,
+    hipExtLaunchKernelGGL((k_sh_syn_mfma<T, NIT, CT, IL>), grid, dim3(256), 0, ctx->stream, e0, e1, 0, N, NT16,
                           ctx->sh_Kp / 32, ctx->sh_Ys, reinterpret_cast<const uint4 *>(ctx->sh_Cs), nb, base,
                           stride, out);
   }
 }
 
+template <typename T, bool IL>
+static void launch_mfma_l(odis_ctx *ctx, bool ana, int nb, const T *x, const T *base, long long stride, T *out,
+                          hipEvent_t e0, hipEvent_t e1) {
+  switch (std::min(8, (nb + 15) / 16)) {
+    case 1: launch_mfma_t<T, 1, IL>(ctx, ana, nb, x, base, stride, out, e0, e1); break;
+    case 2: launch_mfma_t<T, 2, IL>(ctx, ana, nb, x, base, stride, out, e0, e1); break;
+    case 3: launch_mfma_t<T, 3, IL>(ctx, ana, nb, x, base, stride, out, e0, e1); break;
+    case 4: launch_mfma_t<T, 4, IL>(ctx, ana, nb, x, base, stride, out, e0, e1); break;
+    case 5: launch_mfma_t<T, 5, IL>(ctx, ana, nb, x, base, stride, out, e0, e1); break;
+    case 6: launch_mfma_t<T, 6, IL>(ctx, ana, nb, x, base, stride, out, e0, e1); break;
+    case 7: launch_mfma_t<T, 7, IL>(ctx, ana, nb, x, base, stride, out, e0, e1); break;
+    default: launch_mfma_t<T, 8, IL>(ctx, ana, nb, x, base, stride, out, e0, e1); break;
+  }
+}
+
+// il: x, base and out held interleaved (a batch stepping interleaved)
 template <typename T>
 static void launch_mfma(odis_ctx *ctx, bool ana, int nb, const T *x, const T *base, long long stride, T *out,
-                        hipEvent_t e0, hipEvent_t e1) {
-  switch (std::min(8, (nb + 15) / 16)) {
-    case 1: launch_mfma_t<T, 1>(ctx, ana, nb, x, base, stride, out, e0, e1); break;
-    case 2: launch_mfma_t<T, 2>(ctx, ana, nb, x, base, stride, out, e0, e1); break;
-    case 3: launch_mfma_t<T, 3>(ctx, ana, nb, x, base, stride, out, e0, e1); break;
-    case 4: launch_mfma_t<T, 4>(ctx, ana, nb, x, base, stride, out, e0, e1); break;
-    case 5: launch_mfma_t<T, 5>(ctx, ana, nb, x, base, stride, out, e0, e1); break;
-    case 6: launch_mfma_t<T, 6>(ctx, ana, nb, x, base, stride, out, e0, e1); break;
-    case 7: launch_mfma_t<T, 7>(ctx, ana, nb, x, base, stride, out, e0, e1); break;
-    default: launch_mfma_t<T, 8>(ctx, ana, nb, x, base, stride, out, e0, e1); break;
-  }
+                        hipEvent_t e0, hipEvent_t e1, bool il) {
+  if (il)
+    launch_mfma_l<T, true>(ctx, ana, nb, x, base, stride, out, e0, e1);
+  else
+    launch_mfma_l<T, false>(ctx, ana, nb, x, base, stride, out, e0, e1);
 }
 
 // out = base + Yt^T D Minv Yt x for nb instances (x, base, out with instance
@@ -2662,9 +2673,10 @@ static void launch_mfma(odis_ctx *ctx, bool ana, int nb, const T *x, const T *ba
 // alone).  ana / syn: run the analysis + coefficients / the synthesis.
 // ev (nullable): 6 dispatch-stamped events (analysis, coefficients,
 // synthesis).
+// il (batched products only): x, base and out held interleaved.
 template <typename T>
 static int sh_apply(odis_ctx *ctx, const T *x, long long stride, int nb, const double *D, const T *base, T *out,
-                    double *c, bool ana, bool syn, hipEvent_t *ev) {
+                    double *c, bool ana, bool syn, hipEvent_t *ev, bool il = false) {
   const int K = ctx->sh_K, N = ctx->N, L = ctx->sh_L;
   const dim3 blk(kBlock);
   const bool split = ctx->sh_mode == ODIS_SH_SPLIT_BF16;
@@ -2684,7 +2696,8 @@ static int sh_apply(odis_ctx *ctx, const T *x, long long stride, int nb, const d
                       : (ctx->sh_mode == ODIS_SH_RECUR ? (n_chunks + kShWaves - 1) / kShWaves : kShChunks));
   if (n_part > ctx->sh_part) return fail(ctx, ODIS_E_INVALID, "analysis partials exceed their buffer");
   if (mfma) {
-    if (ana) launch_mfma<T>(ctx, true, nb, x, nullptr, stride, nullptr, ev ? ev[0] : nullptr, ev ? ev[1] : nullptr);
+    if (ana)
+      launch_mfma<T>(ctx, true, nb, x, nullptr, stride, nullptr, ev ? ev[0] : nullptr, ev ? ev[1] : nullptr, il);
   } else if (by_pair) {
     if (ana) {
       hipEvent_t e0 = ev ? ev[0] : nullptr, e1 = ev ? ev[1] : nullptr;
@@ -2728,7 +2741,7 @@ static int sh_apply(odis_ctx *ctx, const T *x, long long stride, int nb, const d
     if (!ana)  // coefficients from the caller: their split copy first
       hipLaunchKernelGGL(k_sh_to_split, dim3((K + kBlock - 1) / kBlock, nb), blk, 0, ctx->stream, K,
                          ctx->sh_Kp / 32, c, ctx->sh_Cs);
-    launch_mfma<T>(ctx, false, nb, nullptr, base, stride, out, ev ? ev[4] : nullptr, ev ? ev[5] : nullptr);
+    launch_mfma<T>(ctx, false, nb, nullptr, base, stride, out, ev ? ev[4] : nullptr, ev ? ev[5] : nullptr, il);
   } else if (syn && ctx->sh_mode == ODIS_SH_RECUR) {
     if (!ana)  // coefficients from the caller: their m-major copy first
       hipLaunchKernelGGL(k_sh_to_mmajor, dim3((K + kBlock - 1) / kBlock, nb), blk, 0, ctx->stream, K, c, chain,
@@ -2842,8 +2855,8 @@ constexpr int kInterleaveCells = 20000;  // ... on grids of at least this many c
 static bool il_wanted(const odis_ctx *ctx) {
   const bool on =
       ctx->il_opt == 1 || (ctx->il_opt == 2 && ctx->B >= kInterleaveMin && ctx->N >= kInterleaveCells);
-  return on && ctx->compact && ctx->one_kernel && ctx->patch == 64 && ctx->sh_mode == ODIS_SH_OFF &&
-         step_blocks(ctx) == ctx->n_patch;
+  return on && ctx->compact && ctx->one_kernel && ctx->patch == 64 &&
+         (ctx->sh_mode == ODIS_SH_OFF || ctx->sh_mode == ODIS_SH_MFMA) && step_blocks(ctx) == ctx->n_patch;
 }
 
 template <typename T>
@@ -2928,7 +2941,7 @@ static StepArgs<T> step_args_il(odis_ctx *ctx, Ops<T> &o) {
   a.f.v_cur = q.v[c];
   a.f.v_nxt = q.v[c ^ 1];
   a.f.ring = q.ring_v;
-  a.f.eta = q.eta[c];
+  a.f.eta = ctx->sh_mode != ODIS_SH_OFF ? static_cast<const T *>(ctx->eta_eff) : q.eta[c];  // eta_eff interleaved
   a.f.ts = q.ts[c];
   a.c.v_new = q.v[c ^ 1];
   a.c.eta = q.eta[c];
@@ -3013,7 +3026,14 @@ static int launch_chunk(odis_ctx *ctx, Ops<T> &o, const double *table, int count
         evsh = &ctx->ev[*ev_used];
         *ev_used += 6;
       }
-      const int rc = sh_effective_height(ctx, o, evsh);
+      int rc;
+      if (il) {  // the batched products on the interleaved eta
+        const T *eta = il_ops<T>(ctx).eta[ctx->cur];
+        rc = sh_apply<T>(ctx, eta, ctx->N, ctx->B, ctx->sh_D, eta, static_cast<T *>(ctx->eta_eff), ctx->sh_c, true,
+                         true, evsh, true);
+      } else {
+        rc = sh_effective_height(ctx, o, evsh);
+      }
       if (rc) return rc;
     }
     if (one && il) {
@@ -3832,7 +3852,10 @@ extern "C" int odis_sh_set_basis(odis_ctx *ctx, int l_max, const double *beta, i
     return rc;
   if ((rc = alloc(reinterpret_cast<void **>(&ctx->sh_c), size_t(ctx->B) * K * sizeof(double)))) return rc;
   const size_t tsz = ctx->precision == ODIS_FP64 ? sizeof(double) : sizeof(float);
-  if ((rc = alloc(&ctx->eta_eff, size_t(ctx->B) * N * tsz))) return rc;
+  // eta_eff of every instance, padded to whole groups of kIlGroup for a batch held interleaved
+  const size_t eff_bytes = size_t((ctx->B + kIlGroup - 1) / kIlGroup * kIlGroup) * N * tsz;
+  if ((rc = alloc(&ctx->eta_eff, eff_bytes))) return rc;
+  CK(hipMemset(ctx->eta_eff, 0, eff_bytes));  // padding instances read zeros
   ctx->sh_L = l_max;
   ctx->sh_K = K;
   ctx->sh_mode = mode;

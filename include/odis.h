@@ -256,15 +256,17 @@ int odis_request_stop(odis_ctx *ctx);
  *                    others), 4, 8 or 16; the coefficients depend on it by
  *                    rounding only.
  *  ODIS_OPT_INTERLEAVE a batch steps with its state held interleaved
- *                    (instances innermost), one workgroup per patch and 16
- *                    instances, so that the operators are read once per 16
- *                    instances and each state access is 16 consecutive
- *                    values: 0 off, 1 on, 2 (default) on for 16 instances
- *                    or more on grids of 20000 cells or more.  It applies to the one-kernel step with 64-cell
- *                    patches and no SH basis; the instance-major arrays are
- *                    brought up to date when the state is read or set.  The
- *                    state is the same bit for bit; dissipation sums differ
- *                    by rounding (another grouping of the per-face terms).
+ *                    (groups of 16 instances, instances innermost), one
+ *                    workgroup per patch and 16 instances, so that the
+ *                    operators are read once per 16 instances and each state
+ *                    access is 16 consecutive values: 0 off, 1 on, 2
+ *                    (default) on for 16 instances or more on grids of 20000
+ *                    cells or more.  It applies to the one-kernel step with
+ *                    64-cell patches, without an SH basis or with
+ *                    ODIS_SH_MFMA; the instance-major arrays are brought up
+ *                    to date when the state is read or set.  The state is the
+ *                    same bit for bit; dissipation sums differ by rounding
+ *                    (another grouping of the per-face terms).
  *  ODIS_OPT_INTERLEAVE_ACTIVE (read only) 1 when the next step would run
  *                    interleaved. */
 #define ODIS_OPT_GRAPHS     1

@@ -630,22 +630,32 @@ def test_one_kernel_step_matches_face_cell_pair(product, precision):
             _assert_same_state(one, pair, (0, 1), (level, "patch", cells, lanes, blocks))
 
 
-@pytest.mark.parametrize("precision,n", [(0, 20), (1, 33)])
-def test_interleaved_batch_step(product, precision, n):
+@pytest.mark.parametrize("precision,n,shell", [(0, 20, False), (1, 33, False), (1, 20, True)])
+def test_interleaved_batch_step(product, precision, n, shell):
     """A batch stepping with its state held interleaved (16 instances per
     workgroup, the last group padded) leaves the same state bit for bit as
     the instance-major step, across state reads between calls, a set_state
     of one instance in mid-run, and graph replays; dissipation and its
-    series agree to 1e-12 (another grouping of the face terms).  A context
-    with an SH basis does not interleave."""
+    series agree to 1e-12 (another grouping of the face terms).  With shells
+    coupled through the batched split-BF16 products the products read and
+    write the interleaved eta; a context with another SH mode does not
+    interleave."""
     G = product
     g, _ = generated_grid(5)
     rng = np.random.default_rng(7)
-    ps = [G.make_params(time_step=30.0, h=float(10 ** rng.uniform(3.5, 4.6)), alpha=float(10 ** rng.uniform(-10, -7)),
-                        tide=G.FULL, obliquity=float(np.radians(rng.uniform(0, 1))))[0] for _ in range(n)]
-    mesh = G.Mesh(g, ps[0].radius)
+    if shell:
+        mesh = G.Mesh(g, 232.1e3)
+        ps, beta = _shell_batch(G, mesh, n, 8)
+    else:
+        ps = [G.make_params(time_step=30.0, h=float(10 ** rng.uniform(3.5, 4.6)),
+                            alpha=float(10 ** rng.uniform(-10, -7)), tide=G.FULL,
+                            obliquity=float(np.radians(rng.uniform(0, 1))))[0] for _ in range(n)]
+        mesh = G.Mesh(g, ps[0].radius)
     il = G.Context(mesh, ps, precision=precision)
     im = G.Context(mesh, ps, precision=precision)
+    if shell:
+        for c in (il, im):
+            c.set_sh(8, beta, mode=G.SH_MFMA)
     il.set_option(G.OPT_INTERLEAVE, 1)
     im.set_option(G.OPT_INTERLEAVE, 0)
     assert il.option(G.OPT_INTERLEAVE_ACTIVE) == 1 and im.option(G.OPT_INTERLEAVE_ACTIVE) == 0
@@ -673,6 +683,7 @@ def test_interleaved_batch_step(product, precision, n):
         assert np.max(np.abs(sa - sb) / np.abs(sb)) < 1e-12
     sh = G.Context(mesh, ps, precision=precision)
     sh.set_sh(4, None, mode=G.SH_RECUR)
+    sh.set_option(G.OPT_INTERLEAVE, 1)
     assert sh.option(G.OPT_INTERLEAVE_ACTIVE) == 0
     for c in (il, im, sh):
         c.close()
