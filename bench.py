@@ -22,14 +22,19 @@ stored FP64 basis instead).  The same line carries:
   - the FP32 form on the same workload (FP32 stencils and recurrence, or a
     stored split-BF16 basis with --sh-basis stored), with its error against
     FP64 after the same steps;
+  - FP32 stencils with the SH passes as split-BF16 products on the matrix
+    cores (one instance), with its error against FP64 and the matrix-core
+    utilisation;
   - the free-surface ocean without the shell (workload "free": dt = P/18300,
     where the stencils alone are timed);
   - a batched FP32 sweep: 256 instances on the 40962-cell grid, seeded
     thickness and drag draws, against one plain FP64 instance;
   - an uneven FP32 batch: 512 instances over the 2562- to 163842-cell
-    grids, FULL tide, quadratic drag, half under a shell with SH coupling.
+    grids, FULL tide, quadratic drag, half under a shell with SH coupling
+    (the batched split-BF16 products for shell groups of 4 or more).
 `--workload converge` instead times the plain-method case (10242 cells) to
-converged orbit-averaged dissipation, plain FP64 and FP32.
+converged orbit-averaged dissipation, plain FP64 and FP32; `--workload
+sweep` and `--workload uneven` run one batched extra alone (for profiling).
 
 Prints one JSON line: cell-steps/s of the whole job; the roofline of the
 dominant kernel (HIP events stamped on the kernel dispatches of every 16th
