@@ -541,15 +541,14 @@ __global__ __launch_bounds__(256) void k_step_il(StepArgs<T> a, int energy_quad,
 
 // Per-instance array of n elements between the instance-major layout (std,
 // [B][n]) and the interleaved one (il, sidx<true>): TO_IL copies std into
-// il, else il into std.  Padding instances are not touched.  Thread t takes
-// instance t / n, element t % n, so the instance-major side is contiguous.
+// il, else il into std.  Grid (elements / kBlock, B): workgroup y is one
+// instance, so the instance-major side is contiguous.  Padding instances
+// are not touched.
 template <typename T, bool TO_IL>
-__global__ __launch_bounds__(kBlock) void k_interleave(T *__restrict__ stdp, T *__restrict__ ilp, long long n,
-                                                       int B) {
-  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
-  const int b = int(t / n);
-  const long long i = t % n;
-  if (b >= B) return;
+__global__ __launch_bounds__(kBlock) void k_interleave(T *__restrict__ stdp, T *__restrict__ ilp, long long n) {
+  const int b = blockIdx.y;
+  const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
   if (TO_IL)
     ilp[sidx<true>(b, n, i)] = stdp[b * n + i];
   else
@@ -2906,12 +2905,11 @@ static int il_convert(odis_ctx *ctx, Ops<T> &o, bool to_il) {
               {o.ring_e, q.ring_e, 3 * N}, {o.ts[c], q.ts[c], 2 * E}};
   for (const auto &x : arr) {
     if (!x.stdp) continue;  // no partial sums without the compact form
-    const long long threads = x.n * B;
-    const dim3 g(unsigned((threads + kBlock - 1) / kBlock));
+    const dim3 g(unsigned((x.n + kBlock - 1) / kBlock), B);
     if (to_il)
-      hipLaunchKernelGGL((k_interleave<T, true>), g, dim3(kBlock), 0, ctx->stream, x.stdp, x.ilp, x.n, B);
+      hipLaunchKernelGGL((k_interleave<T, true>), g, dim3(kBlock), 0, ctx->stream, x.stdp, x.ilp, x.n);
     else
-      hipLaunchKernelGGL((k_interleave<T, false>), g, dim3(kBlock), 0, ctx->stream, x.stdp, x.ilp, x.n, B);
+      hipLaunchKernelGGL((k_interleave<T, false>), g, dim3(kBlock), 0, ctx->stream, x.stdp, x.ilp, x.n);
   }
   CK(hipGetLastError());
   return ODIS_OK;
