@@ -313,8 +313,9 @@ def uneven_run(G, args):
     the quantised step below 0.35 / omega_max, omega_max the larger of the
     grid-scale gravity-wave frequency (0.0193 s^-1 at 10242 cells, scaling
     as sqrt(N)) and, under the shell, the degree-16 shell-mode frequency
-    sqrt(g h beta_l l (l + 1)) / R.  FP32 state; the contexts are stepped in
-    turn on one stream."""
+    sqrt(g h beta_l l (l + 1)) / R.  FP32 state.  Each context is timed
+    alone (the per-group rows), then the 8 contexts' steps are enqueued
+    together, one stream each, which gives `value`."""
     import numpy as np
     rng = np.random.default_rng(UNEVEN_SEED)
     groups = {}
@@ -328,7 +329,7 @@ def uneven_run(G, args):
     beta = np.array(G.surface_beta(G.LID_MEMBR, L, 252.1e3, 0.113, 20e3, 38e3))
     ls = np.arange(L + 1)
     w_shell = float(np.max(np.sqrt(0.1166 * 38e3 * np.maximum(beta, 1.0) * ls * (ls + 1)) / 232.1e3))
-    total_cells, elapsed, n_steps, rows = 0, 0.0, max(10, args.steps // 40), []
+    total_cells, elapsed, n_steps, rows, ctxs = 0, 0.0, max(10, args.steps // 40), [], []
     for (level, lid), inst in sorted(groups.items()):
         grid = generated_grid(G, level + 1)
         n_cells = grid.n
@@ -350,7 +351,7 @@ def uneven_run(G, args):
             ctx.set_sh(L, np.tile(beta, (len(ps), 1)), mode=G.SH_MFMA if sh == "mfma" else G.SH_RECUR)
         el, tm = timed_run(ctx, n_steps, 3, 4 if lid else 0)
         ok = bool(all(abs(x) < float("inf") for x in ctx.dissipation()))
-        ctx.close()
+        ctxs.append(ctx)
         elapsed += el
         total_cells += n_cells * len(ps)
         row = {"cells": n_cells, "surface": "LID_MEMBR" if lid else "FREE", "instances": len(ps),
@@ -361,9 +362,22 @@ def uneven_run(G, args):
             if sh == "mfma":
                 row["mfma"] = mfma_utilisation(tm, n_cells, L, len(ps))
         rows.append(row)
+    # all groups at once: every context's steps enqueued on its own stream
+    for c in ctxs:
+        c.sync()
+    t0 = time.perf_counter()
+    for c in ctxs:
+        c.step(n_steps, sync=False)
+    for c in ctxs:
+        c.sync()
+    conc = time.perf_counter() - t0
+    for c in ctxs:
+        c.close()
     return {"instances": args.uneven_instances, "seed": UNEVEN_SEED, "precision": "fp32 state, fp64 dissipation",
-            "value": total_cells * n_steps / elapsed, "ms_per_step_all_groups": 1e3 * elapsed / n_steps,
-            "steps": n_steps, "groups": rows}
+            "value": total_cells * n_steps / conc, "ms_per_step_all_groups": 1e3 * conc / n_steps,
+            "launch": "the 8 contexts' steps enqueued together, one stream each",
+            "value_groups_in_turn": total_cells * n_steps / elapsed,
+            "ms_per_step_groups_in_turn": 1e3 * elapsed / n_steps, "steps": n_steps, "groups": rows}
 
 
 def converge_run(G, args):
