@@ -2835,11 +2835,11 @@ static StepArgs<T> step_args(odis_ctx *ctx, Ops<T> &o) {
 // ---------------------------------------------------------------------------
 // batches held interleaved (k_step_il)
 // ---------------------------------------------------------------------------
-// A batch may step with its state interleaved ([element][Bp]); the
-// instance-major arrays are then brought up to date only when something
-// reads or writes them (state in / out, dissipation, outputs, the other
-// step kernels).  std_fresh / il_fresh say which copy holds the current
-// state (at least one always does).
+// A batch may step with its state interleaved (groups of kIlGroup
+// instances, sidx<true>); the instance-major arrays are then brought up to
+// date only when something reads or writes them (state in / out,
+// dissipation, outputs, the other step kernels).  std_fresh / il_fresh say
+// which copy holds the current state (at least one always does).
 constexpr int kInterleaveGroup = kIlGroup;  // instances per workgroup of k_step_il (one group of the layout)
 // faces and (cell, instance) pairs per lane in flight in k_step_il, the
 // fastest measured: FP64 4 and 2, FP32 2 and 1 (256 instances at 40962
@@ -2884,9 +2884,11 @@ static int il_alloc(odis_ctx *ctx) {
   if (ctx->il_buf) return ODIS_OK;
   const int Bp = (ctx->B + kInterleaveGroup - 1) / kInterleaveGroup * kInterleaveGroup;
   const size_t bytes = (size_t(9) * ctx->E + size_t(5) * ctx->N) * Bp * sizeof(T);
-  CK(hipMalloc(&ctx->il_buf, bytes));
-  ctx->allocs.push_back(ctx->il_buf);
-  CK(hipMemset(ctx->il_buf, 0, bytes));  // padding instances read zeros
+  void *p = nullptr;
+  CK(hipMalloc(&p, bytes));
+  ctx->allocs.push_back(p);              // freed with the context either way
+  CK(hipMemset(p, 0, bytes));            // padding instances read zeros
+  ctx->il_buf = p;                       // in use only once set up
   ctx->il_Bp = Bp;
   return ODIS_OK;
 }
@@ -2897,7 +2899,7 @@ static int il_convert(odis_ctx *ctx, Ops<T> &o, bool to_il) {
   const IlOps<T> q = il_ops<T>(ctx);
   const int c = ctx->cur, B = ctx->B;
   const long long E = ctx->E, N = ctx->N;
-  // (instance-major array [B][n], interleaved array [n][Bp], n)
+  // (instance-major array [B][n], its interleaved copy, n)
   struct {
     T *stdp, *ilp;
     long long n;
