@@ -358,6 +358,7 @@ class Context:
             raise OdisError(rc, lib().odis_last_error(None).decode())
         self._h = h
         self.precision = precision
+        self._ll_rows = 0
 
     def _check(self, rc):
         if rc:
@@ -407,15 +408,15 @@ class Context:
         rp = np.ascontiguousarray(row_ptr, np.int32)
         c = np.ascontiguousarray(col, np.int32)
         v = np.ascontiguousarray(val, np.float64)
-        self._ll_rows = len(rp) - 1
-        self._check(lib().odis_interp_ll_set(self._h, self._ll_rows,
-                                             int(n_cols if n_cols is not None else 3 * self.mesh.N),
+        rows = len(rp) - 1
+        self._check(lib().odis_interp_ll_set(self._h, rows, int(n_cols if n_cols is not None else 3 * self.mesh.N),
                                              _ip(rp), _ip(c), _dp(v)))
+        self._ll_rows = rows  # the matrix in use: only once the library took it
 
     def interp_ll(self, inst=0):
-        out = np.zeros(self._ll_rows)
+        out = np.zeros(max(1, self._ll_rows))
         self._check(lib().odis_interp_ll(self._h, int(inst), _dp(out)))
-        return out
+        return out[:self._ll_rows]
 
     def set_rbf(self, row_ptr=None, col=None, val=None):
         """Attach the RBF Cartesian-velocity operator (3N x E CSR, reference

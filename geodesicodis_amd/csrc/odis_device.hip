@@ -3491,6 +3491,7 @@ extern "C" int odis_interp_ll_set(odis_ctx *ctx, int n_rows, int n_cols, const i
   CK(hipStreamSynchronize(ctx->stream));
   for (void *p : ctx->ll_allocs) (void)hipFree(p);
   ctx->ll_allocs.clear();
+  ctx->ll_rows = 0;  // no matrix until the new one is in place
   auto up = [&](void **dst, const void *src, size_t bytes) -> int {
     CK(hipMalloc(dst, std::max<size_t>(bytes, 8)));
     ctx->ll_allocs.push_back(*dst);
@@ -3537,15 +3538,23 @@ extern "C" int odis_rbf_set(odis_ctx *ctx, int n_rows, int n_cols, const int *ro
   ctx->rbf_idx = nullptr;
   ctx->rbf_w = ctx->rbf_out = nullptr;
   auto up = [&](void **dst, const void *src, size_t bytes) -> int {
-    CK(hipMalloc(dst, bytes));
-    ctx->rbf_allocs.push_back(*dst);
-    if (src) CK(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    void *p = nullptr;
+    CK(hipMalloc(&p, bytes));
+    ctx->rbf_allocs.push_back(p);
+    if (src) CK(hipMemcpy(p, src, bytes, hipMemcpyHostToDevice));
+    *dst = p;
     return ODIS_OK;
   };
+  // the operator is in use (rbf_idx set) only once all of it is in place
+  int *d_idx = nullptr;
+  double *d_w = nullptr, *d_out = nullptr;
   int rc;
-  if ((rc = up(reinterpret_cast<void **>(&ctx->rbf_idx), idx.data(), idx.size() * sizeof(int)))) return rc;
-  if ((rc = up(reinterpret_cast<void **>(&ctx->rbf_w), w.data(), w.size() * sizeof(double)))) return rc;
-  if ((rc = up(reinterpret_cast<void **>(&ctx->rbf_out), nullptr, size_t(3) * N * sizeof(double)))) return rc;
+  if ((rc = up(reinterpret_cast<void **>(&d_idx), idx.data(), idx.size() * sizeof(int)))) return rc;
+  if ((rc = up(reinterpret_cast<void **>(&d_w), w.data(), w.size() * sizeof(double)))) return rc;
+  if ((rc = up(reinterpret_cast<void **>(&d_out), nullptr, size_t(3) * N * sizeof(double)))) return rc;
+  ctx->rbf_w = d_w;
+  ctx->rbf_out = d_out;
+  ctx->rbf_idx = d_idx;
   return ODIS_OK;
 }
 

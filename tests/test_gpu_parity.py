@@ -720,6 +720,54 @@ def test_sh_recurrence_matches_stored_basis(product, precision, L):
                 assert rel_l2(b[k], a[k]) <= tol, (n, i, k, rel_l2(b[k], a[k]))
 
 
+def test_abi_error_paths(product):
+    """Every rejected call returns its error code without touching the
+    state; the context steps on afterwards exactly as a twin that saw no
+    bad call."""
+    G = product
+    g, arrs = generated_grid(4)
+    p, _ = G.make_params(time_step=60.0)
+    mesh = G.Mesh(g, p.radius)
+    ctx = G.Context(mesh, [p, p])
+    twin = G.Context(mesh, [p, p])
+    ctx.step(3)
+    twin.step(3)
+
+    def code(f, *a):
+        with pytest.raises(G.OdisError) as ei:
+            f(*a)
+        return ei.value.code
+
+    E_INVALID, E_RANGE = -1, -6
+    assert code(ctx.get_state, 2) == E_RANGE
+    assert code(ctx.set_state, -1, np.zeros(mesh.E)) == E_RANGE
+    for opt, val in ((G.OPT_INTERLEAVE, 3), (G.OPT_STEP_LANES, 3), (G.OPT_PATCH_CELLS, 100),
+                     (G.OPT_SH_CELLS, 5), (G.OPT_STEP_BLOCKS, -1), (99, 1)):
+        assert code(ctx.set_option, opt, val) == E_INVALID, opt
+    assert code(ctx.option, 99) == E_INVALID
+    assert code(ctx.set_sh, 6, None, 7) == E_INVALID             # no such mode
+    assert code(ctx.set_sh, 60, None, G.SH_RECUR) == E_INVALID   # (l_max + 1)^2 > cells
+    assert code(ctx.velocity_xyz, 0) == E_INVALID                # no RBF operator set
+    assert code(ctx.interp_ll, 0) == E_INVALID                   # no lat-lon matrix set
+    rp, col, val = _synthetic_ll_matrix(arrs, 10, np.random.default_rng(3))
+    ctx.set_interp_ll(rp, col, val)
+    good = ctx.interp_ll(0)
+    assert good.shape == (len(rp) - 1,)
+    # a rejected matrix (wrong column count) leaves the previous one in use
+    assert code(ctx.set_interp_ll, rp[:51], col[:rp[50]], val[:rp[50]], 3 * mesh.N + 3) == E_INVALID
+    assert np.array_equal(ctx.interp_ll(0), good)
+    it = ctx.get_state(0)["iter"]
+    assert code(ctx.dissipation_series, 0, it, 1) == E_RANGE     # step `it` not taken yet
+    assert code(ctx.dissipation_series, 5, 0, 1) == E_RANGE
+    with pytest.raises(G.OdisError):
+        G.Context(mesh, [])
+    ctx.step(40)
+    twin.step(40)
+    _assert_same_state(ctx, twin, (0, 1), "after rejected calls")
+    for c in (ctx, twin):
+        c.close()
+
+
 @pytest.mark.parametrize("precision", [0, 1])
 def test_velocity_xyz_matches_oracle(product, oracle, precision):
     """Cartesian velocity at cell centres by the RBF operator on the GPU
