@@ -473,6 +473,11 @@ def main():
                     help="precision of the headline value (fp32: FP32 stencils, split-BF16 SH basis)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU work of the baseline sample")
+    ap.add_argument("--cpu-sizes", action="store_true",
+                    help="also time the FP64 C oracle on the plain-method free-surface step at 10242, 40962 and "
+                         "163842 cells, with all threads and with one (cpu_baseline_sizes)")
+    ap.add_argument("--cpu-size-seconds", type=float, default=2.0,
+                    help="approximate CPU work of each --cpu-sizes sample")
     ap.add_argument("--sh-basis", choices=["recur", "stored"], default="recur",
                     help="SH basis evaluated on the fly by the Legendre recurrence (default) or stored "
                          "(FP64, or split-BF16 for fp32)")
@@ -755,7 +760,49 @@ def main():
                        "steps%s" % (", SH projection as dense LSQ analysis and synthesis products" if shell else "",
                                     mesh.N, n, " (projector set-up %.1f s not timed)" % setup if shell else "")),
         }
+    if args.cpu_sizes:
+        out["cpu_baseline_sizes"] = cpu_sizes_run(G, args)
     print(json.dumps(out), flush=True)
+
+
+def cpu_sizes_run(G, args):
+    """The FP64 C oracle on the plain-method free-surface step (ECC, 38 km,
+    alpha 1e-8, dt below 0.35 / omega_max) at 10242, 40962 and 163842 cells,
+    with OMP_NUM_THREADS threads and with one: 2 warm-up steps, then steps
+    until args.cpu_size_seconds of stepping."""
+    import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", min(16, os.cpu_count() or 1)))
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    rows = []
+    for level in (6, 7, 8):
+        grid = generated_grid(G, level)
+        p, _ = G.make_params(time_step=0.35 / (0.0193 * (grid.n / 10242.0) ** 0.5))
+        sim = O.Sim(O.Mesh(grid.arrays(), p.radius),
+                    O.make_params(g=p.g, h=p.h, alpha=p.alpha, omega=p.omega, radius=p.radius,
+                                  love_reduct=p.love_reduct, ecc=p.ecc, dt=p.dt, tide=p.tide))
+        for th in (threads, 1):
+            O.set_threads(th)
+            sim.step(2)
+            n, t = 0, 0.0
+            while t < args.cpu_size_seconds:
+                t0 = time.perf_counter()
+                sim.step(2)
+                t += time.perf_counter() - t0
+                n += 2
+            rows.append({"cells": grid.n, "threads": th, "value": grid.n * n / t, "steps": n})
+            print("bench: cpu %d cells, %d threads: %.3e cell-steps/s" % (grid.n, th, rows[-1]["value"]),
+                  file=sys.stderr, flush=True)
+    O.set_threads(threads)
+    return {"workload": "plain method, free surface, ECC, FP64 C oracle (OpenMP rows)", "unit": "cell-steps/s",
+            "host_cpu": model, "logical_cpus": os.cpu_count(), "rows": rows}
 
 
 if __name__ == "__main__":

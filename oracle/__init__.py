@@ -61,6 +61,8 @@ def lib():
         L.orc_sim_set_state.argtypes = [vp, dp, dp, dp, dp, ctypes.c_int]
         L.orc_sim_get_state.argtypes = [vp, dp, dp, dp, dp, ip]
         L.orc_sim_step.argtypes = [vp, ctypes.c_int, dp]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_get_threads.restype = ctypes.c_int
         L.orc_sim_dissipation.argtypes = [vp]
         L.orc_sim_dissipation.restype = ctypes.c_double
         L.orc_sim_forcing.argtypes = [vp, ctypes.c_double, dp]
@@ -157,6 +159,15 @@ def make_params(**kw):
     for k, v in defaults.items():
         setattr(p, k, v)
     return p
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's row loops (results do not depend on it)."""
+    lib().orc_set_threads(int(n))
+
+
+def threads():
+    return int(lib().orc_get_threads())
 
 
 class Sim:

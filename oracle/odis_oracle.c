@@ -32,6 +32,7 @@
 
 #include <complex.h>
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -1083,3 +1084,9 @@ void orc_membrane_nu_beta(double radius, double hs, double ho, int l_max, double
         nu[l] = gam_tide + dgam;
     }
 }
+
+void orc_set_threads(int n) {
+  if (n >= 1) omp_set_num_threads(n);
+}
+
+int orc_get_threads(void) { return omp_get_max_threads(); }

@@ -83,6 +83,10 @@ void  orc_round_period(double ang_vel, double *period, double *omega);
 void  orc_quantise_dt(double period, double target_dt, double *dt, int *n_steps);
 void  orc_analytical_obliq_west(const orc_params *p, double lat, double lon,
                                 double t, double *out6);
+/* OpenMP threads of the row loops (n >= 1), and the count in use; the
+ * results do not depend on it. */
+void  orc_set_threads(int n);
+int   orc_get_threads(void);
 void  orc_membrane_nu_beta(double radius, double shell_thickness, double ocean_thickness,
                            int l_max, double *nu, double *beta,
                            double *radius_ocean, double *grav_ocean);
