@@ -92,6 +92,40 @@ def test_full_size_step_matches_oracle(product, oracle, precision, tols):
     ctx.close()
 
 
+def test_full_size_shell_step_matches_oracle(product, oracle):
+    """The benchmark's own workload at full size: 163842 cells under the
+    20 km LID_MEMBR shell (parameters through the reference's input.in
+    rules), coupled through the SH projection to degree 32 with the basis
+    evaluated on the fly, FP64.  The oracle applies the same projector as
+    dense products from numpy (scipy's harmonics, least squares).  After 1,
+    3 and 20 steps every state array is within 1e-10."""
+    import bench  # the repository root is on the path (conftest)
+    G, O = product, oracle
+    p, _, beta = bench.shell_setup(G)
+    L = bench.SH_DEGREE
+    g, arrs = generated_grid(8)
+    mesh = G.Mesh(g, p.radius)
+    ctx = G.Context(mesh, p)
+    ctx.set_sh(L, beta, mode=G.SH_RECUR)
+    Y = O.real_sh_basis(L, arrs["lat"], arrs["lon"])
+    Pc = np.linalg.solve(Y.T @ Y, Y.T)
+    dfac = np.zeros((L + 1) ** 2)
+    for l in range(2, L + 1):
+        dfac[l * l:(l + 1) ** 2] = beta[l] - 1.0
+    sim = O.Sim(O.Mesh(arrs, p.radius),
+                O.make_params(g=p.g, h=p.h, alpha=p.alpha, omega=p.omega, radius=p.radius,
+                              love_reduct=p.love_reduct, ecc=p.ecc, dt=p.dt, tide=p.tide, surface=p.surface,
+                              shell_thickness=p.shell_thickness))
+    sim.set_projector(Y * dfac[None, :], Pc)
+    del Y, Pc
+    for n in (1, 2, 17):
+        ctx.step(n)
+        sim.step(n)
+        errs = _compare(ctx, sim, TOL_ORBIT)
+        print("163842 cells, shell, degree 32, step %d:" % ctx.get_state()["iter"], errs)
+    ctx.close()
+
+
 @pytest.mark.parametrize("tide", ["OBLIQ", "OBLIQ_WEST", "FULL", "FULL2", "PLANET", "NONE", "ECC_RAD"])
 def test_forcing_types(product, oracle, tide):
     G = product
