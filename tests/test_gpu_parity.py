@@ -70,6 +70,23 @@ def test_ecc_one_orbit_level5(product, oracle):
     print("one-orbit errors", errs)
 
 
+def test_config1_one_orbit_10242_cells(product, oracle):
+    """The plain-method case on its own grid size (10242 cells): Enceladus,
+    h = 38 km, e = 0.0047, alpha = 1e-8, rest start, one full orbit of 4200
+    steps (dt = P / 4200 = 28.19 s; the quantised 30 s target diverges on
+    this grid).  FP64 state and the per-step dissipation series within 1e-10
+    of the oracle after the orbit."""
+    ctx, sim, mesh, p, nsteps = _pair(product, oracle, 6, dict(tide=product.ECC, time_step=28.2))
+    assert mesh.N == 10242 and nsteps == 4200
+    ctx.step(nsteps)
+    d_orc = sim.step(nsteps, record=True)
+    errs = _compare(ctx, sim, TOL_ORBIT)
+    series = ctx.dissipation_series(0, 1, nsteps - 1)
+    assert rel_l2(series, d_orc[:-1]) <= TOL_ORBIT
+    print("10242 cells, one orbit:", errs)
+    ctx.close()
+
+
 @pytest.mark.parametrize("precision,tols", [(0, (TOL_ORBIT,) * 4), (1, (None, None, None, 1e-4))])
 def test_full_size_step_matches_oracle(product, oracle, precision, tols):
     """The benchmark's grid at full size (163842 cells, 491520 faces), the
