@@ -601,13 +601,18 @@ def main():
     # bound by FP64 (FP32) vector arithmetic, the others by HBM bytes
     dom = max(kernels.values(), key=lambda r: r["ms"])
     if "alg_flops" in dom:
+        # the arithmetic roof: MI355X's published FP64 (FP32) peak, which the
+        # vector FMA units reach as the matrix cores do; these passes issue
+        # vector FMAs, no MFMA
         peak = 78.6 if prec == G.FP64 else 157.3
         achieved = dom["alg_flops"] / (dom["ms"] * 1e-3) / 1e12 if dom["ms"] > 0 else None
-        bound, unit = ("fp64" if prec == G.FP64 else "fp32"), "TFLOP/s"
+        bound, unit = "mfma", "TFLOP/s"
+        compute = ("%s vector FMA (no MFMA); the peak is the published %s rate"
+                   % (("fp64", "FP64") if prec == G.FP64 else ("fp32", "FP32")))
     else:
         peak = HBM_PEAK_GBS
         achieved = dom["alg_bytes"] / (dom["ms"] * 1e-3) / 1e9 if dom["ms"] > 0 else None
-        bound, unit = "hbm", "GB/s"
+        bound, unit, compute = "hbm", "GB/s", None
     dk, dt_ = dom["kernel"].split("<", 1)
     traffic = measured_traffic(dk, dt_[:-1], mesh.N)
     step_bytes = sum(alg.values())
@@ -656,6 +661,7 @@ def main():
             "frac": achieved / peak if achieved else None,
             "traffic": traffic,
             "kernel": dom["kernel"],
+            "compute": compute,
             "alg_bytes_per_launch": dom["alg_bytes"],
         },
     }
