@@ -2,6 +2,7 @@
 #include <cmath>
 #include <cstring>
 #include <exception>
+#include <new>
 #include <string>
 
 #include "odis.h"
@@ -73,22 +74,24 @@ extern "C" int odis_grid_arrays(const odis_grid *g, double *lat, double *lon, in
 extern "C" int odis_grid_from_arrays(int n, const double *lat, const double *lon, const int *friends,
                                      const double *clat, const double *clon, odis_grid **out) {
   if (n < 12 || !lat || !lon || !friends || !clat || !clon || !out) return host_fail(ODIS_E_INVALID, "bad argument");
-  auto *g = new odis_grid;
-  g->g.n = n;
-  g->g.lat.assign(lat, lat + n);
-  g->g.lon.assign(lon, lon + n);
-  g->g.friends.assign(friends, friends + size_t(n) * 6);
-  g->g.clat.assign(clat, clat + size_t(n) * 6);
-  g->g.clon.assign(clon, clon + size_t(n) * 6);
-  for (size_t k = 0; k < g->g.friends.size(); k++) {
-    const int f = g->g.friends[k];
-    if (f >= n || (f < 0 && k % 6 != 5)) {
-      delete g;
-      return host_fail(ODIS_E_INVALID, "friend index out of range");
-    }
+  // friend ids lie in [0, n); -1 (no sixth neighbour, pentagons) only in the last slot
+  for (size_t k = 0; k < size_t(n) * 6; k++) {
+    const int f = friends[k];
+    if (f >= n || f < -1 || (f < 0 && k % 6 != 5)) return host_fail(ODIS_E_INVALID, "friend index out of range");
   }
-  *out = g;
-  return ODIS_OK;
+  try {
+    auto *g = new odis_grid;
+    g->g.n = n;
+    g->g.lat.assign(lat, lat + n);
+    g->g.lon.assign(lon, lon + n);
+    g->g.friends.assign(friends, friends + size_t(n) * 6);
+    g->g.clat.assign(clat, clat + size_t(n) * 6);
+    g->g.clon.assign(clon, clon + size_t(n) * 6);
+    *out = g;
+    return ODIS_OK;
+  } catch (const std::bad_alloc &) {
+    return host_fail(ODIS_E_NOMEM, "host allocation failed");
+  }
 }
 
 extern "C" int odis_grid_free(odis_grid *g) {
@@ -206,6 +209,7 @@ extern "C" int odis_mesh_free(odis_mesh *m) {
 extern "C" int odis_round_period(double ang_vel, double *period, double *omega) {
   if (!period || !omega || !(ang_vel > 0)) return host_fail(ODIS_E_INVALID, "bad argument");
   const double per = 2. * odis::kPi / ang_vel;
+  if (!(per >= 2.0 && per < 2e9)) return host_fail(ODIS_E_INVALID, "period outside [2 s, 2e9 s)");
   const int it = int(std::round(per / 2)) * 2;
   *period = double(it);
   *omega = 2 * odis::kPi / double(it);
@@ -214,6 +218,7 @@ extern "C" int odis_round_period(double ang_vel, double *period, double *omega) 
 
 extern "C" int odis_quantise_dt(double period, double target, double *dt, int *n) {
   if (!dt || !n || !(period > 0) || !(target > 0)) return host_fail(ODIS_E_INVALID, "bad argument");
+  if (!(period / target < 1e8)) return host_fail(ODIS_E_INVALID, "more than 1e8 steps per period");
   odis::quantise_dt(period, target, dt, n);
   return ODIS_OK;
 }
@@ -223,6 +228,8 @@ extern "C" int odis_config_read(const char *case_dir, odis_params *p, int *steps
   if (!case_dir) return host_fail(ODIS_E_INVALID, "null case_dir");
   try {
     odis::Config c = odis::read_config(case_dir);
+    if (!(c.period > 0) || !(c.time_step > 0) || !(c.period / c.time_step < 1e8))
+      return host_fail(ODIS_E_INVALID, "time step: need 0 < period / time step < 1e8");
     double dt;
     int n;
     odis::quantise_dt(c.period, c.time_step, &dt, &n);
@@ -258,7 +265,11 @@ extern "C" int odis_config_read(const char *case_dir, odis_params *p, int *steps
 extern "C" int odis_surface_beta(int surface, int l_max, double radius, double g, double shell, double ocean,
                                  double *beta) {
   if (!beta || l_max < 0 || l_max > 1000) return host_fail(ODIS_E_INVALID, "bad argument");
-  const std::vector<double> b = odis::surface_beta(surface, l_max, radius, g, shell, ocean);
-  for (int l = 0; l <= l_max; l++) beta[l] = b[l];
-  return ODIS_OK;
+  try {
+    const std::vector<double> b = odis::surface_beta(surface, l_max, radius, g, shell, ocean);
+    for (int l = 0; l <= l_max; l++) beta[l] = b[l];
+    return ODIS_OK;
+  } catch (const std::exception &e) {
+    return host_fail(ODIS_E_INVALID, e.what());
+  }
 }

@@ -10,6 +10,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -1859,6 +1861,28 @@ static int fail(odis_ctx *ctx, int code, const std::string &msg) {
   return code;
 }
 
+// Runs an entry point's body; a host exception (a staging vector that
+// cannot be allocated, mostly) ends as an error code, so nothing is thrown
+// across the C ABI.
+template <typename F>
+static int guarded(odis_ctx *ctx, F &&body) {
+  try {
+    return body();
+  } catch (const std::bad_alloc &) {
+    try {
+      return fail(ctx, ODIS_E_NOMEM, "host allocation failed");
+    } catch (...) {
+      return ODIS_E_NOMEM;
+    }
+  } catch (const std::exception &ex) {
+    try {
+      return fail(ctx, ODIS_E_INVALID, ex.what());
+    } catch (...) {
+      return ODIS_E_INVALID;
+    }
+  }
+}
+
 template <typename T>
 static int dev_alloc(odis_ctx *ctx, T **p, size_t count) {
   void *q = nullptr;
@@ -2092,8 +2116,10 @@ extern "C" int odis_destroy(odis_ctx *ctx) {
   return ODIS_OK;
 }
 
-extern "C" int odis_create(const odis_mesh_desc *m, const odis_params *params, int n_inst, int precision,
-                           odis_ctx **out) {
+// *building holds the context while it is assembled, for odis_create to free
+// if a host exception ends the assembly
+static int create_impl(const odis_mesh_desc *m, const odis_params *params, int n_inst, int precision,
+                       odis_ctx **out, odis_ctx **building) {
   if (!m || !params || !out || n_inst < 1) return fail(nullptr, ODIS_E_INVALID, "null argument or n_inst < 1");
   if (precision != ODIS_FP64 && precision != ODIS_FP32) return fail(nullptr, ODIS_E_INVALID, "bad precision");
   const int N = m->n_cells, E = m->n_faces;
@@ -2132,6 +2158,7 @@ extern "C" int odis_create(const odis_mesh_desc *m, const odis_params *params, i
     }
 
   odis_ctx *ctx = new odis_ctx();
+  *building = ctx;
   ctx->N = N;
   ctx->E = E;
   ctx->B = n_inst;
@@ -2331,6 +2358,7 @@ extern "C" int odis_create(const odis_mesh_desc *m, const odis_params *params, i
     if (he == hipSuccess) he = hipDeviceSynchronize();
     if (he != hipSuccess) rc = fail(ctx, ODIS_E_HIP, hipGetErrorString(he));
   }
+  *building = nullptr;
   if (rc) {
     const std::string msg = ctx->err;
     odis_destroy(ctx);
@@ -2339,6 +2367,14 @@ extern "C" int odis_create(const odis_mesh_desc *m, const odis_params *params, i
   }
   *out = ctx;
   return ODIS_OK;
+}
+
+extern "C" int odis_create(const odis_mesh_desc *m, const odis_params *params, int n_inst, int precision,
+                           odis_ctx **out) {
+  odis_ctx *building = nullptr;
+  const int rc = guarded(nullptr, [&] { return create_impl(m, params, n_inst, precision, out, &building); });
+  if (building) odis_destroy(building);  // assembly ended by an exception; keeps the error text
+  return rc;
 }
 
 extern "C" const char *odis_last_error(const odis_ctx *ctx) {
@@ -2399,8 +2435,8 @@ static int put_state(odis_ctx *ctx, Ops<T> &o, int b, const double *v, const dou
   return ODIS_OK;
 }
 
-extern "C" int odis_set_state(odis_ctx *ctx, int inst, const double *v, const double *dvdt, const double *eta,
-                              const double *detadt, long long iter) {
+static int set_state_impl(odis_ctx *ctx, int inst, const double *v, const double *dvdt, const double *eta,
+                          const double *detadt, long long iter) {
   if (!ctx) return ODIS_E_INVALID;
   if (inst < 0 || inst >= ctx->B) return fail(ctx, ODIS_E_RANGE, "instance out of range");
   if (iter < 0) return fail(ctx, ODIS_E_INVALID, "iter < 0");
@@ -2411,6 +2447,11 @@ extern "C" int odis_set_state(odis_ctx *ctx, int inst, const double *v, const do
   ctx->il_fresh = 0;
   return ctx->precision == ODIS_FP64 ? put_state(ctx, ctx->d, inst, v, dvdt, eta, detadt, iter)
                                      : put_state(ctx, ctx->f, inst, v, dvdt, eta, detadt, iter);
+}
+
+extern "C" int odis_set_state(odis_ctx *ctx, int inst, const double *v, const double *dvdt, const double *eta,
+                              const double *detadt, long long iter) {
+  return guarded(ctx, [&] { return set_state_impl(ctx, inst, v, dvdt, eta, detadt, iter); });
 }
 
 template <typename T>
@@ -2457,8 +2498,8 @@ static int get_state_t(odis_ctx *ctx, Ops<T> &o, int b, double *v, double *dv, d
   return ODIS_OK;
 }
 
-extern "C" int odis_get_state(odis_ctx *ctx, int inst, double *v, double *dvdt, double *eta, double *detadt,
-                              long long *iter) {
+static int get_state_impl(odis_ctx *ctx, int inst, double *v, double *dvdt, double *eta, double *detadt,
+                          long long *iter) {
   if (!ctx) return ODIS_E_INVALID;
   if (inst < 0 || inst >= ctx->B) return fail(ctx, ODIS_E_RANGE, "instance out of range");
   const int rc = il_out_any(ctx);
@@ -2467,6 +2508,11 @@ extern "C" int odis_get_state(odis_ctx *ctx, int inst, double *v, double *dvdt, 
   if (iter) *iter = ctx->step;
   return ctx->precision == ODIS_FP64 ? get_state_t(ctx, ctx->d, inst, v, dvdt, eta, detadt, ctx->step)
                                      : get_state_t(ctx, ctx->f, inst, v, dvdt, eta, detadt, ctx->step);
+}
+
+extern "C" int odis_get_state(odis_ctx *ctx, int inst, double *v, double *dvdt, double *eta, double *detadt,
+                              long long *iter) {
+  return guarded(ctx, [&] { return get_state_impl(ctx, inst, v, dvdt, eta, detadt, iter); });
 }
 
 // ---------------------------------------------------------------------------
@@ -3239,13 +3285,17 @@ static int step_t(odis_ctx *ctx, Ops<T> &o, int n_steps) {
   return ODIS_OK;
 }
 
-extern "C" int odis_step(odis_ctx *ctx, int n_steps, int flags) {
+static int step_impl(odis_ctx *ctx, int n_steps, int flags) {
   if (!ctx) return ODIS_E_INVALID;
   if (n_steps < 0) return fail(ctx, ODIS_E_INVALID, "n_steps < 0");
   int rc = ctx->precision == ODIS_FP64 ? step_t(ctx, ctx->d, n_steps) : step_t(ctx, ctx->f, n_steps);
   if (rc) return rc;
   if (flags & ODIS_SYNC) CK(hipStreamSynchronize(ctx->stream));
   return ODIS_OK;
+}
+
+extern "C" int odis_step(odis_ctx *ctx, int n_steps, int flags) {
+  return guarded(ctx, [&] { return step_impl(ctx, n_steps, flags); });
 }
 
 extern "C" int odis_sync(odis_ctx *ctx) {
@@ -3272,7 +3322,7 @@ static int diss_t(odis_ctx *ctx, Ops<T> &o) {
   return ODIS_OK;
 }
 
-extern "C" int odis_get_dissipation(odis_ctx *ctx, double *per_inst) {
+static int get_dissipation_impl(odis_ctx *ctx, double *per_inst) {
   if (!ctx || !per_inst) return ODIS_E_INVALID;
   int rc = ctx->precision == ODIS_FP64 ? diss_t(ctx, ctx->d) : diss_t(ctx, ctx->f);
   if (rc) return rc;
@@ -3283,9 +3333,13 @@ extern "C" int odis_get_dissipation(odis_ctx *ctx, double *per_inst) {
   return ODIS_OK;
 }
 
+extern "C" int odis_get_dissipation(odis_ctx *ctx, double *per_inst) {
+  return guarded(ctx, [&] { return get_dissipation_impl(ctx, per_inst); });
+}
+
 extern "C" int odis_series_capacity(const odis_ctx *ctx) { return ctx ? ctx->cap : ODIS_E_INVALID; }
 
-extern "C" int odis_get_dissipation_series(odis_ctx *ctx, int inst, long long first, int count, double *out) {
+static int get_dissipation_series_impl(odis_ctx *ctx, int inst, long long first, int count, double *out) {
   if (!ctx || !out || count < 0) return ODIS_E_INVALID;
   if (inst < 0 || inst >= ctx->B) return fail(ctx, ODIS_E_RANGE, "instance out of range");
   CK(hipStreamSynchronize(ctx->stream));
@@ -3297,6 +3351,10 @@ extern "C" int odis_get_dissipation_series(odis_ctx *ctx, int inst, long long fi
                hipMemcpyDeviceToHost));
   for (int k = 0; k < count; k++) out[k] = all[size_t((first + k) % ctx->cap)];
   return ODIS_OK;
+}
+
+extern "C" int odis_get_dissipation_series(odis_ctx *ctx, int inst, long long first, int count, double *out) {
+  return guarded(ctx, [&] { return get_dissipation_series_impl(ctx, inst, first, count, out); });
 }
 
 template <typename T>
@@ -3335,12 +3393,16 @@ static int vel_t(odis_ctx *ctx, Ops<T> &o, int b, const double *normal, double *
   return ODIS_OK;
 }
 
-extern "C" int odis_get_velocity_en(odis_ctx *ctx, int inst, const double *face_normal, double *uv) {
+static int get_velocity_en_impl(odis_ctx *ctx, int inst, const double *face_normal, double *uv) {
   if (!ctx || !face_normal || !uv) return ODIS_E_INVALID;
   if (inst < 0 || inst >= ctx->B) return fail(ctx, ODIS_E_RANGE, "instance out of range");
   CK(hipStreamSynchronize(ctx->stream));
   return ctx->precision == ODIS_FP64 ? vel_t(ctx, ctx->d, inst, face_normal, uv)
                                      : vel_t(ctx, ctx->f, inst, face_normal, uv);
+}
+
+extern "C" int odis_get_velocity_en(odis_ctx *ctx, int inst, const double *face_normal, double *uv) {
+  return guarded(ctx, [&] { return get_velocity_en_impl(ctx, inst, face_normal, uv); });
 }
 
 extern "C" int odis_request_stop(odis_ctx *ctx) {
@@ -3355,7 +3417,7 @@ extern "C" int odis_enable_timing(odis_ctx *ctx, int stride) {
   return ODIS_OK;
 }
 
-extern "C" int odis_set_option(odis_ctx *ctx, int option, int value) {
+static int set_option_impl(odis_ctx *ctx, int option, int value) {
   if (!ctx) return ODIS_E_INVALID;
   if (option == ODIS_OPT_GRAPHS) {
     ctx->use_graphs = value ? 1 : 0;
@@ -3432,6 +3494,10 @@ extern "C" int odis_set_option(odis_ctx *ctx, int option, int value) {
   return fail(ctx, ODIS_E_INVALID, "unknown option");
 }
 
+extern "C" int odis_set_option(odis_ctx *ctx, int option, int value) {
+  return guarded(ctx, [&] { return set_option_impl(ctx, option, value); });
+}
+
 extern "C" int odis_get_option(odis_ctx *ctx, int option, int *value) {
   if (!ctx || !value) return ODIS_E_INVALID;
   if (option == ODIS_OPT_GRAPHS) *value = ctx->use_graphs;
@@ -3466,8 +3532,8 @@ extern "C" int odis_get_timing(odis_ctx *ctx, double *face_ms, double *cell_ms, 
 // geodesic -> lat-lon interpolation (src/interpolation.cpp:208-287)
 // ---------------------------------------------------------------------------
 
-extern "C" int odis_interp_ll_set(odis_ctx *ctx, int n_rows, int n_cols, const int *row_ptr, const int *col,
-                                  const double *val) {
+static int interp_ll_set_impl(odis_ctx *ctx, int n_rows, int n_cols, const int *row_ptr, const int *col,
+                              const double *val) {
   if (!ctx || !row_ptr || !col || !val || n_rows < 1) return ODIS_E_INVALID;
   if (n_cols != 3 * ctx->N) return fail(ctx, ODIS_E_INVALID, "interpolation matrix must have 3N columns");
   if (row_ptr[0] != 0) return fail(ctx, ODIS_E_INVALID, "row_ptr[0] must be 0");
@@ -3507,8 +3573,13 @@ extern "C" int odis_interp_ll_set(odis_ctx *ctx, int n_rows, int n_cols, const i
   return ODIS_OK;
 }
 
-extern "C" int odis_rbf_set(odis_ctx *ctx, int n_rows, int n_cols, const int *row_ptr, const int *col,
-                            const double *val) {
+extern "C" int odis_interp_ll_set(odis_ctx *ctx, int n_rows, int n_cols, const int *row_ptr, const int *col,
+                                  const double *val) {
+  return guarded(ctx, [&] { return interp_ll_set_impl(ctx, n_rows, n_cols, row_ptr, col, val); });
+}
+
+static int rbf_set_impl(odis_ctx *ctx, int n_rows, int n_cols, const int *row_ptr, const int *col,
+                        const double *val) {
   if (!ctx || !row_ptr || !col || !val) return ODIS_E_INVALID;
   const int N = ctx->N, E = ctx->E;
   if (n_rows != 3 * N || n_cols != E) return fail(ctx, ODIS_E_INVALID, "RBF operator must be 3N x E");
@@ -3558,7 +3629,12 @@ extern "C" int odis_rbf_set(odis_ctx *ctx, int n_rows, int n_cols, const int *ro
   return ODIS_OK;
 }
 
-extern "C" int odis_velocity_xyz(odis_ctx *ctx, int inst, double *xyz) {
+extern "C" int odis_rbf_set(odis_ctx *ctx, int n_rows, int n_cols, const int *row_ptr, const int *col,
+                            const double *val) {
+  return guarded(ctx, [&] { return rbf_set_impl(ctx, n_rows, n_cols, row_ptr, col, val); });
+}
+
+static int velocity_xyz_impl(odis_ctx *ctx, int inst, double *xyz) {
   if (!ctx || !xyz) return ODIS_E_INVALID;
   if (inst < 0 || inst >= ctx->B) return fail(ctx, ODIS_E_RANGE, "instance out of range");
   if (!ctx->rbf_idx) return fail(ctx, ODIS_E_INVALID, "no RBF operator set");
@@ -3583,7 +3659,11 @@ extern "C" int odis_velocity_xyz(odis_ctx *ctx, int inst, double *xyz) {
   return ODIS_OK;
 }
 
-extern "C" int odis_interp_ll(odis_ctx *ctx, int inst, double *ll) {
+extern "C" int odis_velocity_xyz(odis_ctx *ctx, int inst, double *xyz) {
+  return guarded(ctx, [&] { return velocity_xyz_impl(ctx, inst, xyz); });
+}
+
+static int interp_ll_impl(odis_ctx *ctx, int inst, double *ll) {
   if (!ctx || !ll) return ODIS_E_INVALID;
   if (inst < 0 || inst >= ctx->B) return fail(ctx, ODIS_E_RANGE, "instance out of range");
   if (!ctx->ll_rows) return fail(ctx, ODIS_E_INVALID, "no interpolation matrix set");
@@ -3600,6 +3680,10 @@ extern "C" int odis_interp_ll(odis_ctx *ctx, int inst, double *ll) {
   CK(hipMemcpyAsync(ll, ctx->ll_out, sizeof(double) * ctx->ll_rows, hipMemcpyDeviceToHost, ctx->stream));
   CK(hipStreamSynchronize(ctx->stream));
   return ODIS_OK;
+}
+
+extern "C" int odis_interp_ll(odis_ctx *ctx, int inst, double *ll) {
+  return guarded(ctx, [&] { return interp_ll_impl(ctx, inst, ll); });
 }
 
 // ---------------------------------------------------------------------------
@@ -3656,7 +3740,7 @@ static bool spd_inverse(std::vector<double> &A, int n) {
   return true;
 }
 
-extern "C" int odis_sh_set_basis(odis_ctx *ctx, int l_max, const double *beta, int mode) {
+static int sh_set_basis_impl(odis_ctx *ctx, int l_max, const double *beta, int mode) {
   if (!ctx) return ODIS_E_INVALID;
   if (mode != ODIS_SH_OFF && mode != ODIS_SH_FP64 && mode != ODIS_SH_SPLIT_BF16 && mode != ODIS_SH_RECUR &&
       mode != ODIS_SH_MFMA)
@@ -3871,8 +3955,12 @@ extern "C" int odis_sh_set_basis(odis_ctx *ctx, int l_max, const double *beta, i
   return ODIS_OK;
 }
 
+extern "C" int odis_sh_set_basis(odis_ctx *ctx, int l_max, const double *beta, int mode) {
+  return guarded(ctx, [&] { return sh_set_basis_impl(ctx, l_max, beta, mode); });
+}
+
 // field (reference order) -> LSQ coefficients with the context's basis
-extern "C" int odis_sh_project(odis_ctx *ctx, const double *field, double *coeffs) {
+static int sh_project_impl(odis_ctx *ctx, const double *field, double *coeffs) {
   if (!ctx || !field || !coeffs) return ODIS_E_INVALID;
   if (ctx->sh_mode == ODIS_SH_OFF) return fail(ctx, ODIS_E_INVALID, "no spherical-harmonic basis set");
   const int K = ctx->sh_K, N = ctx->N;
@@ -3895,7 +3983,11 @@ extern "C" int odis_sh_project(odis_ctx *ctx, const double *field, double *coeff
   return ODIS_OK;
 }
 
-extern "C" int odis_sh_synthesize(odis_ctx *ctx, const double *coeffs, double *field) {
+extern "C" int odis_sh_project(odis_ctx *ctx, const double *field, double *coeffs) {
+  return guarded(ctx, [&] { return sh_project_impl(ctx, field, coeffs); });
+}
+
+static int sh_synthesize_impl(odis_ctx *ctx, const double *coeffs, double *field) {
   if (!ctx || !field || !coeffs) return ODIS_E_INVALID;
   if (ctx->sh_mode == ODIS_SH_OFF) return fail(ctx, ODIS_E_INVALID, "no spherical-harmonic basis set");
   const int K = ctx->sh_K, N = ctx->N;
@@ -3917,4 +4009,8 @@ extern "C" int odis_sh_synthesize(odis_ctx *ctx, const double *coeffs, double *f
   if (e != hipSuccess) return fail(ctx, ODIS_E_HIP, hipGetErrorString(e));
   for (int r = 0; r < N; r++) field[ctx->cell_inv[r]] = y[r];
   return ODIS_OK;
+}
+
+extern "C" int odis_sh_synthesize(odis_ctx *ctx, const double *coeffs, double *field) {
+  return guarded(ctx, [&] { return sh_synthesize_impl(ctx, coeffs, field); });
 }

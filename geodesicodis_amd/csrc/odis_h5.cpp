@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
+#include <exception>
 #include <string>
 #include <vector>
 
@@ -161,8 +162,8 @@ bool write_face_positions(odis_h5 *h, const double *face_lat, const double *face
 
 }  // namespace
 
-extern "C" int odis_h5_open(const char *path, int n_cells, int n_faces, int n_slices, int fields,
-                            const double *face_lat, const double *face_lon, odis_h5 **out) {
+static int h5_open_impl(const char *path, int n_cells, int n_faces, int n_slices, int fields,
+                        const double *face_lat, const double *face_lon, odis_h5 **out) {
   if (!path || !out || n_cells < 1 || n_faces < 1 || n_slices < 1) return ODIS_E_INVALID;
   *out = nullptr;
   if (!load_api()) return ODIS_E_UNSUPPORTED;
@@ -192,7 +193,16 @@ extern "C" int odis_h5_open(const char *path, int n_cells, int n_faces, int n_sl
   return ODIS_OK;
 }
 
-extern "C" int odis_h5_write(odis_h5 *h, int field, int slice, const double *data) {
+extern "C" int odis_h5_open(const char *path, int n_cells, int n_faces, int n_slices, int fields,
+                            const double *face_lat, const double *face_lon, odis_h5 **out) {
+  try {
+    return h5_open_impl(path, n_cells, n_faces, n_slices, fields, face_lat, face_lon, out);
+  } catch (const std::exception &) {  // host allocation; nothing is thrown across the C ABI
+    return ODIS_E_NOMEM;
+  }
+}
+
+static int h5_write_impl(odis_h5 *h, int field, int slice, const double *data) {
   if (!h || !data) return ODIS_E_INVALID;
   if (slice < 0 || slice >= h->slices) return ODIS_E_RANGE;
   if (!(h->fields & field)) return ODIS_E_INVALID;
@@ -226,6 +236,14 @@ extern "C" int odis_h5_write(odis_h5 *h, int field, int slice, const double *dat
     return ODIS_E_INVALID;
   }
   return ok ? ODIS_OK : ODIS_E_IO;
+}
+
+extern "C" int odis_h5_write(odis_h5 *h, int field, int slice, const double *data) {
+  try {
+    return h5_write_impl(h, field, slice, data);
+  } catch (const std::exception &) {  // the float staging vectors
+    return ODIS_E_NOMEM;
+  }
 }
 
 extern "C" int odis_h5_close(odis_h5 *h) {

@@ -355,7 +355,13 @@ extern "C" int odis_run_case(const char *case_dir, const char *grid_dir, int pre
     return ODIS_E_INVALID;
   }
   log.msg("Entering time solver AB3...");
-  const int rc = ab3_explicit_gpu(c, M, precision, dir, log);
+  int rc;
+  try {
+    rc = ab3_explicit_gpu(c, M, precision, dir, log);
+  } catch (const std::exception &e) {  // host staging buffers; the contexts are freed by their guards
+    log.error(std::string("ERROR: ") + e.what());
+    return ODIS_E_NOMEM;
+  }
   if (rc > 0) log.msg("SOLVER RETURNED WITH AN ERROR...");
   else if (rc == 0) log.msg("Calculations appear to have finished!");
   return rc;

@@ -134,6 +134,28 @@ def test_grid_read_missing_file(product):
     assert ei.value.code == -8
 
 
+def test_host_argument_checks(product):
+    """The host entry points reject, with ODIS_E_INVALID, what would overflow
+    or run away: friend ids outside [-1, n) or -1 outside the sixth slot, a
+    period that does not fit a 32-bit step count, and more than 1e8 steps
+    per period."""
+    g, _ = generated_grid(2)
+    a = g.arrays()
+    for slot, bad in ((5, -2), (0, -1), (3, len(a["lat"]))):
+        fr = a["friends"].copy()
+        fr[7, slot] = bad
+        with pytest.raises(product.OdisError) as ei:
+            product.Grid.from_arrays(a["lat"], a["lon"], fr, a["clat"], a["clon"])
+        assert ei.value.code == -1
+    g2 = product.Grid.from_arrays(a["lat"], a["lon"], a["friends"], a["clat"], a["clon"])
+    assert np.array_equal(g2.arrays()["friends"], a["friends"])
+    with pytest.raises(product.OdisError):
+        product.round_period(1e-12)
+    with pytest.raises(product.OdisError):
+        product.quantise_dt(118394.0, 1e-6)
+    assert product.quantise_dt(118394.0, 30.0)[1] == 4000
+
+
 INPUT_ENCELADUS = """radius; \t 252.1e3; \t top radius \t;
 k2;  0.0;   k2 Love number;
 h2;  0.0;  h2 Love number;
