@@ -212,6 +212,30 @@ def test_config_membrane_shell(product, tmp_path):
     assert 0.9 < p.love_reduct < 1.0 and p.surface == product.LID_MEMBR
 
 
+@pytest.mark.parametrize("limit", ["fluid", "rigid"])
+def test_free_loading_factors_limits(product, tmp_path, limit):
+    """FREE_LOADING (src/boundaryConditions.cpp:29-77) pinned by its two
+    limits.  The body's effective rigidity scales as 1 / (g R): a huge g R
+    makes the body fluid, a tiny one rigid.
+      - Loading factors beta_l = 1 - gamma_l: a fluid body compensates the
+        ocean load, so beta_l = 1 - rho_w / rho_b for every degree; a rigid
+        one leaves the load's self-attraction, beta_l = 1 - 3 rho_w /
+        ((2 l + 1) rho_b).
+      - The tidal factor 1 + k2 - h2 of the config: 0 (fluid), 1 (rigid)."""
+    rho_w, rho_b, L = 1000.0, 1609.22, 12
+    g, r = (1e8, 1e9) if limit == "fluid" else (1e-8, 1e-3)
+    beta = np.array(product.surface_beta(product.FREE_LOADING, L, r, g, 0.0, 38e3))
+    l = np.arange(1, L + 1)
+    want = 1 - rho_w / rho_b + 0 * l if limit == "fluid" else 1 - 3 * rho_w / ((2 * l + 1) * rho_b)
+    assert np.max(np.abs(beta[1:] - want)) < 1e-6
+    text = INPUT_ENCELADUS.replace("FREE;", "FREE_LOADING;").replace("0.113;", "%g;" % g)
+    text = text.replace("252.1e3;", "%g;" % r)
+    (tmp_path / "input.in").write_text(text)
+    p, _ = product.read_config(tmp_path)
+    assert p.surface == product.FREE_LOADING
+    assert abs(p.love_reduct - (0.0 if limit == "fluid" else 1.0)) < 1e-6
+
+
 def test_run_case_without_input(product, tmp_path):
     with pytest.raises(product.OdisError):
         product.run_case(tmp_path)
