@@ -29,7 +29,7 @@ extern "C" {
 #define ODIS_E_HIP          -2   /* HIP runtime error                        */
 #define ODIS_E_NONFINITE    -3   /* non-finite dissipation (blow-up)         */
 #define ODIS_E_UNSUPPORTED  -4   /* option with no live reference semantics  */
-#define ODIS_E_NOMEM        -5
+#define ODIS_E_NOMEM        -5   /* device or host allocation failed        */
 #define ODIS_E_RANGE        -6   /* index / size outside what was created    */
 #define ODIS_E_STOPPED      -7   /* odis_request_stop() honoured             */
 #define ODIS_E_IO           -8   /* file not found / unreadable              */
