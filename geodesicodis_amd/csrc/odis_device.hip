@@ -738,34 +738,38 @@ __global__ __launch_bounds__(kBlock) void k_sh_bsum(int K, int n_part, double *b
   b_part[(size_t(gridDim.y) * n_part + b) * K + j] = t;
 }
 
-// c_k = D_k sum_j Minv[k][j] b_j: one wavefront per k; each lane loads its
-// row entries and b entries (32 of each in flight) and the wave sums in a
-// fixed order.  cmaj (nullable) also receives c in m-major order
-// (put_mmajor), csplit (nullable) its split copy (put_split).
+// c_k = D_k sum_j Minv[k][j] b_j: one workgroup per (k, instance); thread t
+// loads the row and b entries j = t, t + 256, ... (all of a row in flight
+// at once), each wave sums its lanes and thread 0 adds the wave sums in
+// order.  cmaj (nullable) also receives c in m-major order (put_mmajor),
+// csplit (nullable) its split copy (put_split).
 __global__ __launch_bounds__(kBlock) void k_sh_coef(int K, const double *__restrict__ Minv, const double *D,
                                                     const double *__restrict__ bsum, double *c, const double *chain,
                                                     double *cmaj, unsigned short *csplit, int KB) {
-  const int b = blockIdx.y;
-  const int lane = threadIdx.x & 63;
-  const int k = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-  if (k >= K) return;
+  __shared__ double part[kBlock / 64];
+  const int b = blockIdx.y, k = blockIdx.x;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const double *row = Minv + size_t(k) * K, *bv = bsum + size_t(b) * K;
   double acc = 0.0;
-  for (int j0 = lane; j0 < K; j0 += 32 * 64) {
-    double r[32], x[32];
+  for (int j0 = threadIdx.x; j0 < K; j0 += 8 * kBlock) {
+    double r[8], x[8];
 #pragma unroll
-    for (int u = 0; u < 32; ++u) {
-      const bool in = j0 + 64 * u < K;
-      r[u] = in ? row[j0 + 64 * u] : 0.0;
-      x[u] = in ? bv[j0 + 64 * u] : 0.0;
+    for (int u = 0; u < 8; ++u) {
+      const bool in = j0 + kBlock * u < K;
+      r[u] = in ? row[j0 + kBlock * u] : 0.0;
+      x[u] = in ? bv[j0 + kBlock * u] : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < 32; ++u)
-      if (j0 + 64 * u < K) acc += r[u] * x[u];
+    for (int u = 0; u < 8; ++u)
+      if (j0 + kBlock * u < K) acc += r[u] * x[u];
   }
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-  if (lane != 0) return;
-  const double ck = (D ? D[size_t(b) * K + k] : 1.0) * acc;
+  if (lane == 0) part[w] = acc;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double sum = 0.0;
+  for (int q = 0; q < kBlock / 64; ++q) sum += part[q];
+  const double ck = (D ? D[size_t(b) * K + k] : 1.0) * sum;
   c[size_t(b) * K + k] = ck;
   if (cmaj) put_mmajor(K, b, k, ck, chain, cmaj);
   if (csplit) put_split(KB, b, k, ck, csplit);
@@ -2777,7 +2781,7 @@ static int sh_apply(odis_ctx *ctx, const T *x, long long stride, int nb, const d
     const double *bsum = ctx->sh_b + size_t(nb) * n_part * K;
     hipExtLaunchKernelGGL(k_sh_bsum, dim3((K + kBlock - 1) / kBlock, nb), blk, 0, ctx->stream,
                           ev ? ev[2] : nullptr, nullptr, 0, K, n_part, ctx->sh_b);
-    hipExtLaunchKernelGGL(k_sh_coef, dim3((K + 3) / 4, nb), blk, 0, ctx->stream, nullptr, ev ? ev[3] : nullptr,
+    hipExtLaunchKernelGGL(k_sh_coef, dim3(K, nb), blk, 0, ctx->stream, nullptr, ev ? ev[3] : nullptr,
                           0, K, ctx->sh_Minv, D, bsum, c, chain,
                           ctx->sh_mode == ODIS_SH_RECUR ? ctx->sh_cmaj : static_cast<double *>(nullptr),
                           mfma ? ctx->sh_Cs : static_cast<unsigned short *>(nullptr), ctx->sh_Kp / 32);
