@@ -717,25 +717,31 @@ __device__ __forceinline__ void put_split(int KB, int b, int k, double ck, unsig
 }
 
 // b_j = sum_q b_part[q][j] over the n_part analysis partials of instance b
-// ([B][n_part][K]), added in q order; written after the partials of every
-// instance.  One lane per j; the loads of 32 partials are issued before
-// they are added (the partials come from other XCDs' L2, so latency, not
-// bandwidth, sets the time).
+// ([B][n_part][K]): a workgroup per 64 harmonics; wave w adds the partials
+// q = w, w + 4, ... (their loads in flight together) and the four wave sums
+// are added in order through LDS.  Written after the partials of every
+// instance.
 __global__ __launch_bounds__(kBlock) void k_sh_bsum(int K, int n_part, double *b_part) {
-  const int b = blockIdx.y;
-  const int j = blockIdx.x * kBlock + threadIdx.x;
-  if (j >= K) return;
-  const double *p = b_part + size_t(b) * n_part * K + j;
+  constexpr int W = kBlock / 64;
+  __shared__ double part[W][64];
+  const int b = blockIdx.y, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int j = blockIdx.x * 64 + lane;
+  const double *p = b_part + size_t(b) * n_part * K + min(j, K - 1);  // lanes past K load a valid entry
   double t = 0.0;
-  for (int q0 = 0; q0 < n_part; q0 += 32) {
-    double v[32];
+  for (int q0 = w; q0 < n_part; q0 += W * 16) {
+    double v[16];
 #pragma unroll
-    for (int u = 0; u < 32; ++u) v[u] = q0 + u < n_part ? p[size_t(q0 + u) * K] : 0.0;
+    for (int u = 0; u < 16; ++u) v[u] = q0 + W * u < n_part ? p[size_t(q0 + W * u) * K] : 0.0;
 #pragma unroll
-    for (int u = 0; u < 32; ++u)
-      if (q0 + u < n_part) t += v[u];
+    for (int u = 0; u < 16; ++u)
+      if (q0 + W * u < n_part) t += v[u];
   }
-  b_part[(size_t(gridDim.y) * n_part + b) * K + j] = t;
+  part[w][lane] = t;
+  __syncthreads();
+  if (w != 0 || j >= K) return;
+  double sum = 0.0;
+  for (int q = 0; q < W; ++q) sum += part[q][lane];
+  b_part[(size_t(gridDim.y) * n_part + b) * K + j] = sum;
 }
 
 // c_k = D_k sum_j Minv[k][j] b_j: one workgroup per (k, instance); thread t
@@ -2779,7 +2785,7 @@ static int sh_apply(odis_ctx *ctx, const T *x, long long stride, int nb, const d
   if (ana) {
     // partial sums, then c = D Minv b; events around both
     const double *bsum = ctx->sh_b + size_t(nb) * n_part * K;
-    hipExtLaunchKernelGGL(k_sh_bsum, dim3((K + kBlock - 1) / kBlock, nb), blk, 0, ctx->stream,
+    hipExtLaunchKernelGGL(k_sh_bsum, dim3((K + 63) / 64, nb), blk, 0, ctx->stream,
                           ev ? ev[2] : nullptr, nullptr, 0, K, n_part, ctx->sh_b);
     hipExtLaunchKernelGGL(k_sh_coef, dim3(K, nb), blk, 0, ctx->stream, nullptr, ev ? ev[3] : nullptr,
                           0, K, ctx->sh_Minv, D, bsum, c, chain,
