@@ -128,12 +128,18 @@ def sh_bytes(N, K, p, pb, chunks=8):
             "sh_synth": K * N * pb + N * (2 * p + cell) + K * 8}
 
 
+def sh_alg_flops(N, K):
+    """Algorithmic flops of one SH pass (analysis b = Y eta or synthesis
+    Y^T c): one multiply-add per harmonic and cell, 2 K N."""
+    return 2 * K * N
+
+
 def sh_flops(N, L):
-    """Algorithmic flops of one on-the-fly pass (analysis or synthesis) with
-    the rescaled Legendre recurrence q_l = mu q_{l-1} + beta_l q_{l-2}: per
-    cell and (l, m) the recurrence step (one multiplication, one
-    multiply-add: 3 flops) and the multiply-adds of the cos part and, for
-    m > 0, the sin part (2 flops each)."""
+    """Flops one on-the-fly pass issues (analysis or synthesis) with the
+    rescaled Legendre recurrence q_l = mu q_{l-1} + beta_l q_{l-2}: per cell
+    and (l, m) the recurrence step (one multiplication, one multiply-add:
+    3 flops) and the multiply-adds of the cos part and, for m > 0, the sin
+    part (2 flops each); 1.77 times sh_alg_flops at degree 32."""
     nlm = (L + 1) * (L + 2) // 2
     return N * (3 * nlm + 2 * nlm + 2 * (nlm - (L + 1)))
 
@@ -592,7 +598,8 @@ def main():
                 kname_sh = ({"sh_analysis": "k_sh_analysis_pair", "sh_synth": "k_sh_synth_chain"}[k] if compiled
                             else "k_%s_rec" % k)
                 kernels[k] = {"kernel": "%s<%s>" % (kname_sh, tn), "ms": v, "alg_bytes": shb[k],
-                              "alg_flops": sh_flops(mesh.N, SH_DEGREE)}
+                              "alg_flops": sh_alg_flops(mesh.N, K),
+                              "issued_flops": sh_flops(mesh.N, SH_DEGREE)}
             else:
                 kernels[k] = {"kernel": "k_%s<%s>" % (k, tn if recur else ("double" if prec == G.FP64 else
                                                                            "split_bf16")),
@@ -600,15 +607,20 @@ def main():
     # dominant kernel: the longest per step; the on-the-fly SH passes are
     # bound by FP64 (FP32) vector arithmetic, the others by HBM bytes
     dom = max(kernels.values(), key=lambda r: r["ms"])
+    issued = None
     if "alg_flops" in dom:
-        # the arithmetic roof: MI355X's published FP64 (FP32) peak, which the
-        # vector FMA units reach as the matrix cores do; these passes issue
-        # vector FMAs, no MFMA
+        # the arithmetic roof: these passes issue FP64 (FP32) vector FMAs and
+        # no MFMA; the peak is MI355X's published FP64 (FP32) rate, which the
+        # vector units share with the FP64 matrix cores.  achieved counts the
+        # algorithmic 2 K N flops per pass; issued the recurrence's own.
         peak = 78.6 if prec == G.FP64 else 157.3
         achieved = dom["alg_flops"] / (dom["ms"] * 1e-3) / 1e12 if dom["ms"] > 0 else None
-        bound, unit = "mfma", "TFLOP/s"
-        compute = ("%s vector FMA (no MFMA); the peak is the published %s rate"
+        bound, unit = ("fp64-valu" if prec == G.FP64 else "fp32-valu"), "TFLOP/s"
+        compute = ("%s vector FMA (VALU), no MFMA; peak = the published %s rate"
                    % (("fp64", "FP64") if prec == G.FP64 else ("fp32", "FP32")))
+        if dom["ms"] > 0:
+            it = dom["issued_flops"] / (dom["ms"] * 1e-3) / 1e12
+            issued = {"flops_per_launch": dom["issued_flops"], "tflops": it, "frac": it / peak}
     else:
         peak = HBM_PEAK_GBS
         achieved = dom["alg_bytes"] / (dom["ms"] * 1e-3) / 1e9 if dom["ms"] > 0 else None
@@ -661,7 +673,10 @@ def main():
             "frac": achieved / peak if achieved else None,
             "traffic": traffic,
             "kernel": dom["kernel"],
+            "kernel_ms": dom["ms"],
             "compute": compute,
+            "alg_flops_per_launch": dom.get("alg_flops"),
+            "issued": issued,
             "alg_bytes_per_launch": dom["alg_bytes"],
         },
     }
@@ -733,6 +748,7 @@ def main():
         import oracle as O
         threads = min(16, os.cpu_count() or 1)
         os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+        os.environ.setdefault("OMP_PROC_BIND", "close")  # read when the OpenMP runtime loads with the oracle
         arrs = grid.arrays()
         om = O.Mesh(arrs, p.radius)
         sim = O.Sim(om, O.make_params(g=p.g, h=p.h, alpha=p.alpha, omega=p.omega, radius=p.radius,
@@ -750,25 +766,72 @@ def main():
             del Y, Pc
             setup = time.perf_counter() - ts
         sim.step(2)
-        n, t_cpu = 0, 0.0
         batch = 2 if shell else 10
-        while t_cpu < args.cpu_seconds and n < 5000:
-            ts = time.perf_counter()
-            sim.step(batch)
-            t_cpu += time.perf_counter() - ts
-            n += batch
+
+        def cpu_sample(seconds):
+            n, t_cpu = 0, 0.0
+            while t_cpu < seconds and n < 5000:
+                ts = time.perf_counter()
+                sim.step(batch)
+                t_cpu += time.perf_counter() - ts
+                n += batch
+            return n, t_cpu
+
+        used = int(os.environ.get("OMP_NUM_THREADS", threads))
+        O.set_threads(used)
+        n, t_cpu = cpu_sample(args.cpu_seconds)
+        O.set_threads(1)
+        n1, t1 = cpu_sample(args.cpu_seconds / 3.0)
+        O.set_threads(used)
+        host = host_cpu_info()
         out["cpu_baseline"] = {
             "value": mesh.N * n / t_cpu,
             "unit": "cell-steps/s",
-            "cores": int(os.environ.get("OMP_NUM_THREADS", threads)),
+            "cores": used,
             "kind": "port",
             "sample": ("FP64 C oracle (OpenMP rows%s), same %d-cell grid and parameters, %d steps after 2 warm-up "
                        "steps%s" % (", SH projection as dense LSQ analysis and synthesis products" if shell else "",
                                     mesh.N, n, " (projector set-up %.1f s not timed)" % setup if shell else "")),
+            "implementation": "untuned restatement (the CPU oracle), not a tuned CPU port: OpenMP covers its row "
+                              "loops only, its reductions are serial",
+            "host_cpu": host["model"], "physical_cores": host["physical_cores"],
+            "logical_cpus": host["logical_cpus"], "cpus_allowed": host["cpus_allowed"],
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND", "unset"),
+            "one_thread": {"value": mesh.N * n1 / t1, "steps": n1},
         }
     if args.cpu_sizes:
         out["cpu_baseline_sizes"] = cpu_sizes_run(G, args)
     print(json.dumps(out), flush=True)
+
+
+def host_cpu_info():
+    """CPU model, physical cores (distinct (package, core) pairs), logical
+    CPUs and the CPUs this process may run on."""
+    model, pairs, phys, core = "", set(), None, None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and not model:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None:
+                    pairs.add((phys, core))
+                    phys = core = None
+        if phys is not None:
+            pairs.add((phys, core))
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = None
+    return {"model": model, "physical_cores": len(pairs) or None, "logical_cpus": os.cpu_count(),
+            "cpus_allowed": allowed}
 
 
 def cpu_sizes_run(G, args):

@@ -39,6 +39,7 @@ EXPORTS = [
     "odis_sh_project", "odis_sh_synthesize", "odis_surface_beta", "odis_set_option",
     "odis_get_option", "odis_mesh_compact_check", "odis_get_kernel_times", "odis_rbf_set",
     "odis_velocity_xyz", "odis_h5_open", "odis_h5_write", "odis_h5_close", "odis_h5_last_error",
+    "odis_analytical_obliq_west",
 ]
 H5_DISPLACEMENT, H5_VELOCITY, H5_VELOCITY_XYZ, H5_DISSIPATION, H5_DISS_AVG = 1, 2, 4, 8, 16
 SH_OFF, SH_FP64, SH_SPLIT_BF16, SH_RECUR, SH_MFMA = 0, 1, 2, 3, 4
@@ -143,6 +144,8 @@ def lib():
             "odis_sh_synthesize": ([vp, dp, dp], ctypes.c_int),
             "odis_surface_beta": ([ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                    ctypes.c_double, ctypes.c_double, dp], ctypes.c_int),
+            "odis_analytical_obliq_west": ([ctypes.POINTER(Params), ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_double, dp], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -511,6 +514,14 @@ def surface_beta(surface, l_max, top_radius, g, shell_thickness, ocean_thickness
     _host_check(lib().odis_surface_beta(int(surface), int(l_max), float(top_radius), float(g),
                                         float(shell_thickness), float(ocean_thickness), _dp(b)))
     return b
+
+
+def analytical_obliq_west(params, lat, lon, t):
+    """(U, V, dU/dt, dV/dt, eta, deta/dt) of the OBLIQ_WEST analytic tide
+    (analyticalLTE, src/analyticalLTE.cpp:133-180) at (lat, lon, t)."""
+    o = np.zeros(6)
+    _host_check(lib().odis_analytical_obliq_west(ctypes.byref(params), float(lat), float(lon), float(t), _dp(o)))
+    return o
 
 
 def run_case(case_dir, grid_dir=None, precision=FP64):

@@ -273,3 +273,11 @@ extern "C" int odis_surface_beta(int surface, int l_max, double radius, double g
     return host_fail(ODIS_E_INVALID, e.what());
   }
 }
+
+extern "C" int odis_analytical_obliq_west(const odis_params *p, double lat, double lon, double t, double *out) {
+  if (!p || !out) return host_fail(ODIS_E_INVALID, "null argument");
+  if (!(p->radius > 0) || !(p->omega > 0) || !(p->g > 0) || !(p->h > 0))
+    return host_fail(ODIS_E_INVALID, "radius, omega, g and h must be positive");
+  odis::analytical_obliq_west(p->radius, p->omega, p->g, p->h, p->obliquity, p->alpha, lat, lon, t, out);
+  return ODIS_OK;
+}

@@ -1802,6 +1802,7 @@ struct odis_ctx {
   std::string one_kernel_why;
   std::vector<int> gi_host, cyc_host;  // device-order face cells and cell cycles, to rebuild patches
   long long step = 0;          // next step index n, shared by all instances
+  long long series_lo = 0;     // first step of the series ring written since the last set_state
   double *partials = nullptr;  // [2][B][part_cap], by step parity
   double *series = nullptr;    // [B][cap]
   int *nonfinite = nullptr;    // [B]
@@ -2139,10 +2140,6 @@ static int create_impl(const odis_mesh_desc *m, const odis_params *params, int n
     return fail(nullptr, ODIS_E_INVALID, "mesh descriptor has a null array");
   for (int b = 0; b < n_inst; b++) {
     const odis_params &p = params[b];
-    if (p.tide == ODIS_GENERAL)
-      return fail(nullptr, ODIS_E_UNSUPPORTED,
-                  "GENERAL forcing needs the Legendre tables of CalcLegendreFuncs, which the reference leaves "
-                  "commented out");
     if (p.tide < 0 || p.tide > ODIS_NONE) return fail(nullptr, ODIS_E_INVALID, "unknown potential");
     if (p.tide != params[0].tide || p.omega != params[0].omega || p.friction != params[0].friction ||
         p.quad_drag != params[0].quad_drag)
@@ -2454,6 +2451,7 @@ static int set_state_impl(odis_ctx *ctx, int inst, const double *v, const double
   if (rc) return rc;
   CK(hipStreamSynchronize(ctx->stream));
   ctx->step = iter;  // the step counter is shared by all instances of a context
+  ctx->series_lo = iter;  // ring entries below iter belong to an earlier trajectory
   ctx->il_fresh = 0;
   return ctx->precision == ODIS_FP64 ? put_state(ctx, ctx->d, inst, v, dvdt, eta, detadt, iter)
                                      : put_state(ctx, ctx->f, inst, v, dvdt, eta, detadt, iter);
@@ -3353,8 +3351,8 @@ static int get_dissipation_series_impl(odis_ctx *ctx, int inst, long long first,
   if (!ctx || !out || count < 0) return ODIS_E_INVALID;
   if (inst < 0 || inst >= ctx->B) return fail(ctx, ODIS_E_RANGE, "instance out of range");
   CK(hipStreamSynchronize(ctx->stream));
-  // series[k] exists for max(0, step - cap) <= k < step
-  if (first < 0 || first + count > ctx->step || first < ctx->step - ctx->cap)
+  // series[k] exists for max(series_lo, step - cap) <= k < step
+  if (first < ctx->series_lo || first + count > ctx->step || first < ctx->step - ctx->cap)
     return fail(ctx, ODIS_E_RANGE, "series range not available");
   std::vector<double> all(ctx->cap);
   CK(hipMemcpy(all.data(), ctx->series + size_t(inst) * ctx->cap, sizeof(double) * ctx->cap,

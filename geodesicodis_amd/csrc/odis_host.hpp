@@ -105,6 +105,11 @@ void finalize_config(Config &c);   // period rounding, enum mapping, BCs
 // smallest multiple of 100 such that P / N_s <= target.
 void quantise_dt(double period, double target, double *dt, int *n_steps);
 
+// analyticalLTE OBLIQ_WEST (src/analyticalLTE.cpp:133-180): o = (U, V,
+// dU/dt, dV/dt, eta, deta/dt) at (lat, lon, t) (odis_solver.cpp).
+void analytical_obliq_west(double R, double Om, double g, double h, double obl, double al, double lat,
+                           double lon, double t, double *o);
+
 // membraneNuBeta (src/membraneConstants.cpp:8-144).
 void membrane_nu_beta(double radius, double hs, double ho, int l_max, std::vector<double> &nu,
                       std::vector<double> &beta, double *radius_ocean, double *grav_ocean);

@@ -10,12 +10,14 @@
 // enables (displacement, east / north velocity, RBF Cartesian velocity,
 // per-face dissipation, average dissipation; src/outFiles.cpp:138-681 and
 // src/timeIntegrator.cpp:140-152, 170-200, 280-304), and the end-of-run text
-// checkpoint InitialConditions/{vel,pres}_init.txt in %1.6E.
+// checkpoint InitialConditions/{vel,pres}_init.txt in %1.6E, written beside
+// an unrounded binary FP64 copy {vel,pres}_init.bin that INIT_LOAD prefers.
 #include <sys/stat.h>
 
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstring>
 #include <fstream>
 #include <iomanip>
 #include <sstream>
@@ -28,36 +30,14 @@
 
 namespace odis {
 
-namespace {
-
-struct Log {
-  std::string out, err;
-  void open(const std::string &dir) {
-    ::mkdir((dir + "/DATA").c_str(), 0775);
-    out = dir + "/DATA/OUTPUT.txt";
-    err = dir + "/DATA/ERROR.txt";
-    std::remove(out.c_str());
-    std::remove(err.c_str());
-    std::ofstream(err) << "ODIS error file. Warnings and model termination errors will be written here."
-                       << std::endl
-                       << std::endl;
-  }
-  void msg(const std::string &s) const {
-    std::ofstream f(out, std::ios::app);
-    f << s << std::endl;
-  }
-  void error(const std::string &s) const {
-    std::ofstream f(err, std::ios::app);
-    f << s << std::endl;
-  }
-};
-
 // analyticalLTE OBLIQ_WEST (src/analyticalLTE.cpp:133-180): returns
-// (U, V, dU/dt, dV/dt, eta, deta/dt) at (lat, lon, t).
-void analytical_obliq_west(const Config &c, double lat, double lon, double t, double *o) {
+// (U, V, dU/dt, dV/dt, eta, deta/dt) at (lat, lon, t) for the ocean of radius
+// R, rotation Om, gravity g, thickness h, obliquity obl (rad) and drag al.
+void analytical_obliq_west(double R, double Om, double g, double h, double obl, double al, double lat,
+                           double lon, double t, double *o) {
   using cd = std::complex<double>;
   const cd J(0.0, 1.0);
-  const double R = c.radius, Om = c.ang_vel, g = c.g, h = c.h, w = -Om, obl = c.obliquity, al = c.alpha;
+  const double w = -Om;
   const double lam = w / (2 * Om), lambs = 4 * Om * Om * R * R / (g * h);
   const double p21 = (2 + 1) * (2 + 1) / (2.0 * (2 * 2 + 1)), q11 = 1.0 * (1 + 1 - 1) / ((1 + 1.0) * (2 * 1 + 1));
   const cd L1 = lam + 1.0 / (1 * 2) + J * al / (2 * Om);
@@ -87,6 +67,30 @@ void analytical_obliq_west(const Config &c, double lat, double lon, double t, do
   o[4] = ET.real();
   o[5] = dE.real();
 }
+
+namespace {
+
+struct Log {
+  std::string out, err;
+  void open(const std::string &dir) {
+    ::mkdir((dir + "/DATA").c_str(), 0775);
+    out = dir + "/DATA/OUTPUT.txt";
+    err = dir + "/DATA/ERROR.txt";
+    std::remove(out.c_str());
+    std::remove(err.c_str());
+    std::ofstream(err) << "ODIS error file. Warnings and model termination errors will be written here."
+                       << std::endl
+                       << std::endl;
+  }
+  void msg(const std::string &s) const {
+    std::ofstream f(out, std::ios::app);
+    f << s << std::endl;
+  }
+  void error(const std::string &s) const {
+    std::ofstream f(err, std::ios::app);
+    f << s << std::endl;
+  }
+};
 
 // loadInitialConditions (src/initialConditions.cpp:19-144): up to `rows`
 // lines of "x, d0, d1, d2"; missing file -> false.
@@ -121,6 +125,51 @@ void write_text(const std::string &path, const std::vector<double> &x, const std
   for (size_t i = 0; i < x.size(); i++)
     std::fprintf(f, "%1.6E, %1.6E, %1.6E, %1.6E\n", x[i], ds[3 * i], ds[3 * i + 1], ds[3 * i + 2]);
   std::fclose(f);
+}
+
+// Binary FP64 companion of the text checkpoint: the same rows (x, d0, d1,
+// d2) unrounded, after a 16-byte header "ODISCKP1" + row count (int64,
+// little-endian as the host).  INIT_LOAD prefers it to the %1.6E text when
+// its row count matches, so a resumed run continues from the exact state.
+const char kCkpMagic[8] = {'O', 'D', 'I', 'S', 'C', 'K', 'P', '1'};
+
+void write_bin(const std::string &path, const std::vector<double> &x, const std::vector<double> &ds) {
+  std::remove(path.c_str());
+  FILE *f = std::fopen(path.c_str(), "wb");
+  if (!f) throw std::runtime_error("cannot write " + path);
+  const long long rows = (long long)x.size();
+  std::vector<double> buf(size_t(4) * x.size());
+  for (size_t i = 0; i < x.size(); i++) {
+    buf[4 * i] = x[i];
+    buf[4 * i + 1] = ds[3 * i];
+    buf[4 * i + 2] = ds[3 * i + 1];
+    buf[4 * i + 3] = ds[3 * i + 2];
+  }
+  const bool ok = std::fwrite(kCkpMagic, 1, 8, f) == 8 && std::fwrite(&rows, sizeof(rows), 1, f) == 1 &&
+                  std::fwrite(buf.data(), sizeof(double), buf.size(), f) == buf.size();
+  std::fclose(f);
+  if (!ok) throw std::runtime_error("short write " + path);
+}
+
+// false when the file is absent, not a checkpoint, or of another row count
+bool load_bin(const std::string &path, int rows, std::vector<double> &x, std::vector<double> &ds) {
+  FILE *f = std::fopen(path.c_str(), "rb");
+  if (!f) return false;
+  char magic[8];
+  long long n = -1;
+  std::vector<double> buf(size_t(4) * rows);
+  const bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kCkpMagic, 8) == 0 &&
+                  std::fread(&n, sizeof(n), 1, f) == 1 && n == rows &&
+                  std::fread(buf.data(), sizeof(double), buf.size(), f) == buf.size();
+  std::fclose(f);
+  if (!ok) return false;
+  for (int i = 0; i < rows; i++) {
+    x[i] = buf[4 * size_t(i)];
+    ds[3 * size_t(i)] = buf[4 * size_t(i) + 1];
+    ds[3 * size_t(i) + 1] = buf[4 * size_t(i) + 2];
+    ds[3 * size_t(i) + 2] = buf[4 * size_t(i) + 3];
+  }
+  return true;
 }
 
 std::string dump_line(double t_over_p, double diss_gw, int count) {
@@ -183,30 +232,35 @@ static int ab3_explicit_gpu(const Config &c, const MeshOps &M, int precision, co
   std::vector<double> v(M.E, 0.0), dv(size_t(3) * M.E, 0.0), eta(M.N, 0.0), de(size_t(3) * M.N, 0.0);
   if (c.init_type == 1) {
     const std::string base = dir + "/InitialConditions/";
-    if (!load_text(base + "vel_init.txt", M.E, v, dv))
+    if (load_bin(base + "vel_init.bin", M.E, v, dv))
+      log.msg("Loaded FP64 checkpoint " + base + "vel_init.bin");
+    else if (!load_text(base + "vel_init.txt", M.E, v, dv))
       log.error("WARNING: NO INITIAL CONDITION FILE FOUND " + base + "vel_init.txt");
-    if (!load_text(base + "pres_init.txt", M.N, eta, de))
+    if (load_bin(base + "pres_init.bin", M.N, eta, de))
+      log.msg("Loaded FP64 checkpoint " + base + "pres_init.bin");
+    else if (!load_text(base + "pres_init.txt", M.N, eta, de))
       log.error("WARNING: NO INITIAL CONDITION FILE FOUND " + base + "pres_init.txt");
   } else if (c.init_type == 2) {
     // analyticalInitialConditions (src/initialConditions.cpp:146-206)
+    const double R0 = c.radius, Om0 = c.ang_vel;
     double o[6];
     for (int e = 0; e < M.E; e++) {
       const double nx = M.face_normal[2 * e], ny = M.face_normal[2 * e + 1];
-      analytical_obliq_west(c, M.face_lat[e], M.face_lon[e], 0.0, o);
+      analytical_obliq_west(R0, Om0, c.g, c.h, c.obliquity, c.alpha, M.face_lat[e], M.face_lon[e], 0.0, o);
       v[e] = o[0] * nx + o[1] * ny;
       dv[3 * e] = o[2] * nx + o[3] * ny;
-      analytical_obliq_west(c, M.face_lat[e], M.face_lon[e], -dt, o);
+      analytical_obliq_west(R0, Om0, c.g, c.h, c.obliquity, c.alpha, M.face_lat[e], M.face_lon[e], -dt, o);
       dv[3 * e + 1] = o[2] * nx + o[3] * ny;
-      analytical_obliq_west(c, M.face_lat[e], M.face_lon[e], -2 * dt, o);
+      analytical_obliq_west(R0, Om0, c.g, c.h, c.obliquity, c.alpha, M.face_lat[e], M.face_lon[e], -2 * dt, o);
       dv[3 * e + 2] = o[2] * nx + o[3] * ny;
     }
     for (int i = 0; i < M.N; i++) {
-      analytical_obliq_west(c, M.lat[i], M.lon[i], 0.0, o);
+      analytical_obliq_west(R0, Om0, c.g, c.h, c.obliquity, c.alpha, M.lat[i], M.lon[i], 0.0, o);
       eta[i] = o[4];
       de[3 * i] = o[5];
-      analytical_obliq_west(c, M.lat[i], M.lon[i], -dt, o);
+      analytical_obliq_west(R0, Om0, c.g, c.h, c.obliquity, c.alpha, M.lat[i], M.lon[i], -dt, o);
       de[3 * i + 1] = o[5];
-      analytical_obliq_west(c, M.lat[i], M.lon[i], -2 * dt, o);
+      analytical_obliq_west(R0, Om0, c.g, c.h, c.obliquity, c.alpha, M.lat[i], M.lon[i], -2 * dt, o);
       de[3 * i + 2] = o[5];
     }
   }
@@ -308,6 +362,8 @@ static int ab3_explicit_gpu(const Config &c, const MeshOps &M, int precision, co
   ::mkdir((dir + "/InitialConditions").c_str(), 0775);
   write_text(dir + "/InitialConditions/vel_init.txt", v, dv);
   write_text(dir + "/InitialConditions/pres_init.txt", eta, de);
+  write_bin(dir + "/InitialConditions/vel_init.bin", v, dv);
+  write_bin(dir + "/InitialConditions/pres_init.bin", eta, de);
   return status;
 }
 

@@ -355,6 +355,15 @@ int odis_config_read(const char *case_dir, odis_params *params, int *steps_per_o
  * (src/boundaryConditions.cpp:29-77) with gravity g; otherwise 1. */
 int odis_surface_beta(int surface, int l_max, double top_radius, double g, double shell_thickness,
                       double ocean_thickness, double *beta);
+/* analyticalLTE, the live OBLIQ_WEST case (src/analyticalLTE.cpp:47-228,
+ * 133-180): the forced, Rayleigh-damped degree-2 obliquity-west tide of the
+ * ocean p->radius, p->omega, p->g, p->h, p->obliquity (rad), p->alpha;
+ * out[6] = (U, V, dU/dt, dV/dt, eta, deta/dt) at (lat, lon) radians and time
+ * t, with U, V the components whose projection on a face normal (nx, ny)
+ * the reference's analyticalInitialConditions takes as the face velocity
+ * (src/initialConditions.cpp:146-206).  The INIT_ANALYTICAL start of
+ * odis_run_case uses it. */
+int odis_analytical_obliq_west(const odis_params *p, double lat, double lon, double t, double *out);
 /* Last error of a host entry point (grid, mesh, config). */
 const char *odis_host_last_error(void);
 

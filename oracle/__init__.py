@@ -75,6 +75,8 @@ def lib():
                                                 ctypes.c_double, ctypes.c_double, dp]
         L.orc_membrane_nu_beta.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                            ctypes.c_int, dp, dp, dp, dp]
+        L.orc_membrane_nu_beta_gen.argtypes = [ctypes.c_double] * 3 + [ctypes.c_int] + \
+            [ctypes.c_double] * 3 + [dp, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -254,6 +256,16 @@ def membrane_nu_beta(radius, hs, ho, l_max):
     ro, go = ctypes.c_double(), ctypes.c_double()
     lib().orc_membrane_nu_beta(float(radius), float(hs), float(ho), int(l_max), _dp(nu),
                                _dp(beta), ctypes.byref(ro), ctypes.byref(go))
+    return nu, beta, ro.value, go.value
+
+
+def membrane_nu_beta_gen(radius, hs, ho, l_max, mass_total, rig_core, rig_shell):
+    """membraneNuBeta with the body mass and core / shell rigidities free."""
+    nu, beta = np.zeros(l_max + 1), np.zeros(l_max + 1)
+    ro, go = ctypes.c_double(), ctypes.c_double()
+    lib().orc_membrane_nu_beta_gen(float(radius), float(hs), float(ho), int(l_max), float(mass_total),
+                                   float(rig_core), float(rig_shell), _dp(nu), _dp(beta),
+                                   ctypes.byref(ro), ctypes.byref(go))
     return nu, beta, ro.value, go.value
 
 

@@ -1040,13 +1040,17 @@ void orc_analytical_obliq_west(const orc_params *p, double lat, double lon, doub
     o[3] = creal(dV); o[4] = creal(ETA); o[5] = creal(dE);
 }
 
-/* membraneNuBeta, src/membraneConstants.cpp:8-144 (Enceladus constants) */
-void orc_membrane_nu_beta(double radius, double hs, double ho, int l_max, double *nu,
-                          double *beta, double *radius_ocean, double *grav_ocean)
+/* membraneNuBeta, src/membraneConstants.cpp:8-144, with the body's mass and
+ * the core and shell rigidities as arguments (the reference hard-codes
+ * Enceladus: 1.08e20 kg, 40 GPa, 3.5 GPa), so that the tests can drive the
+ * formula to its fluid-core, rigid-core and vanishing-shell limits. */
+void orc_membrane_nu_beta_gen(double radius, double hs, double ho, int l_max, double mass_total,
+                              double rig_core, double rig_shell, double *nu, double *beta,
+                              double *radius_ocean, double *grav_ocean)
 {
-    double G = 6.67384e-11, pois = 0.5, rig_shell = 3.5e9, rig_core = 40e9;
+    double G = 6.67384e-11, pois = 0.5;
     double rc = radius - (hs + ho), ro = radius - hs;
-    double mass_total = 1.08e20, den_ocean = 1000.0, den_shell = 940.0;
+    double den_ocean = 1000.0, den_shell = 940.0;
     double grav_surf = G * mass_total / pow(radius, 2.0);
     double vol_total = 4. / 3. * PI * pow(radius, 3.0);
     double vol_core = 4. / 3. * PI * pow(rc, 3.0);
@@ -1083,6 +1087,15 @@ void orc_membrane_nu_beta(double radius, double hs, double ho, int l_max, double
         beta[l] = 1. - xi * gam_load + spr + dspr;
         nu[l] = gam_tide + dgam;
     }
+}
+
+/* membraneNuBeta with the reference's Enceladus constants
+ * (src/membraneConstants.cpp:8-144). */
+void orc_membrane_nu_beta(double radius, double hs, double ho, int l_max, double *nu,
+                          double *beta, double *radius_ocean, double *grav_ocean)
+{
+    orc_membrane_nu_beta_gen(radius, hs, ho, l_max, 1.08e20, 40e9, 3.5e9, nu, beta, radius_ocean,
+                             grav_ocean);
 }
 
 void orc_set_threads(int n) {

@@ -90,6 +90,10 @@ int   orc_get_threads(void);
 void  orc_membrane_nu_beta(double radius, double shell_thickness, double ocean_thickness,
                            int l_max, double *nu, double *beta,
                            double *radius_ocean, double *grav_ocean);
+/* The same with the body mass and the core / shell rigidities as arguments. */
+void  orc_membrane_nu_beta_gen(double radius, double shell_thickness, double ocean_thickness,
+                               int l_max, double mass_total, double rig_core, double rig_shell,
+                               double *nu, double *beta, double *radius_ocean, double *grav_ocean);
 
 #ifdef __cplusplus
 }

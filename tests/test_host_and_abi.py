@@ -302,6 +302,104 @@ def test_compact_trisk_form_rejects_other_operators(product):
     assert not ok and "face_len" in why
 
 
+def _shim_descriptor(mesh, friends_ref, weights_ref, node_friends, count_rule):
+    """The descriptor the reference-side shim builds (the ab3Explicit body of
+    the integration guide) from reference-shaped Mesh members: the [E][10]
+    face_interp_friends / face_interp_weights, zero-padded as Array2D leaves
+    them (include/array2d.h:33), and node_friends [N][6] with -1 at a
+    pentagon's sixth slot.  count_rule "pentagon" bounds each row as
+    interpolateVelocity does (src/interpolation.cpp:36-46): 10 entries,
+    one fewer per pentagon endpoint; "skip_negative" is the rule that skips
+    -1 entries."""
+    import ctypes
+    E = mesh.E
+    fn = mesh.get("face_nodes").reshape(E, 2)
+    flen, fdist = mesh.get("face_len"), mesh.get("face_node_dist")
+    tf = np.full((E, 10), -1, np.int32)
+    tc = np.zeros((E, 10))
+    cnt = np.zeros(E, np.int32)
+    for e in range(E):
+        if count_rule == "pentagon":
+            k = 10 - int(node_friends[fn[e, 0], 5] < 0) - int(node_friends[fn[e, 1], 5] < 0)
+            idx = np.arange(k)
+        else:
+            idx = np.nonzero(friends_ref[e] >= 0)[0]
+        f = friends_ref[e, idx]
+        tf[e, :len(idx)] = f
+        tc[e, :len(idx)] = weights_ref[e, idx] * flen[f] / fdist[e]
+        cnt[e] = len(idx)
+    d = mesh.describe()
+    keep = (tf, tc, cnt)
+    d.trisk_faces = tf.ctypes.data
+    d.trisk_coef = tc.ctypes.data
+    d.trisk_count = cnt.ctypes.data
+    return d, keep
+
+
+@pytest.mark.parametrize("level", [3, 5])
+def test_reference_shaped_descriptor_keeps_compact_form(product, level):
+    """The shim's descriptor from zero-padded reference arrays: with the
+    pentagon count rule it has the product's own TRiSK stencils (same faces,
+    same coefficients to rounding) and passes the cell-local form check, so
+    a linked reference run gets the compact one-kernel step.  Skipping -1
+    entries instead would keep the zero padding (face 0, weight 0) as a
+    10th entry next to every pentagon, and the check refuses that."""
+    g, arrs = generated_grid(level)
+    mesh = product.Mesh(g, 252.1e3)
+    E = mesh.E
+    fr = mesh.get("interp_friends").reshape(E, 10)
+    w = mesh.get("interp_weights").reshape(E, 10)
+    pad = fr < 0
+    assert pad.sum() == 60  # 60 faces touch a pentagon, one slot each
+    fr_ref = np.where(pad, 0, fr).astype(np.int32)
+    w_ref = np.where(pad, 0.0, w)
+    d, keep = _shim_descriptor(mesh, fr_ref, w_ref, arrs["friends"], "pentagon")
+    tf, tc, cnt = keep
+    assert np.array_equal(tf, fr) and np.array_equal(cnt, mesh.get("interp_count"))
+    want = mesh.get("trisk_coef").reshape(E, 10)
+    assert np.max(np.abs(tc - want)) <= 1e-14 * np.max(np.abs(want))
+    ok, why, ce, de, ae = mesh.compact_check(d)
+    assert ok, why
+    d_bad, _ = _shim_descriptor(mesh, fr_ref, w_ref, arrs["friends"], "skip_negative")
+    ok, why, _, _, _ = mesh.compact_check(d_bad)
+    assert not ok
+
+
+def test_analytical_obliq_west_matches_oracle(product, oracle):
+    """The product's analyticalLTE OBLIQ_WEST (src/analyticalLTE.cpp:133-180),
+    used by the INIT_ANALYTICAL start, against the oracle's restatement at
+    random points and times (complex arithmetic of C++ and C99: 1e-13)."""
+    G = product
+    p, _ = G.make_params(tide=G.OBLIQ_WEST, obliquity=0.1 * np.pi / 180, alpha=3e-7, h=12e3)
+    op = oracle.make_params(g=p.g, h=p.h, alpha=p.alpha, omega=p.omega, radius=p.radius,
+                            obliquity=p.obliquity, dt=p.dt, tide=p.tide)
+    rng = np.random.default_rng(3)
+    for la, lo, t in zip(rng.uniform(-1.5, 1.5, 40), rng.uniform(0, 2 * np.pi, 40), rng.uniform(-1e3, 3e5, 40)):
+        a = G.analytical_obliq_west(p, la, lo, t)
+        b = oracle.analytical_obliq_west(op, la, lo, t)
+        scale = np.array([abs(b[0]) + abs(b[1])] * 2 + [abs(b[2]) + abs(b[3])] * 2 + [abs(b[4]), abs(b[5])])
+        assert np.all(np.abs(a - b) <= 1e-13 * scale + 1e-300), (a, b)
+    bad = G.Params(**p.to_dict())
+    bad.h = 0.0
+    with pytest.raises(G.OdisError):
+        G.analytical_obliq_west(bad, 0.1, 0.2, 0.0)
+
+
+@pytest.mark.parametrize("hs,ho", [(20e3, 38e3), (5e3, 60e3), (1e3, 10e3)])
+def test_surface_beta_lid_membr_matches_oracle(product, oracle, hs, ho):
+    """odis_surface_beta(LID_MEMBR) is membraneNuBeta's beta_l
+    (src/membraneConstants.cpp:87-141) for the top radius and the shell and
+    ocean thicknesses: equal to the oracle's restatement to the last bits
+    (both evaluate the same expressions in the same order, no contraction)."""
+    L = 40
+    b = product.surface_beta(product.LID_MEMBR, L, R_TOP, 0.113, hs, ho)
+    nu, beta, _, _ = oracle.membrane_nu_beta(R_TOP, hs, ho, L)
+    assert np.max(np.abs(b[1:] - beta[1:]) / np.abs(beta[1:])) <= 4e-16
+
+
+R_TOP = 252.1e3
+
+
 def _rbf_check(product, oracle, g, arrs, tol):
     pm = product.Mesh(g, 252.1e3)
     om = oracle.Mesh(arrs, 252.1e3)

@@ -415,6 +415,158 @@ def test_membrane_love_numbers_cross_check(oracle):
     assert 0.8 < nu[2] < 1.1 and abs(fx["upsilon2_hs20km"] - 1.1176) < 1e-4
 
 
+def test_membrane_vanishing_shell_and_ocean_is_free_loading(oracle, product):
+    """With no shell and no ocean (h_s = h_o = 0) membraneNuBeta
+    (src/membraneConstants.cpp:87-141) has no spring terms, its core is the
+    whole body (density = bulk density, gravity = G M / R^2) and its loading
+    Love numbers use the same 40 GPa rigidity as the FREE_LOADING branch of
+    applySurfaceBCs (src/boundaryConditions.cpp:29-77).  The two independent
+    formulas must then give the same beta_l = 1 - 3 rho_w (1 + k_l - h_l) /
+    ((2l + 1) rho_bulk) for every degree; the product's FREE_LOADING factors
+    are pinned separately by their fluid and rigid limits."""
+    R, Gc = R_ENC, 6.67384e-11
+    mass = 1609.22 * 4.0 / 3.0 * np.pi * R ** 3    # the FREE_LOADING bulk density
+    nu, beta, ro, go = oracle.membrane_nu_beta_gen(R, 0.0, 0.0, 40, mass, 40e9, 3.5e9)
+    assert ro == R and abs(go - Gc * mass / R ** 2) < 1e-15 * go
+    fl = product.surface_beta(product.FREE_LOADING, 40, R, Gc * mass / R ** 2, 0.0, 0.0)
+    assert np.max(np.abs(beta[1:] - fl[1:]) / np.abs(fl[1:])) < 1e-14
+
+
+def test_membrane_core_limits(oracle):
+    """membraneNuBeta's Love numbers at the two ends of the core rigidity
+    (src/membraneConstants.cpp:87-130):
+    - fluid core (mu = 0), no shell: the ocean loading is cancelled by the
+      core's response, beta_l = 1 - rho_w / rho_bulk for every l >= 1, and the
+      tidal factor nu_l = 1 + k_l - h_l vanishes for l >= 2;
+    - rigid core (mu -> inf): no core response, nu_l = 1, and beta_l =
+      1 - xi_l + Lambda_l with Lambda_l the shell's spring constant, linear
+      (membrane) plus cubic (bending) in the shell thickness: a cubic fit in
+      h_s has no quadratic term and the constant 1 - 3 rho_w / ((2l+1)
+      rho_bulk); at high degree bending dominates, so doubling h_s multiplies
+      beta_l - (1 - xi_l) by 8 (by 2 where the membrane term dominates)."""
+    R, M, ho = R_ENC, 1.08e20, 38e3
+    rho_bulk = M / (4.0 / 3.0 * np.pi * R ** 3)
+    nu, beta, _, _ = oracle.membrane_nu_beta_gen(R, 0.0, ho, 60, M, 0.0, 3.5e9)
+    assert np.max(np.abs(beta[1:] - (1.0 - 1000.0 / rho_bulk))) < 1e-14
+    assert np.max(np.abs(nu[2:])) < 1e-14
+    L = 400
+    l = np.arange(L + 1)
+    xi = 3.0 / (2 * l + 1) * 1000.0 / rho_bulk
+    hs = np.array([5e3, 10e3, 20e3, 30e3])
+    B = []
+    for h in hs:
+        nu, beta, _, _ = oracle.membrane_nu_beta_gen(R, h, ho, L, M, 1e300, 3.5e9)
+        assert np.all(nu[2:] == 1.0)
+        B.append(beta)
+    B = np.array(B)
+    A = np.stack([np.ones(4), hs, hs ** 2, hs ** 3], 1)
+    for deg in (2, 5, 16, 32, 100):
+        c = np.linalg.solve(A, B[:, deg])
+        assert abs(c[0] - (1.0 - xi[deg])) < 1e-9, (deg, c)
+        assert abs(c[2]) * hs[-1] ** 2 < 1e-9 * abs(B[-1, deg]), (deg, c)
+        assert c[1] > 0 and c[3] > 0
+    ratio = (B[2, 2:] - (1 - xi[2:])) / (B[1, 2:] - (1 - xi[2:]))    # h_s = 20 km over 10 km
+    assert 1.95 < ratio[0] < 2.05 and abs(ratio[-1] - 8.0) < 1e-4
+    assert np.all(np.diff(ratio) > 0)
+
+
+def _sim_for_tide(oracle, product, tide, **kw):
+    m, arrs = _omesh(oracle, 4)
+    op, _ = _params(oracle, product, tide=tide, **kw)
+    if "semimajor" in kw:
+        op.semimajor = kw["semimajor"]
+    return oracle.Sim(m, op), op, arrs
+
+
+def test_full2_reduces_to_ecc_and_obliq(oracle, product):
+    """FULL2 (src/tidalPotentials.cpp:133-156), an expansion in eccentricity
+    and obliquity, against the separately pinned ECC and OBLIQ closed forms:
+    - at zero obliquity it equals ECC to first order in e (relative
+      difference O(e), halving with e);
+    - at zero eccentricity it equals OBLIQ a quarter period later,
+      U_FULL2(t) = U_OBLIQ(t + P/4), to first order in the obliquity."""
+    G = product
+    P = 118394.0
+    for t in (0.0, 1.3e4, 7.7e4):
+        d = []
+        for e in (1e-3, 5e-4):
+            s2, _, _ = _sim_for_tide(oracle, product, G.FULL2, ecc=e, obliquity=0.0)
+            se, _, _ = _sim_for_tide(oracle, product, G.ECC, ecc=e, obliquity=0.0)
+            d.append(rel_l2(s2.forcing(t), se.forcing(t)))
+        assert d[0] < 2e-2 and 1.8 < d[0] / d[1] < 2.2, d
+        d = []
+        for ob in (1e-3, 5e-4):
+            s2, _, _ = _sim_for_tide(oracle, product, G.FULL2, ecc=0.0, obliquity=ob)
+            so, _, _ = _sim_for_tide(oracle, product, G.OBLIQ, ecc=0.0, obliquity=ob)
+            ref = so.forcing(t + P / 4)
+            d.append(np.linalg.norm(s2.forcing(t) - ref) / np.linalg.norm(so.forcing(0.0)))
+        assert d[0] < 2e-3 and 1.8 < d[0] / d[1] < 2.2, d
+
+
+def test_planet_potential_degree_two_and_amplitude(oracle, product):
+    """PLANET (src/tidalPotentials.cpp:176-225): the tide of a companion of
+    mass m at distance d, U = G m R^2 / d^3 P_2(cos gamma), with d from the
+    law of cosines between the two orbits (a1, a2, angle (n1 - n2) t) and
+    gamma the angle to the companion's direction.  Checked: a least-squares
+    fit on the degree <= 2 harmonics leaves no residual and no degree 0 / 1
+    part, and the fitted field is G m R^2 / d^3 towards the companion and
+    -G m R^2 / (2 d^3) perpendicular to it."""
+    G = product
+    a2 = 671100000.0
+    sim, op, arrs = _sim_for_tide(oracle, product, G.PLANET, semimajor=a2)
+    la, lo = arrs["lat"], arrs["lon"]
+
+    def basis(la, lo):
+        x, c = np.sin(la), np.cos(la)
+        return np.stack([np.ones_like(la), x, c * np.cos(lo), c * np.sin(lo),
+                         1.5 * x * x - 0.5, x * c * np.cos(lo), x * c * np.sin(lo),
+                         c * c * np.cos(2 * lo), c * c * np.sin(2 * lo)], -1)
+
+    a1, m2, Gc = 421800000.0, 8.931938e22, 6.67408e-11
+    for t in (0.0, 2.1e4, 9.0e4):
+        U = sim.forcing(t)
+        coef, *_ = np.linalg.lstsq(basis(la, lo), U, rcond=None)
+        assert rel_l2(basis(la, lo) @ coef, U) < 1e-13
+        assert np.max(np.abs(coef[:4])) < 1e-12 * np.max(np.abs(coef))
+        ang = op.omega * t                       # (n1 - n2) t with n1 = 2 n2
+        dx, dy = a1 - a2 * np.cos(ang), a2 * np.sin(ang)
+        d = np.sqrt(a1 ** 2 + a2 ** 2 - 2 * a1 * a2 * np.cos(ang))
+        amp = Gc * m2 * R_ENC ** 2 / d ** 3
+        lon_c = np.arctan2(dy, dx)
+        towards = basis(np.array([0.0]), np.array([lon_c]))[0] @ coef
+        perp = basis(np.array([0.0, np.pi / 2]), np.array([lon_c + np.pi / 2, 0.0])) @ coef
+        assert abs(towards - amp) < 1e-12 * amp
+        assert np.all(np.abs(perp + 0.5 * amp) < 1e-12 * amp)
+
+
+def test_obliq_west_closed_form_continuity(oracle, product):
+    """The OBLIQ_WEST solution (src/analyticalLTE.cpp:133-180) is built to
+    satisfy continuity exactly, d(eta)/dt = -h div(u), with U eastward and V
+    northward (its velocity potential carries the divergence); checked by
+    centred differences on the sphere at random points and times.  Its time
+    derivatives match centred differences in t.  A sign error in either
+    velocity term or in eta breaks the identity at O(1)."""
+    G = product
+    p, _ = G.make_params(tide=G.OBLIQ_WEST, obliquity=0.1 * np.pi / 180, alpha=1e-7)
+    op = oracle.make_params(g=p.g, h=p.h, alpha=p.alpha, omega=p.omega, radius=p.radius,
+                            obliquity=p.obliquity, dt=p.dt, tide=p.tide)
+    f = lambda a, b, t: oracle.analytical_obliq_west(op, a, b, t)
+    R, h, d = p.radius, p.h, 1e-5
+    rng = np.random.default_rng(11)
+    for la, lo, t in zip(rng.uniform(-1.3, 1.3, 8), rng.uniform(0, 2 * np.pi, 8), rng.uniform(0, 2e5, 8)):
+        o = f(la, lo, t)
+        du = (f(la, lo + d, t)[0] - f(la, lo - d, t)[0]) / (2 * d)
+        dv = (f(la + d, lo, t)[1] * np.cos(la + d) - f(la - d, lo, t)[1] * np.cos(la - d)) / (2 * d)
+        div = (du + dv) / (R * np.cos(la))
+        assert abs(o[5] + h * div) < 1e-6 * abs(o[5]), (o[5], -h * div)
+        wrong = (du - dv) / (R * np.cos(la))        # V taken southward
+        assert abs(o[5] + h * wrong) > 1e-2 * abs(o[5])
+        dt = 1.0
+        fp, fm = f(la, lo, t + dt), f(la, lo, t - dt)
+        for k, dk in ((0, 2), (1, 3), (4, 5)):
+            assert abs((fp[k] - fm[k]) / (2 * dt) - o[dk]) < 1e-7 * (abs(o[dk]) + 1e-30)
+
+
 def _h5_dataset(path, name):
     import re
     import subprocess
