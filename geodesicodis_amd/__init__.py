@@ -39,7 +39,7 @@ EXPORTS = [
     "odis_sh_project", "odis_sh_synthesize", "odis_surface_beta", "odis_set_option",
     "odis_get_option", "odis_mesh_compact_check", "odis_get_kernel_times", "odis_rbf_set",
     "odis_velocity_xyz", "odis_h5_open", "odis_h5_write", "odis_h5_close", "odis_h5_last_error",
-    "odis_analytical_obliq_west",
+    "odis_analytical_obliq_west", "odis_sh_project_noalloc",
 ]
 H5_DISPLACEMENT, H5_VELOCITY, H5_VELOCITY_XYZ, H5_DISSIPATION, H5_DISS_AVG = 1, 2, 4, 8, 16
 SH_OFF, SH_FP64, SH_SPLIT_BF16, SH_RECUR, SH_MFMA = 0, 1, 2, 3, 4
@@ -141,6 +141,7 @@ def lib():
             "odis_h5_last_error": ([], ctypes.c_char_p),
             "odis_sh_set_basis": ([vp, ctypes.c_int, dp, ctypes.c_int], ctypes.c_int),
             "odis_sh_project": ([vp, dp, dp], ctypes.c_int),
+            "odis_sh_project_noalloc": ([vp, dp, dp], ctypes.c_int),
             "odis_sh_synthesize": ([vp, dp, dp], ctypes.c_int),
             "odis_surface_beta": ([ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                    ctypes.c_double, ctypes.c_double, dp], ctypes.c_int),
@@ -453,6 +454,14 @@ class Context:
         f = np.ascontiguousarray(field, np.float64)
         c = np.zeros((self.sh_l_max + 1) ** 2)
         self._check(lib().odis_sh_project(self._h, _dp(f), _dp(c)))
+        return c
+
+    def sh_project_noalloc(self, field):
+        """sh_project through odis_sh_project_noalloc (no device allocation;
+        scratch in a static device array shared by every call on the GPU)."""
+        f = np.ascontiguousarray(field, np.float64)
+        c = np.zeros((self.sh_l_max + 1) ** 2)
+        self._check(lib().odis_sh_project_noalloc(self._h, _dp(f), _dp(c)))
         return c
 
     def sh_synthesize(self, coeffs):

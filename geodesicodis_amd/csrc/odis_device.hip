@@ -11,6 +11,7 @@
 #include <cstdint>
 #include <cstring>
 #include <exception>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -1843,6 +1844,7 @@ struct odis_ctx {
   int sh_part = 0;                               // capacity of analysis partials per instance
   int sh_cpt = 0;                                // cells per lane of the on-the-fly analysis (0: auto)
   void *eta_eff = nullptr;  // [B][N] of the stencil precision
+  int *sh_inv = nullptr;    // [N] cell_inv on the device (reference index of device cell r)
   std::vector<void *> sh_allocs;
   std::vector<double> cell_lat, cell_lon;  // reference order
   // lat-lon interpolation operator (eta columns only, device cell indices)
@@ -3766,6 +3768,7 @@ static int sh_set_basis_impl(odis_ctx *ctx, int l_max, const double *beta, int m
   ctx->sh_Cs = nullptr;
   ctx->sh_Kp = ctx->sh_qpr = ctx->sh_rng = 0;
   ctx->eta_eff = nullptr;
+  ctx->sh_inv = nullptr;
   if (mode == ODIS_SH_OFF) return ODIS_OK;
   if (l_max < 0 || l_max > 100) return fail(ctx, ODIS_E_INVALID, "l_max out of range");
   const int K = (l_max + 1) * (l_max + 1), N = ctx->N;
@@ -3957,6 +3960,9 @@ static int sh_set_basis_impl(odis_ctx *ctx, int l_max, const double *beta, int m
   const size_t eff_bytes = size_t((ctx->B + kIlGroup - 1) / kIlGroup * kIlGroup) * N * tsz;
   if ((rc = alloc(&ctx->eta_eff, eff_bytes))) return rc;
   CK(hipMemset(ctx->eta_eff, 0, eff_bytes));  // padding instances read zeros
+  // device-order permutation, for the projection entry point that allocates nothing
+  if ((rc = alloc(reinterpret_cast<void **>(&ctx->sh_inv), size_t(N) * sizeof(int)))) return rc;
+  CK(hipMemcpy(ctx->sh_inv, ctx->cell_inv.data(), size_t(N) * sizeof(int), hipMemcpyHostToDevice));
   ctx->sh_L = l_max;
   ctx->sh_K = K;
   ctx->sh_mode = mode;
@@ -3993,6 +3999,85 @@ static int sh_project_impl(odis_ctx *ctx, const double *field, double *coeffs) {
 
 extern "C" int odis_sh_project(odis_ctx *ctx, const double *field, double *coeffs) {
   return guarded(ctx, [&] { return sh_project_impl(ctx, field, coeffs); });
+}
+
+// ---------------------------------------------------------------------------
+// Projection that allocates nothing.  odis_sh_project allocates the field in
+// device order and the coefficients per call; odis_sh_project_noalloc keeps
+// both in `scratch`, a file-scope device array of kNoAllocScratch doubles
+// (8 MiB per GPU, held until the process ends): [0, N) the field as given,
+// [N, 2N) the field in device order, [2N, 2N + K) the coefficients.  It
+// rejects 2N + K > kNoAllocScratch.  Host code never uses the array's name
+// as an address: each device's address comes from hipGetSymbolAddress, once
+// per device (that device current) at the entry point's first call on it.
+// Every call on a GPU shares that GPU's array, so calls that overlap on one
+// GPU (two host threads, two contexts) race; each call writes every word it
+// reads before reading it, so nothing needs clearing between calls.
+// ---------------------------------------------------------------------------
+static constexpr size_t kNoAllocScratch = size_t(1) << 20;
+static __device__ __align__(256) double scratch[kNoAllocScratch];
+
+static constexpr int kMaxDevices = 64;
+static double *g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// the current device's address of `scratch` (nullptr and an error text if
+// it cannot be had)
+static double *scratch_for_current_device(odis_ctx *ctx) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    fail(ctx, ODIS_E_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    return nullptr;
+  }
+  if (dev < 0 || dev >= kMaxDevices) {
+    fail(ctx, ODIS_E_RANGE, "device index beyond the scratch table");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      fail(ctx, ODIS_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = static_cast<double *>(ptr);
+  }
+  return g_scratch_ptr[dev];
+}
+
+// y[r] = x[inv[r]]: a cell field in reference order to device order
+__global__ __launch_bounds__(kBlock) void k_to_device_order(int n, const int *__restrict__ inv,
+                                                            const double *__restrict__ x, double *__restrict__ y) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r < n) y[r] = x[inv[r]];
+}
+
+static int sh_project_noalloc_impl(odis_ctx *ctx, const double *field, double *coeffs) {
+  if (!ctx || !field || !coeffs) return ODIS_E_INVALID;
+  if (ctx->sh_mode == ODIS_SH_OFF) return fail(ctx, ODIS_E_INVALID, "no spherical-harmonic basis set");
+  const int K = ctx->sh_K, N = ctx->N;
+  if (2 * size_t(N) + size_t(K) > kNoAllocScratch)
+    return fail(ctx, ODIS_E_RANGE,
+                "odis_sh_project_noalloc: 2 N + K = " + std::to_string(2 * size_t(N) + size_t(K)) +
+                    " doubles exceed its scratch of " + std::to_string(kNoAllocScratch));
+  double *ptr = scratch_for_current_device(ctx);
+  if (!ptr) return ODIS_E_HIP;
+  double *x_ref = ptr, *x_dev = ptr + N, *c = ptr + 2 * size_t(N);
+  CK(hipMemcpyAsync(x_ref, field, sizeof(double) * N, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_to_device_order, dim3((N + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, N,
+                     ctx->sh_inv, x_ref, x_dev);
+  CK(hipGetLastError());
+  const int rc = sh_apply<double>(ctx, x_dev, N, 1, nullptr, nullptr, nullptr, c, true, false, nullptr);
+  if (rc) return rc;
+  CK(hipMemcpyAsync(coeffs, c, sizeof(double) * K, hipMemcpyDeviceToHost, ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  return ODIS_OK;
+}
+
+extern "C" int odis_sh_project_noalloc(odis_ctx *ctx, const double *field, double *coeffs) {
+  return guarded(ctx, [&] { return sh_project_noalloc_impl(ctx, field, coeffs); });
 }
 
 static int sh_synthesize_impl(odis_ctx *ctx, const double *coeffs, double *field) {

@@ -220,6 +220,15 @@ int odis_sh_set_basis(odis_ctx *ctx, int l_max, const double *beta, int mode);
  * the synthesis of a field from coefficients, with the context's basis. */
 int odis_sh_project(odis_ctx *ctx, const double *field, double *coeffs);
 int odis_sh_synthesize(odis_ctx *ctx, const double *coeffs, double *field);
+/* odis_sh_project without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as odis_sh_project (bit for bit);
+ * its device scratch (the field twice and the K coefficients) is a
+ * file-scope array of 2^20 doubles (8 MiB per GPU, held until the process
+ * ends), so it returns ODIS_E_RANGE when 2 N + K exceeds 2^20.  Calls that
+ * overlap on one GPU share that array and race. */
+int odis_sh_project_noalloc(odis_ctx *ctx, const double *field, double *coeffs);
 
 /* Async-signal-safe stop request (replaces CatchExit, src/timeIntegrator.cpp:34-37):
  * odis_step returns ODIS_E_STOPPED at the next step boundary. */
