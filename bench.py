@@ -304,6 +304,42 @@ UNEVEN_SEED = 20261204
 UNEVEN_SH_DEGREE = 16
 
 
+def uneven_groups(G, n, seed=UNEVEN_SEED):
+    """The uneven-batch instances grouped by (level, shell): each instance
+    draws (level, obliquity, quadratic-drag coefficient, surface) in that
+    order from numpy.random.default_rng(seed).  Returns ({(level, lid):
+    [Params]}, beta) with beta the degree-16 LID_MEMBR factors of the 20 km
+    shell; dt is each group's quantised step below 0.35 / omega_max."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    draws = {}
+    for _ in range(n):
+        level = int(rng.choice([4, 5, 6, 7]))
+        theta = rng.uniform(0.0, 1.0)
+        cd = rng.uniform(1e-3, 5e-3)
+        lid = bool(rng.uniform() < 0.5)
+        draws.setdefault((level, lid), []).append((theta, cd))
+    L = UNEVEN_SH_DEGREE
+    beta = np.array(G.surface_beta(G.LID_MEMBR, L, 252.1e3, 0.113, 20e3, 38e3))
+    ls = np.arange(L + 1)
+    w_shell = float(np.max(np.sqrt(0.1166 * 38e3 * np.maximum(beta, 1.0) * ls * (ls + 1)) / 232.1e3))
+    groups = {}
+    for (level, lid), inst in draws.items():
+        n_cells = 10 * 4 ** level + 2
+        w = 0.0193 * (n_cells / 10242.0) ** 0.5
+        if lid:
+            w = max(w, w_shell)
+        ps = []
+        for theta, cd in inst:
+            kw = dict(tide=G.FULL, obliquity=np.radians(theta), friction=G.QUADRATIC, alpha=cd, quad_drag=1,
+                      time_step=0.35 / w)
+            if lid:
+                kw.update(radius=232.1e3, g=0.1166, surface=G.LID_MEMBR, shell_thickness=20e3, love_reduct=0.99)
+            ps.append(G.make_params(**kw)[0])
+        groups[(level, lid)] = ps
+    return groups, beta
+
+
 def uneven_run(G, args):
     """Uneven batch: args.uneven_instances instances, each drawing (level,
     obliquity, quadratic-drag coefficient, surface) in that order from
@@ -323,32 +359,12 @@ def uneven_run(G, args):
     alone (the per-group rows), then the 8 contexts' steps are enqueued
     together, one stream each, which gives `value`."""
     import numpy as np
-    rng = np.random.default_rng(UNEVEN_SEED)
-    groups = {}
-    for _ in range(args.uneven_instances):
-        level = int(rng.choice([4, 5, 6, 7]))
-        theta = rng.uniform(0.0, 1.0)
-        cd = rng.uniform(1e-3, 5e-3)
-        lid = bool(rng.uniform() < 0.5)
-        groups.setdefault((level, lid), []).append((theta, cd))
+    groups, beta = uneven_groups(G, args.uneven_instances)
     L = UNEVEN_SH_DEGREE
-    beta = np.array(G.surface_beta(G.LID_MEMBR, L, 252.1e3, 0.113, 20e3, 38e3))
-    ls = np.arange(L + 1)
-    w_shell = float(np.max(np.sqrt(0.1166 * 38e3 * np.maximum(beta, 1.0) * ls * (ls + 1)) / 232.1e3))
     total_cells, elapsed, n_steps, rows, ctxs = 0, 0.0, max(10, args.steps // 40), [], []
-    for (level, lid), inst in sorted(groups.items()):
+    for (level, lid), ps in sorted(groups.items()):
         grid = generated_grid(G, level + 1)
         n_cells = grid.n
-        w = 0.0193 * (n_cells / 10242.0) ** 0.5
-        if lid:
-            w = max(w, w_shell)
-        ps = []
-        for theta, cd in inst:
-            kw = dict(tide=G.FULL, obliquity=np.radians(theta), friction=G.QUADRATIC, alpha=cd, quad_drag=1,
-                      time_step=0.35 / w)
-            if lid:
-                kw.update(radius=232.1e3, g=0.1166, surface=G.LID_MEMBR, shell_thickness=20e3, love_reduct=0.99)
-            ps.append(G.make_params(**kw)[0])
         mesh = G.Mesh(grid, ps[0].radius)
         ctx = G.Context(mesh, ps, precision=G.FP32)
         sh = None

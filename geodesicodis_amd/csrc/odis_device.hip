@@ -542,6 +542,11 @@ __global__ __launch_bounds__(256) void k_step_il(StepArgs<T> a, int energy_quad,
   }
 }
 
+// p[0 .. n) = 0, grid-stride
+__global__ __launch_bounds__(kBlock) void k_zero_words(unsigned int *__restrict__ p, size_t n) {
+  for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += size_t(gridDim.x) * kBlock) p[i] = 0u;
+}
+
 // Per-instance array of n elements between the instance-major layout (std,
 // [B][n]) and the interleaved one (il, sidx<true>): TO_IL copies std into
 // il, else il into std.  Grid (elements / kBlock, B): workgroup y is one
@@ -2311,7 +2316,11 @@ static int create_impl(const odis_mesh_desc *m, const odis_params *params, int n
   }
 
   int rc = ODIS_OK;
-  hipError_t he = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  // A blocking stream: the synchronous hipMemset / hipMemcpy calls of set-up
+  // and state transfer run on the null stream, and only a blocking stream is
+  // ordered after them (a hipMemset may still be running on the device when
+  // it returns).  Contexts' streams still run concurrently with each other.
+  hipError_t he = hipStreamCreateWithFlags(&ctx->stream, hipStreamDefault);
   if (he != hipSuccess) rc = fail(ctx, ODIS_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
   if (!rc) {
     if (precision == ODIS_FP64)
@@ -2943,7 +2952,12 @@ static int il_alloc(odis_ctx *ctx) {
   void *p = nullptr;
   CK(hipMalloc(&p, bytes));
   ctx->allocs.push_back(p);              // freed with the context either way
-  CK(hipMemset(p, 0, bytes));            // padding instances read zeros
+  // padding instances read zeros; cleared on the context stream, so ordered
+  // before the layout conversion that fills the live instances
+  const size_t words = bytes / 4;
+  hipLaunchKernelGGL(k_zero_words, dim3(unsigned(std::min<size_t>((words + kBlock - 1) / kBlock, 65535))),
+                     dim3(kBlock), 0, ctx->stream, static_cast<unsigned int *>(p), words);
+  CK(hipGetLastError());
   ctx->il_buf = p;                       // in use only once set up
   ctx->il_Bp = Bp;
   return ODIS_OK;
