@@ -39,7 +39,7 @@ EXPORTS = [
     "odis_sh_project", "odis_sh_synthesize", "odis_surface_beta", "odis_set_option",
     "odis_get_option", "odis_mesh_compact_check", "odis_get_kernel_times", "odis_rbf_set",
     "odis_velocity_xyz", "odis_h5_open", "odis_h5_write", "odis_h5_close", "odis_h5_last_error",
-    "odis_analytical_obliq_west", "odis_sh_project_noalloc",
+    "odis_analytical_obliq_west", "odis_sh_project_noalloc", "odis_orbit_stats", "odis_retire",
 ]
 H5_DISPLACEMENT, H5_VELOCITY, H5_VELOCITY_XYZ, H5_DISSIPATION, H5_DISS_AVG = 1, 2, 4, 8, 16
 SH_OFF, SH_FP64, SH_SPLIT_BF16, SH_RECUR, SH_MFMA = 0, 1, 2, 3, 4
@@ -104,6 +104,8 @@ def lib():
             "odis_get_dissipation_series": ([vp, ctypes.c_int, ctypes.c_longlong, ctypes.c_int, dp],
                                             ctypes.c_int),
             "odis_series_capacity": ([vp], ctypes.c_int),
+            "odis_orbit_stats": ([vp, ctypes.c_int, ip, ip, dp], ctypes.c_int),
+            "odis_retire": ([vp, ip, ctypes.c_int], ctypes.c_int),
             "odis_get_velocity_en": ([vp, ctypes.c_int, dp, dp], ctypes.c_int),
             "odis_request_stop": ([vp], ctypes.c_int),
             "odis_enable_timing": ([vp, ctypes.c_int], ctypes.c_int),
@@ -399,6 +401,24 @@ class Context:
         self._check(lib().odis_get_dissipation_series(self._h, int(inst), int(first), int(count),
                                                       _dp(out)))
         return out
+
+    def orbit_stats(self, inst, steps_per_orbit):
+        """(mean dissipation, v residual, eta residual) per selected instance
+        whose orbit of steps_per_orbit steps ended at the current step
+        (odis_orbit_stats); an (n, 3) array."""
+        sel = np.ascontiguousarray(inst, np.int32)
+        ns = np.ascontiguousarray(np.broadcast_to(np.asarray(steps_per_orbit, np.int32), sel.shape))
+        out = np.zeros(3 * len(sel))
+        self._check(lib().odis_orbit_stats(self._h, len(sel), _ip(sel), _ip(ns), _dp(out)))
+        return out.reshape(-1, 3)
+
+    def retire(self, keep):
+        """Keep only the instances keep (increasing), renumbered 0.. in that
+        order (odis_retire)."""
+        k = np.ascontiguousarray(keep, np.int32)
+        self._check(lib().odis_retire(self._h, _ip(k), len(k)))
+        self.params = [self.params[i] for i in k]
+        self.B = len(k)
 
     def velocity_en(self, inst=0):
         normal = np.ascontiguousarray(self.mesh.get("face_normal"))

@@ -1809,6 +1809,12 @@ struct odis_ctx {
   std::vector<int> gi_host, cyc_host;  // device-order face cells and cell cycles, to rebuild patches
   long long step = 0;          // next step index n, shared by all instances
   long long series_lo = 0;     // first step of the series ring written since the last set_state
+  // orbit statistics: FP64 state snapshot [B][E + N] per instance at its last
+  // orbit end, whether it holds one, and the small argument / result buffers
+  double *snap = nullptr;
+  std::vector<int> snap_valid;
+  int *orb_sel = nullptr, *orb_ns = nullptr;
+  double *orb_out = nullptr;
   double *partials = nullptr;  // [2][B][part_cap], by step parity
   double *series = nullptr;    // [B][cap]
   int *nonfinite = nullptr;    // [B]
@@ -3379,6 +3385,176 @@ static int get_dissipation_series_impl(odis_ctx *ctx, int inst, long long first,
 
 extern "C" int odis_get_dissipation_series(odis_ctx *ctx, int inst, long long first, int count, double *out) {
   return guarded(ctx, [&] { return get_dissipation_series_impl(ctx, inst, first, count, out); });
+}
+
+// ---------------------------------------------------------------------------
+// orbit statistics and retirement (batched runs to converged dissipation)
+// ---------------------------------------------------------------------------
+
+// Workgroup j: instance sel[j] whose orbit of ns[j] steps ended at `step`:
+// out[3j] = mean of its dissipation series over the orbit, out[3j + 1],
+// out[3j + 2] = ||v - v_snap|| / ||v||, ||eta - eta_snap|| / ||eta|| in FP64
+// against its snapshot, which then takes the current state.  Fixed-order
+// sums (strided per lane, then the wave tree): deterministic.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_orbit_stats(const int *__restrict__ sel, const int *__restrict__ ns,
+                                                        long long step, int cap, const double *__restrict__ series,
+                                                        const T *__restrict__ v, const T *__restrict__ eta,
+                                                        long long E, long long N, double *__restrict__ snap,
+                                                        double *__restrict__ out) {
+  __shared__ double sh[kBlock / 64];
+  const int j = blockIdx.x, b = sel[j], m = ns[j];
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < m; k += kBlock) acc += series[size_t(b) * cap + size_t((step - m + k) % cap)];
+  const double tot = block_sum<kBlock / 64>(acc, sh);
+  double dv = 0.0, nv = 0.0, de = 0.0, ne = 0.0;
+  double *sp = snap + size_t(b) * size_t(E + N);
+  const T *vb = v + size_t(b) * E, *eb = eta + size_t(b) * N;
+  for (long long i = threadIdx.x; i < E; i += kBlock) {
+    const double x = double(vb[i]), d = x - sp[i];
+    dv += d * d;
+    nv += x * x;
+    sp[i] = x;
+  }
+  for (long long i = threadIdx.x; i < N; i += kBlock) {
+    const double x = double(eb[i]), d = x - sp[E + i];
+    de += d * d;
+    ne += x * x;
+    sp[E + i] = x;
+  }
+  __syncthreads();
+  const double sdv = block_sum<kBlock / 64>(dv, sh);
+  __syncthreads();
+  const double snv = block_sum<kBlock / 64>(nv, sh);
+  __syncthreads();
+  const double sde = block_sum<kBlock / 64>(de, sh);
+  __syncthreads();
+  const double sne = block_sum<kBlock / 64>(ne, sh);
+  if (threadIdx.x == 0) {
+    out[3 * j] = tot / double(m);
+    out[3 * j + 1] = snv > 0.0 ? sqrt(sdv / snv) : (sdv > 0.0 ? HUGE_VAL : 0.0);
+    out[3 * j + 2] = sne > 0.0 ? sqrt(sde / sne) : (sde > 0.0 ? HUGE_VAL : 0.0);
+  }
+}
+
+template <typename T>
+static int orbit_stats_t(odis_ctx *ctx, Ops<T> &o, int n, const int *inst, const int *ns, double *out) {
+  const int B = ctx->B;
+  const long long E = ctx->E, N = ctx->N;
+  int rc = il_out(ctx, o);
+  if (rc) return rc;
+  if (!ctx->snap) {
+    if ((rc = dev_alloc(ctx, &ctx->snap, size_t(B) * size_t(E + N)))) return rc;
+    ctx->snap_valid.assign(B, 0);
+  }
+  if (!ctx->orb_sel) {  // sized for the batch at the first call; retirement only shrinks it
+    if ((rc = dev_alloc(ctx, &ctx->orb_sel, size_t(B))) || (rc = dev_alloc(ctx, &ctx->orb_ns, size_t(B))) ||
+        (rc = dev_alloc(ctx, &ctx->orb_out, size_t(3) * B)))
+      return rc;
+  }
+  CK(hipMemcpyAsync(ctx->orb_sel, inst, sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+  CK(hipMemcpyAsync(ctx->orb_ns, ns, sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL((k_orbit_stats<T>), dim3(n), dim3(kBlock), 0, ctx->stream, ctx->orb_sel, ctx->orb_ns,
+                     ctx->step, ctx->cap, ctx->series, o.v[ctx->cur], o.eta[ctx->cur], E, N, ctx->snap,
+                     ctx->orb_out);
+  CK(hipGetLastError());
+  CK(hipMemcpyAsync(out, ctx->orb_out, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  for (int j = 0; j < n; j++) {
+    if (!ctx->snap_valid[inst[j]]) out[3 * j + 1] = out[3 * j + 2] = std::nan("");
+    ctx->snap_valid[inst[j]] = 1;
+  }
+  return ODIS_OK;
+}
+
+static int orbit_stats_impl(odis_ctx *ctx, int n, const int *inst, const int *ns, double *out) {
+  if (!ctx || n < 0 || (n > 0 && (!inst || !ns || !out))) return ODIS_E_INVALID;
+  if (n > ctx->B) return fail(ctx, ODIS_E_RANGE, "more instances than the batch holds");
+  std::vector<char> seen(ctx->B, 0);
+  for (int j = 0; j < n; j++) {
+    if (inst[j] < 0 || inst[j] >= ctx->B || seen[inst[j]]) return fail(ctx, ODIS_E_RANGE, "bad or repeated instance");
+    seen[inst[j]] = 1;
+    if (ns[j] < 1 || ns[j] > ctx->cap || ctx->step - ns[j] < ctx->series_lo)
+      return fail(ctx, ODIS_E_RANGE, "orbit longer than the series held since the last set_state");
+  }
+  if (n == 0) return ODIS_OK;
+  return ctx->precision == ODIS_FP64 ? orbit_stats_t(ctx, ctx->d, n, inst, ns, out)
+                                     : orbit_stats_t(ctx, ctx->f, n, inst, ns, out);
+}
+
+extern "C" int odis_orbit_stats(odis_ctx *ctx, int n, const int *inst, const int *steps_per_orbit, double *out) {
+  return guarded(ctx, [&] { return orbit_stats_impl(ctx, n, inst, steps_per_orbit, out); });
+}
+
+// Move instance keep[j] to slot j of a per-instance device array of `per`
+// elements per instance (keep strictly increasing, so a source slot is never
+// overwritten before it is read).
+template <typename U>
+static int compact_slots(odis_ctx *ctx, U *base, size_t per, const int *keep, int n_keep) {
+  if (!base) return ODIS_OK;
+  for (int j = 0; j < n_keep; j++)
+    if (keep[j] != j)
+      CK(hipMemcpyAsync(base + size_t(j) * per, base + size_t(keep[j]) * per, per * sizeof(U),
+                        hipMemcpyDeviceToDevice, ctx->stream));
+  return ODIS_OK;
+}
+
+template <typename T>
+static int retire_t(odis_ctx *ctx, Ops<T> &o, const int *keep, int n_keep) {
+  const size_t E = ctx->E, N = ctx->N;
+  int rc = il_out(ctx, o);  // the instance-major arrays hold the state
+  if (rc) return rc;
+  for (int k = 0; k < 2 && !rc; k++) {
+    if (!rc) rc = compact_slots(ctx, o.v[k], E, keep, n_keep);
+    if (!rc) rc = compact_slots(ctx, o.eta[k], N, keep, n_keep);
+    if (!rc) rc = compact_slots(ctx, o.ts[k], 2 * E, keep, n_keep);
+  }
+  if (!rc) rc = compact_slots(ctx, o.ring_v, 3 * E, keep, n_keep);
+  if (!rc) rc = compact_slots(ctx, o.ring_e, 3 * N, keep, n_keep);
+  if (!rc) rc = compact_slots(ctx, ctx->inst, size_t(8), keep, n_keep);
+  if (!rc) rc = compact_slots(ctx, ctx->dconst, size_t(1), keep, n_keep);
+  if (!rc) rc = compact_slots(ctx, ctx->series, size_t(ctx->cap), keep, n_keep);
+  if (!rc) rc = compact_slots(ctx, ctx->nonfinite, size_t(1), keep, n_keep);
+  if (!rc) rc = compact_slots(ctx, ctx->snap, E + N, keep, n_keep);
+  if (!rc && ctx->sh_mode != ODIS_SH_OFF) rc = compact_slots(ctx, ctx->sh_D, size_t(ctx->sh_K), keep, n_keep);
+  if (rc) return rc;
+  auto pick = [&](auto &vec) {
+    for (int j = 0; j < n_keep; j++) vec[j] = vec[keep[j]];
+    vec.resize(n_keep);
+  };
+  pick(ctx->params);
+  pick(ctx->tides);
+  pick(ctx->init_hist_v);
+  pick(ctx->init_hist_e);
+  if (!ctx->snap_valid.empty()) pick(ctx->snap_valid);
+  ctx->B = n_keep;
+  drop_graphs(ctx);
+  if (ctx->il_buf) {  // re-laid out from the compacted arrays at the next interleaved step
+    ctx->il_Bp = (n_keep + kInterleaveGroup - 1) / kInterleaveGroup * kInterleaveGroup;
+    const size_t words = (size_t(9) * E + size_t(5) * N) * size_t(ctx->il_Bp) * sizeof(T) / 4;
+    hipLaunchKernelGGL(k_zero_words, dim3(unsigned(std::min<size_t>((words + kBlock - 1) / kBlock, 65535))),
+                       dim3(kBlock), 0, ctx->stream, static_cast<unsigned int *>(ctx->il_buf), words);
+    CK(hipGetLastError());
+  }
+  ctx->il_fresh = 0;
+  ctx->std_fresh = 1;
+  CK(hipStreamSynchronize(ctx->stream));
+  return ODIS_OK;
+}
+
+static int retire_impl(odis_ctx *ctx, const int *keep, int n_keep) {
+  if (!ctx || (n_keep > 0 && !keep)) return ODIS_E_INVALID;
+  if (n_keep < 1 || n_keep > ctx->B) return fail(ctx, ODIS_E_RANGE, "n_keep outside [1, B]");
+  for (int j = 0; j < n_keep; j++)
+    if (keep[j] < 0 || keep[j] >= ctx->B || (j > 0 && keep[j] <= keep[j - 1]))
+      return fail(ctx, ODIS_E_INVALID, "keep must be strictly increasing instance indices");
+  if (n_keep == ctx->B) return ODIS_OK;
+  CK(hipStreamSynchronize(ctx->stream));
+  return ctx->precision == ODIS_FP64 ? retire_t(ctx, ctx->d, keep, n_keep) : retire_t(ctx, ctx->f, keep, n_keep);
+}
+
+extern "C" int odis_retire(odis_ctx *ctx, const int *keep, int n_keep) {
+  return guarded(ctx, [&] { return retire_impl(ctx, keep, n_keep); });
 }
 
 template <typename T>

@@ -142,6 +142,23 @@ int odis_get_dissipation(odis_ctx *ctx, double *per_inst);
 int odis_get_dissipation_series(odis_ctx *ctx, int inst, long long first, int count,
                                 double *out);
 int odis_series_capacity(const odis_ctx *ctx);
+/* Orbit statistics for a run to converged dissipation.  The reference logs
+ * the dissipation at its output cadence only (src/timeIntegrator.cpp:280-304,
+ * src/energy.cpp:13-62); this is the per-orbit form of the same quantity.
+ * For the n instances inst[j] whose orbit of steps_per_orbit[j] steps ended
+ * at the current step: out[3j] = mean of the per-step FP64 dissipation over
+ * those steps (W/m^2), out[3j+1] and out[3j+2] = the periodicity residuals
+ * ||v - v'|| / ||v|| and ||eta - eta'|| / ||eta|| (FP64) against the state
+ * v', eta' the previous call stored for that instance (NaN at its first
+ * call); the call then stores the current state.  Orbits must lie within the
+ * series held since the last odis_set_state (ODIS_E_RANGE otherwise). */
+int odis_orbit_stats(odis_ctx *ctx, int n, const int *inst, const int *steps_per_orbit, double *out);
+/* Retire instances from a batch: keep[0..n_keep) (strictly increasing)
+ * become instances 0..n_keep-1, in that order, with their state, history,
+ * parameters, dissipation series, orbit snapshots and shell factors; the
+ * others are dropped and later steps cost only the kept instances.  No
+ * counterpart in the reference (one instance per run). */
+int odis_retire(odis_ctx *ctx, const int *keep, int n_keep);
 /* Reconstructed (east, north) velocity per face, interpolateVelocity
  * (src/interpolation.cpp:26-62); uv[E][2]. */
 int odis_get_velocity_en(odis_ctx *ctx, int inst, const double *face_normal, double *uv);
