@@ -40,6 +40,9 @@ EXPORTS = [
     "odis_get_option", "odis_mesh_compact_check", "odis_get_kernel_times", "odis_rbf_set",
     "odis_velocity_xyz", "odis_h5_open", "odis_h5_write", "odis_h5_close", "odis_h5_last_error",
     "odis_analytical_obliq_west", "odis_sh_project_noalloc", "odis_orbit_stats", "odis_retire",
+    "odis_sh_basis_eval", "odis_mlp_create", "odis_mlp_destroy", "odis_mlp_param_count",
+    "odis_mlp_get_params", "odis_mlp_set_params", "odis_mlp_forward", "odis_mlp_set_data",
+    "odis_mlp_loss_grad", "odis_mlp_train", "odis_mlp_last_error",
 ]
 H5_DISPLACEMENT, H5_VELOCITY, H5_VELOCITY_XYZ, H5_DISSIPATION, H5_DISS_AVG = 1, 2, 4, 8, 16
 SH_OFF, SH_FP64, SH_SPLIT_BF16, SH_RECUR, SH_MFMA = 0, 1, 2, 3, 4
@@ -89,6 +92,7 @@ def lib():
                               "(make -C geodesicodis_amd)" % LIB_PATH)
         L = ctypes.CDLL(LIB_PATH)
         vp, ip, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
+        fp = ctypes.POINTER(ctypes.c_float)
         llp = ctypes.POINTER(ctypes.c_longlong)
         vpp = ctypes.POINTER(ctypes.c_void_p)
         sig = {
@@ -147,6 +151,18 @@ def lib():
             "odis_sh_synthesize": ([vp, dp, dp], ctypes.c_int),
             "odis_surface_beta": ([ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                    ctypes.c_double, ctypes.c_double, dp], ctypes.c_int),
+            "odis_sh_basis_eval": ([ctypes.c_int, ctypes.c_int, dp, dp, dp], ctypes.c_int),
+            "odis_mlp_create": ([ctypes.c_int] * 4 + [ctypes.c_ulonglong, vpp], ctypes.c_int),
+            "odis_mlp_destroy": ([vp], ctypes.c_int),
+            "odis_mlp_param_count": ([vp], ctypes.c_longlong),
+            "odis_mlp_get_params": ([vp, fp], ctypes.c_int),
+            "odis_mlp_set_params": ([vp, fp], ctypes.c_int),
+            "odis_mlp_forward": ([vp, ctypes.c_int, fp, fp], ctypes.c_int),
+            "odis_mlp_set_data": ([vp, ctypes.c_int, fp, fp], ctypes.c_int),
+            "odis_mlp_loss_grad": ([vp, ctypes.c_int, ip, dp, fp], ctypes.c_int),
+            "odis_mlp_train": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                ctypes.c_ulonglong, dp], ctypes.c_int),
+            "odis_mlp_last_error": ([], ctypes.c_char_p),
             "odis_analytical_obliq_west": ([ctypes.POINTER(Params), ctypes.c_double, ctypes.c_double,
                                             ctypes.c_double, dp], ctypes.c_int),
         }
@@ -536,6 +552,72 @@ class Context:
         self.close()
 
 
+def _fp(a):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class Mlp:
+    """The warm start's network on the GPU (odis_mlp_*): n_in -> width x
+    depth (tanh) -> n_out, FP32, trained by the library's kernels."""
+
+    def __init__(self, n_in, width, depth, n_out, seed=0):
+        h = ctypes.c_void_p()
+        self._mcheck(lib().odis_mlp_create(int(n_in), int(width), int(depth), int(n_out), int(seed),
+                                           ctypes.byref(h)))
+        self._h = h
+        self.shape = (int(n_in), int(width), int(depth), int(n_out))
+        self.n_params = int(lib().odis_mlp_param_count(h))
+
+    @staticmethod
+    def _mcheck(rc):
+        if rc:
+            raise OdisError(rc, lib().odis_mlp_last_error().decode())
+
+    def params(self):
+        p = np.zeros(self.n_params, np.float32)
+        self._mcheck(lib().odis_mlp_get_params(self._h, _fp(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        if p.size != self.n_params:
+            raise ValueError("expected %d parameters" % self.n_params)
+        self._mcheck(lib().odis_mlp_set_params(self._h, _fp(p)))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros((len(x), self.shape[3]), np.float32)
+        self._mcheck(lib().odis_mlp_forward(self._h, len(x), _fp(x), _fp(y)))
+        return y
+
+    def set_data(self, x, y):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        self._mcheck(lib().odis_mlp_set_data(self._h, len(x), _fp(x), _fp(y)))
+        self.n_data = len(x)
+
+    def loss_grad(self, idx=None):
+        idx = np.arange(self.n_data, dtype=np.int32) if idx is None else np.ascontiguousarray(idx, np.int32)
+        loss = ctypes.c_double()
+        g = np.zeros(self.n_params, np.float32)
+        self._mcheck(lib().odis_mlp_loss_grad(self._h, len(idx), _ip(idx), ctypes.byref(loss), _fp(g)))
+        return loss.value, g
+
+    def train(self, epochs, batch, lr_start=1e-3, lr_end=1e-5, seed=0):
+        out = np.zeros(int(epochs))
+        self._mcheck(lib().odis_mlp_train(self._h, int(epochs), int(batch), float(lr_start), float(lr_end),
+                                          int(seed), _dp(out)))
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().odis_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
 def surface_beta(surface, l_max, top_radius, g, shell_thickness, ocean_thickness):
     """Per-degree pressure factors beta_l of a surface type (membraneNuBeta for
     LID_MEMBR, 1 - loading factor for FREE_LOADING, else 1)."""
@@ -551,6 +633,16 @@ def analytical_obliq_west(params, lat, lon, t):
     o = np.zeros(6)
     _host_check(lib().odis_analytical_obliq_west(ctypes.byref(params), float(lat), float(lon), float(t), _dp(o)))
     return o
+
+
+def sh_basis(l_max, lat, lon):
+    """Real 4pi-normalised harmonics Y[i, l*l + l + m] at the points (lat,
+    lon) radians (odis_sh_basis_eval): the basis of the SH coupling."""
+    la = np.ascontiguousarray(lat, np.float64)
+    lo = np.ascontiguousarray(lon, np.float64)
+    Y = np.zeros((len(la), (l_max + 1) ** 2))
+    _host_check(lib().odis_sh_basis_eval(int(l_max), len(la), _dp(la), _dp(lo), _dp(Y)))
+    return Y
 
 
 def run_case(case_dir, grid_dir=None, precision=FP64):

@@ -281,3 +281,16 @@ extern "C" int odis_analytical_obliq_west(const odis_params *p, double lat, doub
   odis::analytical_obliq_west(p->radius, p->omega, p->g, p->h, p->obliquity, p->alpha, lat, lon, t, out);
   return ODIS_OK;
 }
+
+extern "C" int odis_sh_basis_eval(int l_max, int n, const double *lat, const double *lon, double *Y) {
+  if (n < 0 || (n > 0 && (!lat || !lon || !Y)) || l_max < 0 || l_max > 200)
+    return host_fail(ODIS_E_INVALID, "bad argument");
+  try {
+    const size_t K = size_t(l_max + 1) * size_t(l_max + 1);
+    std::vector<double> a, b, sect, f1;
+    for (int i = 0; i < n; i++) odis::sh_basis_row(l_max, lat[i], lon[i], Y + size_t(i) * K);
+    return ODIS_OK;
+  } catch (const std::exception &e) {
+    return host_fail(ODIS_E_INVALID, e.what());
+  }
+}

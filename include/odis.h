@@ -325,6 +325,36 @@ int odis_get_timing(odis_ctx *ctx, double *face_ms, double *cell_ms, double *ste
 #define ODIS_T_COUNT       6
 int odis_get_kernel_times(odis_ctx *ctx, double *ms, int n);
 
+/* ---- learned warm start: the network (no counterpart in the reference) --- */
+
+/* A fully connected network n_in -> width x depth (tanh) -> n_out (linear)
+ * on the GPU, FP32 weights and products (matrix cores), for the warm-start
+ * prediction of periodic states.  Parameters: per layer l, W_l [out][in]
+ * row-major then b_l [out]; Glorot-uniform weights and zero biases from
+ * seed at creation.  Loss: sum (y - t)^2 / (2 n n_out) over the rows. */
+typedef struct odis_mlp odis_mlp;
+int odis_mlp_create(int n_in, int width, int depth, int n_out, unsigned long long seed, odis_mlp **out);
+int odis_mlp_destroy(odis_mlp *m);
+long long odis_mlp_param_count(const odis_mlp *m);
+int odis_mlp_get_params(odis_mlp *m, float *params);
+/* also resets the Adam moments */
+int odis_mlp_set_params(odis_mlp *m, const float *params);
+/* y[n][n_out] = network(x[n][n_in]) */
+int odis_mlp_forward(odis_mlp *m, int n, const float *x, float *y);
+/* the training set, copied to the device: x[n][n_in], y[n][n_out] */
+int odis_mlp_set_data(odis_mlp *m, int n, const float *x, const float *y);
+/* loss and gradient (param_count floats, nullable) over rows idx[0..n) of the
+ * training set; the parameters do not change */
+int odis_mlp_loss_grad(odis_mlp *m, int n, const int *idx, double *loss, float *grad);
+/* Adam (beta 0.9, 0.999, eps 1e-8) over `epochs` passes of the training set
+ * in shuffled minibatches of `batch` rows (order from seed), learning rate
+ * from lr_start to lr_end by cosine decay; epoch_loss (nullable, epochs
+ * entries) receives each epoch's mean minibatch loss.  Deterministic for a
+ * given seed. */
+int odis_mlp_train(odis_mlp *m, int epochs, int batch, float lr_start, float lr_end, unsigned long long seed,
+                   double *epoch_loss);
+const char *odis_mlp_last_error(void);
+
 /* ---- host: grid and mesh (src/mesh.cpp:32-197, 436-1154, 2808-3287) ---- */
 
 /* Check of the cell-local TRiSK form on the host (odis_create runs the same
@@ -390,6 +420,12 @@ int odis_surface_beta(int surface, int l_max, double top_radius, double g, doubl
  * (src/initialConditions.cpp:146-206).  The INIT_ANALYTICAL start of
  * odis_run_case uses it. */
 int odis_analytical_obliq_west(const odis_params *p, double lat, double lon, double t, double *out);
+/* Real 4pi-normalised spherical harmonics to degree l_max at n points
+ * (radians), the basis of the SH coupling: Y[i][l*l + l + m], m >= 0 the
+ * cos(m lon) and m < 0 the sin(|m| lon) harmonic, no Condon-Shortley phase.
+ * Host-only (the reference's commented CalcLegendreFuncs, src/mesh.cpp:
+ * 2154-2259, tabulated the same functions). */
+int odis_sh_basis_eval(int l_max, int n, const double *lat, const double *lon, double *Y);
 /* Last error of a host entry point (grid, mesh, config). */
 const char *odis_host_last_error(void);
 

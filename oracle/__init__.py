@@ -75,6 +75,9 @@ def lib():
                                                 ctypes.c_double, ctypes.c_double, dp]
         L.orc_membrane_nu_beta.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                            ctypes.c_int, dp, dp, dp, dp]
+        L.orc_mlp_forward.argtypes = [ctypes.c_int] * 4 + [dp, ctypes.c_int, dp, dp]
+        L.orc_mlp_loss_grad.argtypes = [ctypes.c_int] * 4 + [dp, ctypes.c_int, dp, dp, dp]
+        L.orc_mlp_loss_grad.restype = ctypes.c_double
         L.orc_membrane_nu_beta_gen.argtypes = [ctypes.c_double] * 3 + [ctypes.c_int] + \
             [ctypes.c_double] * 3 + [dp, dp, dp, dp]
         _lib = L
@@ -267,6 +270,28 @@ def membrane_nu_beta_gen(radius, hs, ho, l_max, mass_total, rig_core, rig_shell)
                                    float(rig_core), float(rig_shell), _dp(nu), _dp(beta),
                                    ctypes.byref(ro), ctypes.byref(go))
     return nu, beta, ro.value, go.value
+
+
+def mlp_forward(shape, params, x):
+    """FP64 forward pass of the warm-start network shape (n_in, width, depth,
+    n_out) with flat parameters (per layer W [out][in], then b)."""
+    n_in, width, depth, n_out = shape
+    x = np.ascontiguousarray(x, np.float64)
+    p = np.ascontiguousarray(params, np.float64)
+    y = np.zeros((len(x), n_out))
+    lib().orc_mlp_forward(n_in, width, depth, n_out, _dp(p), len(x), _dp(x), _dp(y))
+    return y
+
+
+def mlp_loss_grad(shape, params, x, t):
+    """FP64 loss sum (y - t)^2 / (2 n n_out) and its gradient."""
+    n_in, width, depth, n_out = shape
+    x = np.ascontiguousarray(x, np.float64)
+    t = np.ascontiguousarray(t, np.float64)
+    p = np.ascontiguousarray(params, np.float64)
+    g = np.zeros_like(p)
+    loss = lib().orc_mlp_loss_grad(n_in, width, depth, n_out, _dp(p), len(x), _dp(x), _dp(t), _dp(g))
+    return loss, g
 
 
 def read_reference_grid(path):

@@ -1103,3 +1103,113 @@ void orc_set_threads(int n) {
 }
 
 int orc_get_threads(void) { return omp_get_max_threads(); }
+
+/* ------------------------------------------------------------------------ */
+/* FP64 restatement of the warm-start network (no reference counterpart):   */
+/* n_in -> width x depth (tanh) -> n_out (linear); parameters per layer l:  */
+/* W_l [out][in] row-major, then b_l [out]; loss = sum (y - t)^2 / (2 n     */
+/* n_out).  The checker of the GPU network's forward pass and gradients.    */
+/* ------------------------------------------------------------------------ */
+
+static void mlp_dims(int n_in, int width, int depth, int n_out, int l, int *ni, int *no)
+{
+    *ni = l == 0 ? n_in : width;
+    *no = l == depth ? n_out : width;
+}
+
+/* activations of n rows: a[0] = x, a[l + 1] = layer l output; returns malloc'd block */
+static double **mlp_run(int n_in, int width, int depth, int n_out, const double *p, int n, const double *x)
+{
+    double **a = calloc((size_t)depth + 2, sizeof(double *));
+    a[0] = malloc(sizeof(double) * (size_t)n * n_in);
+    memcpy(a[0], x, sizeof(double) * (size_t)n * n_in);
+    size_t off = 0;
+    for (int l = 0; l <= depth; l++) {
+        int ni, no;
+        mlp_dims(n_in, width, depth, n_out, l, &ni, &no);
+        const double *W = p + off, *b = p + off + (size_t)ni * no;
+        off += (size_t)ni * no + no;
+        a[l + 1] = malloc(sizeof(double) * (size_t)n * no);
+        for (int r = 0; r < n; r++)
+            for (int j = 0; j < no; j++) {
+                double s = 0.0;
+                for (int k = 0; k < ni; k++) s += a[l][(size_t)r * ni + k] * W[(size_t)j * ni + k];
+                s += b[j];
+                a[l + 1][(size_t)r * no + j] = l < depth ? tanh(s) : s;
+            }
+    }
+    return a;
+}
+
+static void mlp_free(double **a, int depth)
+{
+    for (int l = 0; l <= depth + 1; l++) free(a[l]);
+    free(a);
+}
+
+void orc_mlp_forward(int n_in, int width, int depth, int n_out, const double *p, int n, const double *x,
+                     double *y)
+{
+    double **a = mlp_run(n_in, width, depth, n_out, p, n, x);
+    memcpy(y, a[depth + 1], sizeof(double) * (size_t)n * n_out);
+    mlp_free(a, depth);
+}
+
+double orc_mlp_loss_grad(int n_in, int width, int depth, int n_out, const double *p, int n, const double *x,
+                         const double *t, double *grad)
+{
+    double **a = mlp_run(n_in, width, depth, n_out, p, n, x);
+    const double cnt = (double)n * n_out;
+    double loss = 0.0;
+    int wmax = width > n_out ? width : n_out;
+    if (n_in > wmax) wmax = n_in;
+    double *dz = malloc(sizeof(double) * (size_t)n * wmax), *dx = malloc(sizeof(double) * (size_t)n * wmax);
+    for (size_t i = 0; i < (size_t)n * n_out; i++) {
+        double d = a[depth + 1][i] - t[i];
+        loss += d * d;
+        dz[i] = d / cnt;
+    }
+    loss /= 2.0 * cnt;
+    size_t *offs = malloc(sizeof(size_t) * ((size_t)depth + 1));
+    size_t off = 0;
+    for (int l = 0; l <= depth; l++) {
+        int ni, no;
+        mlp_dims(n_in, width, depth, n_out, l, &ni, &no);
+        offs[l] = off;
+        off += (size_t)ni * no + no;
+    }
+    for (int l = depth; l >= 0; l--) {
+        int ni, no;
+        mlp_dims(n_in, width, depth, n_out, l, &ni, &no);
+        if (l < depth)  /* through tanh: dz *= 1 - a^2 */
+            for (size_t i = 0; i < (size_t)n * no; i++) dz[i] *= 1.0 - a[l + 1][i] * a[l + 1][i];
+        double *gW = grad + offs[l], *gb = grad + offs[l] + (size_t)ni * no;
+        const double *W = p + offs[l];
+        for (int j = 0; j < no; j++) {
+            double sb = 0.0;
+            for (int r = 0; r < n; r++) sb += dz[(size_t)r * no + j];
+            gb[j] = sb;
+            for (int k = 0; k < ni; k++) {
+                double s = 0.0;
+                for (int r = 0; r < n; r++) s += dz[(size_t)r * no + j] * a[l][(size_t)r * ni + k];
+                gW[(size_t)j * ni + k] = s;
+            }
+        }
+        if (l > 0) {
+            for (int r = 0; r < n; r++)
+                for (int k = 0; k < ni; k++) {
+                    double s = 0.0;
+                    for (int j = 0; j < no; j++) s += dz[(size_t)r * no + j] * W[(size_t)j * ni + k];
+                    dx[(size_t)r * ni + k] = s;
+                }
+            double *tmp = dz;
+            dz = dx;
+            dx = tmp;
+        }
+    }
+    free(offs);
+    free(dz);
+    free(dx);
+    mlp_free(a, depth);
+    return loss;
+}

@@ -95,6 +95,12 @@ void  orc_membrane_nu_beta_gen(double radius, double shell_thickness, double oce
                                int l_max, double mass_total, double rig_core, double rig_shell,
                                double *nu, double *beta, double *radius_ocean, double *grav_ocean);
 
+/* ---- warm-start network (FP64 checker of the GPU network) --------------- */
+void  orc_mlp_forward(int n_in, int width, int depth, int n_out, const double *params, int n,
+                      const double *x, double *y);
+double orc_mlp_loss_grad(int n_in, int width, int depth, int n_out, const double *params, int n,
+                         const double *x, const double *t, double *grad);
+
 #ifdef __cplusplus
 }
 #endif

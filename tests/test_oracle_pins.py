@@ -684,3 +684,58 @@ def test_rbf_reconstructs_solid_body_rotation(oracle):
         errs.append(np.abs(y - U).max() / np.abs(U).max())
     assert errs[2] < 3e-3, errs
     assert errs[0] / errs[1] > 3.5 and errs[1] / errs[2] > 3.5, errs
+
+
+# ---------------------------------------------------------------------------
+# (d) the warm-start network's FP64 checker
+# ---------------------------------------------------------------------------
+
+def _mlp_params(rng, shape, scale=0.5):
+    n_in, width, depth, n_out = shape
+    ps = []
+    for l in range(depth + 1):
+        ni = n_in if l == 0 else width
+        no = n_out if l == depth else width
+        ps += [rng.normal(scale=scale, size=ni * no), rng.normal(scale=0.1, size=no)]
+    return np.concatenate(ps)
+
+
+def test_oracle_mlp_closed_forms(oracle):
+    """The oracle network's forward pass on cases fixed by hand: zero
+    weights give the output bias for any input; one hidden unit gives
+    y = w2 tanh(w1 . x + b1) + b2."""
+    shape = (3, 4, 2, 2)
+    rng = np.random.default_rng(0)
+    p = _mlp_params(rng, shape)
+    n_w = 4 * 2   # the output layer's weights are the last 8 + 2 parameters
+    p0 = p.copy()
+    p0[-(n_w + 2):-2] = 0.0
+    x = rng.normal(size=(5, 3))
+    y = oracle.mlp_forward(shape, p0, x)
+    assert np.all(y == p0[-2:][None, :])
+    w1, b1, w2, b2 = np.array([0.3, -0.7]), 0.2, -1.5, 0.4
+    y = oracle.mlp_forward((2, 1, 1, 1), np.array([w1[0], w1[1], b1, w2, b2]), np.array([[1.0, 2.0]]))
+    assert abs(y[0, 0] - (w2 * np.tanh(w1 @ [1.0, 2.0] + b1) + b2)) < 1e-15
+    # two hidden layers of one unit with unit weights: tanh applied twice
+    y = oracle.mlp_forward((2, 1, 2, 1), np.array([w1[0], w1[1], b1, 1.0, 0.0, w2, b2]), np.array([[1.0, 2.0]]))
+    assert abs(y[0, 0] - (w2 * np.tanh(np.tanh(w1 @ [1.0, 2.0] + b1)) + b2)) < 1e-15
+
+
+def test_oracle_mlp_gradient_finite_differences(oracle):
+    """The oracle network's gradient against centred differences of its
+    loss, every parameter of a small network (relative 1e-7)."""
+    shape = (3, 5, 2, 4)
+    rng = np.random.default_rng(1)
+    p = _mlp_params(rng, shape)
+    x, t = rng.normal(size=(7, 3)), rng.normal(size=(7, 4))
+    loss, g = oracle.mlp_loss_grad(shape, p, x, t)
+    y = oracle.mlp_forward(shape, p, x)
+    assert abs(loss - np.sum((y - t) ** 2) / (2 * t.size)) < 1e-15 * loss
+    h = 1e-6
+    fd = np.zeros_like(p)
+    for i in range(len(p)):
+        a, b = p.copy(), p.copy()
+        a[i] += h
+        b[i] -= h
+        fd[i] = (oracle.mlp_loss_grad(shape, a, x, t)[0] - oracle.mlp_loss_grad(shape, b, x, t)[0]) / (2 * h)
+    assert np.max(np.abs(fd - g)) < 1e-7 * np.max(np.abs(g)), np.max(np.abs(fd - g))
