@@ -49,16 +49,18 @@ class InstanceRecord:
 
 
 def run_to_convergence(ctx, steps_per_orbit, tol=1e-4, stable=3, max_orbits=2000, chunk=100, retire=True,
-                       d_ref=None, band=0.01, log=None):
+                       d_ref=None, band=0.01, log=None, on_stop=None):
     """Step the batch of ctx until every instance has stopped.
 
     steps_per_orbit: N_s of each instance (multiples of chunk).  d_ref (optional,
     one value per instance): stop an instance once its orbit means stay within
     `band` of d_ref for `stable` orbits instead of at stationarity.  The
-    context's state is the start (odis_set_state or rest) at step 0.  Returns
-    one InstanceRecord per instance, in the context's original order;
-    ctx ends holding only the instances that never stopped (none, unless
-    retire is False)."""
+    context's state is the start (odis_set_state or rest) at step 0.
+    on_stop(slot, b), if given, runs when instance b (context slot `slot`)
+    stops, before it is retired (to read its final state).  Returns one
+    InstanceRecord per instance, in the context's original order; ctx ends
+    holding only the instances that never stopped (none, unless retire is
+    False)."""
     ns = np.asarray(steps_per_orbit, np.int64)
     B = len(ns)
     if ctx.B != B:
@@ -99,6 +101,8 @@ def run_to_convergence(ctx, steps_per_orbit, tol=1e-4, stable=3, max_orbits=2000
             if r.stop:
                 r.wall_s = now
                 done[b] = True
+                if on_stop is not None:
+                    on_stop(j, b)
         finished = [j for j in slots if done[live[j]]]
         if finished and log:
             log("converge: step %d, %d of %d instances stopped (%.1f s)" % (step, int(done.sum()), B, now))

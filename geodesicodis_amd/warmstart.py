@@ -199,7 +199,8 @@ def draw_instances(G, n, seed, tides=TIDES):
     U(-11, -7) s^-1 (the sweep's ranges), obliquity U(0, 1) degrees and the
     tide from `tides`, drawn in that order per instance from
     numpy.random.default_rng(seed).  Free surface, e = 0.0047, linear drag.
-    Returns the feature rows [log10 h, log10 alpha, obliquity_deg, tide]."""
+    Returns the feature rows [log10 h, log10 alpha, obliquity_deg, tide]
+    (obliquity 0 for ECC)."""
     rng = np.random.default_rng(seed)
     rows = []
     for _ in range(n):
@@ -211,27 +212,40 @@ def draw_instances(G, n, seed, tides=TIDES):
     return np.array(rows, np.float64)
 
 
-def instance_params(G, row, cells):
+ECC_DEFAULT = 0.0047
+
+
+def instance_params(G, row, cells, ecc=ECC_DEFAULT):
     """Params of one feature row on a grid of `cells` cells: dt the quantised
-    step below 0.35 / omega_max(h) of that grid."""
+    step below 0.35 / omega_max(h) of that grid.  Every row runs as FULL
+    (src/tidalPotentials.cpp:158-174): an ECC row with zero obliquity, the
+    same potential as ECC, so that all instances share a batch."""
     lh, la, ob, td = row
     h = 10.0 ** lh
     w = OMEGA_MAX_40962_38KM * (cells / 40962.0) ** 0.5 * (h / 38e3) ** 0.5
-    p, n = G.make_params(h=h, alpha=10.0 ** la, tide=int(td), obliquity=np.radians(ob), time_step=0.35 / w)
+    obl = np.radians(ob) if int(td) == FULL else 0.0
+    p, n = G.make_params(h=h, alpha=10.0 ** la, tide=FULL, ecc=ecc, obliquity=obl, time_step=0.35 / w)
     return p, n
 
 
 def features(rows):
-    """Network inputs of feature rows: log10 h - 4, (log10 alpha + 9) / 2,
-    2 obliquity - 1, and one-hot of the tide (ECC, FULL)."""
+    """Network inputs of feature rows: log10 h - 4 and (log10 alpha + 9) / 2.
+    The obliquity and the tide enter through the superposition of the
+    eccentricity and obliquity responses (the LTE is linear in the tide):
+    state = response(ECC, e) + obliquity_deg x response(OBLIQ, 1 degree)."""
     rows = np.asarray(rows, np.float64)
-    x = np.zeros((len(rows), 3 + len(TIDES)), np.float32)
-    x[:, 0] = rows[:, 0] - 4.0
-    x[:, 1] = (rows[:, 1] + 9.0) / 2.0
-    x[:, 2] = 2.0 * rows[:, 2] - 1.0
-    for k, t in enumerate(TIDES):
-        x[:, 3 + k] = rows[:, 3] == t
-    return x
+    return np.stack([rows[:, 0] - 4.0, (rows[:, 1] + 9.0) / 2.0], 1).astype(np.float32)
+
+
+def combine(rows, pair):
+    """Coefficients of the rows' states from [ECC | OBLIQ at 1 degree]
+    response pairs (one block each, of any number of phases): ECC plus the
+    row's obliquity in degrees times the obliquity response (0 for ECC)."""
+    rows = np.asarray(rows, np.float64)
+    pair = np.asarray(pair)
+    half = pair.shape[1] // 2
+    ob = np.where(rows[:, 3] == FULL, rows[:, 2], 0.0)
+    return pair[:, :half] + ob[:, None] * pair[:, half:]
 
 
 _POOL_STATE = {}
@@ -248,35 +262,297 @@ def _pool_init(level, L):
     _POOL_STATE["cells"] = mesh.N
 
 
-def _pool_solve(row):
+def _pool_solve(hrow):
+    """[ECC phase 0, phase pi/2 | OBLIQ (1 degree) phase 0, phase pi/2]
+    coefficients of the periodic states of thickness / drag row hrow."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
     G, ps, sb = _POOL_STATE["G"], _POOL_STATE["solver"], _POOL_STATE["basis"]
-    p, _ = instance_params(G, row, _POOL_STATE["cells"])
-    V, H = ps.solve(p)
-    return np.concatenate([sb.project(V.real, H.real), sb.project(V.imag, H.imag)])
+    p, _ = instance_params(G, (hrow[0], hrow[1], 1.0, FULL), _POOL_STATE["cells"])
+    w = ps.omega
+    A = ((1j * w + p.alpha) * sp.identity(ps.E, format="csr") - ps.C + (p.g * p.h / (1j * w)) * ps.GD).tocsc()
+    lu = spl.splu(A)
+    out = []
+    for tide, ecc, obl in ((ECC, ECC_DEFAULT, 0.0), (OBLIQ, 0.0, np.radians(1.0))):
+        p.tide, p.ecc, p.obliquity = tide, ecc, obl
+        V = lu.solve(ps.G @ forcing_amplitude(p, ps.lat, ps.lon))
+        H = p.h * (ps.D @ V) / (1j * w)
+        out += [sb.project(V.real, H.real), sb.project(V.imag, H.imag)]
+    return np.concatenate(out)
 
 
-def periodic_coefficients(rows, level=6, L=16, workers=8):
-    """Coefficients [phase 0 | phase pi/2] of the periodic states of the
-    feature rows, solved on the generated grid of reference level `level`
-    (10242 cells at 6) in `workers` processes: shape (n, 2 * basis size).
-    The state at phase 0 is Re(V), Re(H); at phase pi/2 (a quarter period
-    later, up to sign) Im(V), Im(H)."""
+def periodic_pairs(hrows, level=6, L=16, workers=8):
+    """[ECC | OBLIQ at 1 degree] x [phase 0 | phase pi/2] coefficients of the
+    periodic states of (log10 h, log10 alpha) rows, solved on the generated
+    grid of reference level `level` (10242 cells at 6) in `workers`
+    processes (one LU per row, two tides): shape (n, 4 x basis size).  The
+    state at phase 0 is Re(V), Re(H); a quarter period later Im(V), Im(H)
+    up to sign."""
     import multiprocessing as mp
     import os
-    rows = [tuple(r) for r in np.asarray(rows)]
+    hrows = [tuple(r[:2]) for r in np.asarray(hrows)]
     if workers <= 1:
         _pool_init(level, L)
-        return np.array([_pool_solve(r) for r in rows])
+        return np.array([_pool_solve(r) for r in hrows])
     # fresh interpreters (no inherited thread pools), one BLAS / OpenMP thread each
     keep = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")}
     for k in keep:
         os.environ[k] = "1"
     try:
         with mp.get_context("spawn").Pool(workers, initializer=_pool_init, initargs=(level, L)) as pool:
-            return np.array(pool.map(_pool_solve, rows, chunksize=4))
+            return np.array(pool.map(_pool_solve, hrows, chunksize=4))
     finally:
         for k, v in keep.items():
             if v is None:
                 os.environ.pop(k, None)
             else:
                 os.environ[k] = v
+
+
+def periodic_states(rows, level=6, workers=8):
+    """[phase 0 | phase pi/2] coefficients of the periodic states of feature
+    rows (the superposition of their tide's responses)."""
+    pr = periodic_pairs(rows, level=level, workers=workers)
+    q = pr.shape[1] // 4
+    return combine(rows, np.concatenate([pr[:, :2 * q], pr[:, 2 * q:]], 1))
+
+
+# ---------------------------------------------------------------------------
+# converged targets, the model, and seeding
+# ---------------------------------------------------------------------------
+
+def _grid_mesh(G, level):
+    g = G.Grid.generate(level, 64)
+    p0, _ = G.make_params()
+    return g, G.Mesh(g, p0.radius)
+
+
+def seed_states(ctx, basis, coeffs, first=0):
+    """Set instances first, first + 1, ... of ctx to the states of the
+    coefficient rows (phase 0), at step 0 with the Euler start (the rest
+    start's, src/temporalOperators.cpp:48-65), as the reference seeds its
+    analytic start (src/initialConditions.cpp:146-206)."""
+    for k, c in enumerate(np.asarray(coeffs)):
+        v, eta = basis.synthesize(c)
+        ctx.set_state(first + k, v, None, eta, None, 0)
+
+
+def periodic_dissipation(G, mesh, params, basis, coeffs2, precision=None):
+    """Orbit-mean dissipation D* of periodic states given as [phase 0 |
+    phase pi/2] coefficient rows: the state is a cos t + b sin t, so the
+    orbit mean of the quadratic dissipation (src/energy.cpp:13-62) is
+    (D(a) + D(b)) / 2, evaluated by the GPU context on the mesh."""
+    n = basis.size
+    ctx = G.Context(mesh, list(params), precision=G.FP64 if precision is None else precision)
+    out = np.zeros(len(params))
+    for half in (0, 1):
+        seed_states(ctx, basis, np.asarray(coeffs2)[:, half * n:(half + 1) * n])
+        out += 0.5 * ctx.dissipation()
+    ctx.close()
+    return out
+
+
+def converged_targets(G, hrows, pairs, level=7, orbits=2, batch=256, precision=None, log=None):
+    """Training targets from converged runs of the solver: for every (h,
+    alpha) row, its ECC response and its OBLIQ response (1 degree) are
+    started from their periodic states (pairs, phase 0) on the generated
+    grid of reference level `level` and stepped `orbits` whole orbits by the
+    batched plain step (FP32 by default, both tides as FULL); each target is
+    the coefficient vector of the state after the last orbit (phase 0
+    again).  Returns (targets [ECC | OBLIQ], D_last, resid_v, resid_eta),
+    the last three per run (2 per row)."""
+    from .converge import run_to_convergence
+    g, mesh = _grid_mesh(G, level)
+    basis = StateBasis(mesh, g.arrays(), L=16)
+    n = basis.size
+    hrows = np.asarray(hrows)
+    m = len(hrows)
+    # runs: row r's ECC response is run 2r, its OBLIQ response run 2r + 1
+    runs = []
+    for r in hrows:
+        runs.append(((r[0], r[1], 0.0, ECC), ECC_DEFAULT))
+        runs.append(((r[0], r[1], 1.0, FULL), 0.0))
+    q = np.asarray(pairs).shape[1] // 4
+    start = np.zeros((2 * m, n))
+    start[0::2] = np.asarray(pairs)[:, :n]
+    start[1::2] = np.asarray(pairs)[:, 2 * q:2 * q + n]
+    tg = np.zeros((2 * m, n))
+    dl, rv, re = np.zeros(2 * m), np.zeros(2 * m), np.zeros(2 * m)
+    prec = G.FP32 if precision is None else precision
+    for s in range(0, 2 * m, batch):
+        sub = runs[s:s + batch]
+        ps, ns = zip(*[instance_params(G, row, mesh.N, ecc=e) for row, e in sub])
+        ctx = G.Context(mesh, list(ps), precision=prec)
+        seed_states(ctx, basis, start[s:s + batch])
+
+        def grab(slot, b, ctx=ctx, s=s):
+            st = ctx.get_state(slot)
+            tg[s + b] = basis.project(st["v"], st["eta"])
+
+        recs = run_to_convergence(ctx, ns, max_orbits=orbits, stable=10 ** 9, on_stop=grab)
+        for b, r in enumerate(recs):
+            dl[s + b], rv[s + b], re[s + b] = r.D[-1], r.resid_v[-1], r.resid_eta[-1]
+        ctx.close()
+        if log:
+            log("warm start: converged targets %d / %d runs" % (s + len(sub), 2 * m))
+    return np.concatenate([tg[0::2], tg[1::2]], 1), dl, rv, re
+
+
+class WarmStartModel:
+    """Network + target normalisation.  The network maps (log10 h, log10
+    alpha) to the phase-0 coefficients of the ECC response and of the
+    OBLIQ response at 1 degree; a row's state is their superposition
+    (combine).  Each response's coefficient vector c is learned as its
+    direction c / |c| (standardised by one scale over all components, so
+    that the loss is its relative L2 error) and log |c| (standardised)."""
+
+    def __init__(self, G, width=256, depth=4, size=865, seed=TRAIN_SEED):
+        self.G = G
+        self.size = size
+        self.mlp = G.Mlp(2, width, depth, 2 * (size + 1), seed=seed)
+        self.norm = None
+
+    def _split(self, c):
+        n = self.size
+        return [c[:, :n], c[:, n:2 * n]]
+
+    def fit(self, hrows, coeffs, epochs=1500, batch=250, lr_start=2e-3, lr_end=2e-5, seed=TRAIN_SEED):
+        """coeffs: (n, 2 size) [ECC | OBLIQ 1 degree] phase-0 targets."""
+        c = np.asarray(coeffs, np.float64)
+        self.norm = {}
+        z = []
+        for k, part in enumerate(self._split(c)):
+            nrm = np.linalg.norm(part, axis=1, keepdims=True)
+            d = part / nrm
+            ln = np.log(nrm[:, 0])
+            self.norm.update({"dmu%d" % k: d.mean(0), "dsd%d" % k: float(d.std()) + 1e-12,
+                              "lmu%d" % k: float(ln.mean()), "lsd%d" % k: float(ln.std()) + 1e-12})
+            z += [(d - self.norm["dmu%d" % k]) / self.norm["dsd%d" % k],
+                  ((ln - self.norm["lmu%d" % k]) / self.norm["lsd%d" % k])[:, None]]
+        self.mlp.set_data(features(hrows), np.concatenate(z, 1).astype(np.float32))
+        return self.mlp.train(epochs, batch, lr_start, lr_end, seed)
+
+    def predict_pairs(self, rows):
+        """(n, 2 size) predicted [ECC | OBLIQ 1 degree] phase-0 responses."""
+        n = self.size
+        z = self.mlp.forward(features(rows)).astype(np.float64)
+        out = []
+        for k in range(2):
+            zk = z[:, k * (n + 1):(k + 1) * (n + 1)]
+            d = zk[:, :n] * self.norm["dsd%d" % k] + self.norm["dmu%d" % k]
+            d /= np.linalg.norm(d, axis=1, keepdims=True)
+            out.append(d * np.exp(zk[:, n:] * self.norm["lsd%d" % k] + self.norm["lmu%d" % k]))
+        return np.concatenate(out, 1)
+
+    def predict(self, rows):
+        """(n, size) predicted phase-0 coefficients of the rows' states."""
+        return combine(rows, self.predict_pairs(rows))
+
+    def save(self, path):
+        np.savez(path, params=self.mlp.params(), shape=np.array(self.mlp.shape), size=self.size,
+                 **{k: np.asarray(v) for k, v in self.norm.items()})
+
+    @classmethod
+    def load(cls, G, path):
+        f = np.load(path)
+        shape = [int(x) for x in f["shape"]]
+        m = cls(G, width=shape[1], depth=shape[2], size=int(f["size"]))
+        m.mlp.set_params(f["params"])
+        m.norm = {k: (float(f[k]) if f[k].ndim == 0 else f[k]) for k in f.files
+                  if k[:3] in ("dmu", "dsd", "lmu", "lsd")}
+        return m
+
+
+def build_model(G, n_train=2000, solve_level=6, run_level=7, orbits=2, epochs=1500, batch=250, workers=16,
+                seed=TRAIN_SEED, log=None):
+    """The whole training pipeline: draw n_train instances (seed), solve the
+    periodic ECC and OBLIQ responses of their (h, alpha) on the solve_level
+    grid, run each two orbits on the run_level grid from that state
+    (converged runs of the solver), and train the network on the GPU on the
+    resulting phase-0 coefficients.  Returns (model, info)."""
+    import time
+    t0 = time.perf_counter()
+    rows = draw_instances(G, n_train, seed)
+    pairs = periodic_pairs(rows, level=solve_level, workers=workers)
+    t1 = time.perf_counter()
+    if log:
+        log("warm start: %d periodic solves (2 tides) on level %d in %.1f s" % (n_train, solve_level, t1 - t0))
+    tg, dl, rv, re = converged_targets(G, rows, pairs, level=run_level, orbits=orbits, log=log)
+    t2 = time.perf_counter()
+    model = WarmStartModel(G, seed=seed)
+    loss = model.fit(rows, tg, epochs=epochs, batch=batch, seed=seed)
+    t3 = time.perf_counter()
+    n = model.size
+    q = pairs.shape[1] // 4
+    solved = np.concatenate([pairs[:, :n], pairs[:, 2 * q:2 * q + n]], 1)
+    shift = np.linalg.norm(tg - solved, axis=1) / np.linalg.norm(solved, axis=1)
+    fit = np.linalg.norm(model.predict_pairs(rows) - tg, axis=1) / np.linalg.norm(tg, axis=1)
+    info = {"n_train": n_train, "seed": seed, "solve_level": solve_level, "run_level": run_level,
+            "orbits": orbits, "epochs": epochs, "batch": batch, "loss_first": float(loss[0]),
+            "loss_last": float(loss[-1]), "solve_s": t1 - t0, "runs_s": t2 - t1, "train_s": t3 - t2,
+            "target_vs_solve_median": float(np.median(shift)), "target_vs_solve_max": float(shift.max()),
+            "periodicity_resid_v_median": float(np.median(rv)), "periodicity_resid_v_max": float(np.max(rv)),
+            "periodicity_resid_eta_median": float(np.median(re)),
+            "train_fit_rel_err_median": float(np.median(fit)), "train_fit_rel_err_max": float(fit.max())}
+    return model, info
+
+
+def evaluate(G, model, rows, level=7, solve_level=6, cap_warm=30, cap_rest=20, band=0.01, stable=3,
+             precision=None, workers=16, log=None):
+    """Held-out check by continued plain stepping: every instance of `rows`
+    runs batched from its predicted state and from rest until its orbit
+    means stay within `band` of D* (the orbit mean of its periodic state,
+    solved on the solve_level grid) for `stable` orbits, or to the caps.
+    Also times one plain FP64 instance on the grid, for the sequential
+    plain method's wall time: sum over instances of orbits-to-band x steps
+    per orbit x that step time (a lower bound where the rest start capped).
+    Returns a dict of per-instance orbits and the wall-time ratio."""
+    import time
+    from .converge import run_to_convergence
+    prec = G.FP32 if precision is None else precision
+    g, mesh = _grid_mesh(G, level)
+    basis = StateBasis(mesh, g.arrays(), L=16)
+    rows = np.asarray(rows)
+    ps, ns = zip(*[instance_params(G, r, mesh.N) for r in rows])
+    C2 = periodic_states(rows, level=solve_level, workers=workers)
+    d_star = periodic_dissipation(G, mesh, ps, basis, C2)
+    pred = model.predict(rows)
+    n = basis.size
+    pred_err = np.linalg.norm(pred - C2[:, :n], axis=1) / np.linalg.norm(C2[:, :n], axis=1)
+    out = {"instances": len(rows), "cells": mesh.N, "band": band, "stable_orbits": stable,
+           "pred_rel_err_median": float(np.median(pred_err)), "pred_rel_err_p90": float(np.percentile(pred_err, 90)),
+           "pred_rel_err_max": float(pred_err.max())}
+    res = {}
+    for name, seed, cap in (("warm", True, cap_warm), ("rest", False, cap_rest)):
+        ctx = G.Context(mesh, list(ps), precision=prec)
+        if seed:
+            seed_states(ctx, basis, pred)
+        t0 = time.perf_counter()
+        recs = run_to_convergence(ctx, ns, d_ref=d_star, band=band, stable=stable, max_orbits=cap, log=log)
+        wall = time.perf_counter() - t0
+        ctx.close()
+        orb = np.array([r.orbits - stable + 1 if r.stop == "band" else r.orbits for r in recs])
+        res[name] = (recs, orb, wall)
+        out[name] = {"wall_s": wall, "cap_orbits": cap,
+                     "in_band": int(sum(r.stop == "band" for r in recs)),
+                     "orbits_to_band_median": float(np.median(orb)), "orbits_to_band_max": int(orb.max()),
+                     "first_orbit_in_band": int(sum(r.stop == "band" and r.orbits == stable for r in recs))}
+        if log:
+            log("warm start: %s start, %d of %d in band, %.1f s" % (name, out[name]["in_band"], len(rows), wall))
+    # one plain FP64 instance on this grid: step time
+    hs = [p.h for p in ps]
+    mid = int(np.argsort(hs)[len(hs) // 2])
+    c1 = G.Context(mesh, ps[mid], precision=G.FP64)
+    c1.step(50)
+    t0 = time.perf_counter()
+    c1.step(1000)
+    t_fp64 = (time.perf_counter() - t0) / 1000
+    c1.close()
+    rest_orb = res["rest"][1]
+    plain = float(np.sum(rest_orb * np.array(ns) * t_fp64))
+    out["plain_fp64_step_s"] = t_fp64
+    out["plain_fp64_sequential_s"] = plain
+    out["plain_is_lower_bound"] = bool(any(r.stop != "band" for r in res["rest"][0]))
+    out["speedup_vs_plain_fp64"] = plain / res["warm"][2]
+    out["rest_batched_over_warm_batched"] = res["rest"][2] / res["warm"][2]
+    return out
