@@ -281,7 +281,7 @@ def _pool_solve(hrow):
     return np.concatenate(out)
 
 
-def periodic_pairs(hrows, level=6, L=16, workers=8):
+def periodic_pairs(hrows, level=6, L=16, workers=8, log=None):
     """[ECC | OBLIQ at 1 degree] x [phase 0 | phase pi/2] coefficients of the
     periodic states of (log10 h, log10 alpha) rows, solved on the generated
     grid of reference level `level` (10242 cells at 6) in `workers`
@@ -300,7 +300,12 @@ def periodic_pairs(hrows, level=6, L=16, workers=8):
         os.environ[k] = "1"
     try:
         with mp.get_context("spawn").Pool(workers, initializer=_pool_init, initargs=(level, L)) as pool:
-            return np.array(pool.map(_pool_solve, hrows, chunksize=4))
+            out = []
+            for r in pool.imap(_pool_solve, hrows, chunksize=4):
+                out.append(r)
+                if log and len(out) % 200 == 0:
+                    log("warm start: %d / %d periodic solves" % (len(out), len(hrows)))
+            return np.array(out)
     finally:
         for k, v in keep.items():
             if v is None:
@@ -309,10 +314,10 @@ def periodic_pairs(hrows, level=6, L=16, workers=8):
                 os.environ[k] = v
 
 
-def periodic_states(rows, level=6, workers=8):
+def periodic_states(rows, level=6, workers=8, log=None):
     """[phase 0 | phase pi/2] coefficients of the periodic states of feature
     rows (the superposition of their tide's responses)."""
-    pr = periodic_pairs(rows, level=level, workers=workers)
+    pr = periodic_pairs(rows, level=level, workers=workers, log=log)
     q = pr.shape[1] // 4
     return combine(rows, np.concatenate([pr[:, :2 * q], pr[:, 2 * q:]], 1))
 
@@ -473,7 +478,7 @@ def build_model(G, n_train=2000, solve_level=6, run_level=7, orbits=2, epochs=15
     import time
     t0 = time.perf_counter()
     rows = draw_instances(G, n_train, seed)
-    pairs = periodic_pairs(rows, level=solve_level, workers=workers)
+    pairs = periodic_pairs(rows, level=solve_level, workers=workers, log=log)
     t1 = time.perf_counter()
     if log:
         log("warm start: %d periodic solves (2 tides) on level %d in %.1f s" % (n_train, solve_level, t1 - t0))
@@ -514,7 +519,7 @@ def evaluate(G, model, rows, level=7, solve_level=6, cap_warm=30, cap_rest=20, b
     basis = StateBasis(mesh, g.arrays(), L=16)
     rows = np.asarray(rows)
     ps, ns = zip(*[instance_params(G, r, mesh.N) for r in rows])
-    C2 = periodic_states(rows, level=solve_level, workers=workers)
+    C2 = periodic_states(rows, level=solve_level, workers=workers, log=log)
     d_star = periodic_dissipation(G, mesh, ps, basis, C2)
     pred = model.predict(rows)
     n = basis.size
@@ -556,3 +561,53 @@ def evaluate(G, model, rows, level=7, solve_level=6, cap_warm=30, cap_rest=20, b
     out["speedup_vs_plain_fp64"] = plain / res["warm"][2]
     out["rest_batched_over_warm_batched"] = res["rest"][2] / res["warm"][2]
     return out
+
+
+def _main(argv=None):
+    import argparse
+    import json
+    import sys
+    import time
+
+    import geodesicodis_amd as G
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    ap = argparse.ArgumentParser(prog="python -m geodesicodis_amd.warmstart")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    b = sub.add_parser("build", help="train the network from converged runs and save it")
+    b.add_argument("--out", required=True)
+    b.add_argument("--n-train", type=int, default=2000)
+    b.add_argument("--solve-level", type=int, default=6)
+    b.add_argument("--run-level", type=int, default=7)
+    b.add_argument("--epochs", type=int, default=2000)
+    b.add_argument("--batch", type=int, default=250)
+    b.add_argument("--workers", type=int, default=16)
+    e = sub.add_parser("evaluate", help="held-out instances from the network's states and from rest")
+    e.add_argument("--model", required=True)
+    e.add_argument("--n", type=int, default=256)
+    e.add_argument("--level", type=int, default=7)
+    e.add_argument("--solve-level", type=int, default=6)
+    e.add_argument("--cap-warm", type=int, default=30)
+    e.add_argument("--cap-rest", type=int, default=20)
+    e.add_argument("--workers", type=int, default=16)
+    a = ap.parse_args(argv)
+    t0 = time.perf_counter()
+    if a.cmd == "build":
+        model, info = build_model(G, n_train=a.n_train, solve_level=a.solve_level, run_level=a.run_level,
+                                  epochs=a.epochs, batch=a.batch, workers=a.workers, log=log)
+        model.save(a.out)
+        info["wall_s"] = time.perf_counter() - t0
+        print(json.dumps(info), flush=True)
+    else:
+        model = WarmStartModel.load(G, a.model)
+        rows = draw_instances(G, a.n, HELDOUT_SEED)
+        ev = evaluate(G, model, rows, level=a.level, solve_level=a.solve_level, cap_warm=a.cap_warm,
+                      cap_rest=a.cap_rest, workers=a.workers, log=log)
+        ev["wall_s"] = time.perf_counter() - t0
+        print(json.dumps(ev), flush=True)
+
+
+if __name__ == "__main__":
+    _main()
