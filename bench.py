@@ -257,7 +257,14 @@ def sweep_run(G, args):
     ps = sweep_params(G, args.sweep_instances)
     mesh = G.Mesh(grid, ps[0].radius)
     n_steps, n_warm = max(20, args.steps // 10), 10
+    tr = None
     ctx = G.Context(mesh, ps, precision=G.FP32)
+    if not args.no_tune:
+        from geodesicodis_amd import tune as T
+        t0 = time.perf_counter()
+        tr = T.tune(mesh, ps, precision=G.FP32, log=lambda m: print("bench: " + m, file=sys.stderr, flush=True))
+        tr["tune_s"] = time.perf_counter() - t0
+        T.apply(ctx, T.option_ids(tr["options"]))
     el, _ = timed_run(ctx, n_steps, n_warm, 0)
     ok = bool(all(abs(x) < float("inf") for x in ctx.dissipation()))
     ctx.close()
@@ -271,7 +278,9 @@ def sweep_run(G, args):
     return {"instances": len(ps), "cells": mesh.N, "seed": SWEEP_SEED, "precision": "fp32 state, fp64 dissipation",
             "value": batched, "ms_per_step": 1e3 * el / n_steps, "steps": n_steps,
             "plain_fp64_single_instance": single, "batched_over_single": batched / single,
-            "finite": ok}
+            "finite": ok,
+            "autotune": ({x: tr[x] for x in ("options", "default", "ms_per_step", "default_ms_per_step", "speedup",
+                                             "candidates", "cached", "tune_s")} if tr else "off")}
 
 
 def mfma_flops(n_cells, l_max, n_inst):
@@ -467,6 +476,34 @@ def converge_run(G, args):
     return out
 
 
+WARMSTART_MODEL = os.path.join(ROOT, "geodesicodis_amd", "data", "warmstart_c5.npz")
+
+
+def converged_leg(G, args):
+    """Second half of the metric: wall time to the converged dissipation band
+    (orbit means within 1 % of D* for 3 orbits) of held-out sweep instances
+    (seed 20261206: thickness 10^U(3,5) m, drag 10^U(-11,-7) s^-1, ECC or
+    FULL tide) on the 40962-cell grid.  Batched FP32 from the learned warm
+    start (the committed network, trained on the GPU from 2000 converged
+    runs) against batched FP32 from rest, with retirement, and against the
+    plain FP64 method run one instance after another (orbits from rest x
+    steps per orbit x the measured FP64 step time; a lower bound where the
+    rest start reached its cap).  D* is each instance's periodic state,
+    solved on the 10242-cell grid."""
+    from geodesicodis_amd import warmstart as W
+    model = W.WarmStartModel.load(G, WARMSTART_MODEL)
+    n = args.converged_instances if args.workload != "warmstart" else max(args.converged_instances, 256)
+    rows = W.draw_instances(G, n, W.HELDOUT_SEED)
+    t0 = time.perf_counter()
+    ev = W.evaluate(G, model, rows, level=7, solve_level=6, cap_warm=30, cap_rest=args.converged_cap_rest,
+                    workers=min(16, os.cpu_count() or 1),
+                    log=lambda m: print("bench: " + m, file=sys.stderr, flush=True))
+    ev["wall_s"] = time.perf_counter() - t0
+    ev["seed"] = W.HELDOUT_SEED
+    ev["model"] = os.path.relpath(WARMSTART_MODEL, ROOT)
+    return ev
+
+
 def rel_l2(a, b):
     import numpy as np
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
@@ -477,10 +514,19 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--workload", choices=["shell", "free", "converge", "sweep", "uneven"], default="shell",
+    ap.add_argument("--workload", choices=["shell", "free", "converge", "sweep", "uneven", "warmstart"],
+                    default="shell",
                     help="shell: ocean under a 20 km ice shell with SH coupling to degree 32 (default); "
                          "free: free-surface ocean, stencils only; converge: time to converged dissipation "
-                         "of the plain-method case, FP64 and FP32; sweep / uneven: the batched extras alone")
+                         "of the plain-method case, FP64 and FP32; sweep / uneven: the batched extras alone; "
+                         "warmstart: time to the converged dissipation band of held-out sweep instances "
+                         "from the learned warm start and from rest")
+    ap.add_argument("--no-converged", action="store_true",
+                    help="skip the time-to-converged leg (learned warm start vs rest, held-out instances)")
+    ap.add_argument("--converged-instances", type=int, default=64,
+                    help="held-out sweep instances of the time-to-converged leg")
+    ap.add_argument("--converged-cap-rest", type=int, default=10,
+                    help="orbit cap of the rest start in the time-to-converged leg")
     ap.add_argument("--alpha", type=float, default=1e-8, help="linear drag of the converge workload (s^-1)")
     ap.add_argument("--max-orbits", type=int, default=3000, help="orbit cap of the converge workload")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched-sweep extra")
@@ -514,6 +560,9 @@ def main():
     ap.add_argument("--face-cell", action="store_true",
                     help="a face and a cell kernel per step instead of the one-kernel step")
     ap.add_argument("--graphs", action="store_true", help="replay 64-step chunks as HIP graphs")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="launch options from the flags / library defaults instead of the autotuner's choice "
+                         "(the tuner runs when no launch flag is given)")
     ap.add_argument("--patch-cells", type=int, default=0,
                     help="cells per patch of the one-kernel step (64, 128, 256; 0: library default)")
     ap.add_argument("--step-lanes", type=int, default=0,
@@ -545,8 +594,12 @@ def main():
     import numpy as np
     import geodesicodis_amd as G
 
+    host = host_cpu_info()  # before any OpenMP runtime binds this thread to one place
     if args.workload == "converge":
         print(json.dumps(converge_run(G, args)), flush=True)
+        return
+    if args.workload == "warmstart":
+        print(json.dumps(converged_leg(G, args)), flush=True)
         return
     if args.workload in ("sweep", "uneven"):
         r = sweep_run(G, args) if args.workload == "sweep" else uneven_run(G, args)
@@ -570,9 +623,32 @@ def main():
           file=sys.stderr, flush=True)
     K = (SH_DEGREE + 1) ** 2
 
+    from geodesicodis_amd import tune as T
+    explicit = (args.tabulated or args.face_cell or args.graphs or args.patch_cells or args.step_lanes or
+                args.step_blocks)
+    tuned = {}
+
+    def tuned_options(prec):
+        """The autotuner's launch options of the headline workload at prec."""
+        if args.no_tune or explicit:
+            return None
+        if prec not in tuned:
+            sh = None
+            if shell:
+                sh = (SH_DEGREE, beta, G.SH_RECUR if args.sh_basis == "recur" else
+                      (G.SH_FP64 if prec == G.FP64 else G.SH_SPLIT_BF16))
+            t0 = time.perf_counter()
+            tuned[prec] = T.tune(mesh, [p], precision=prec, sh=sh,
+                                 log=lambda m: print("bench: " + m, file=sys.stderr, flush=True))
+            tuned[prec]["tune_s"] = time.perf_counter() - t0
+        return tuned[prec]
+
     def make_ctx(prec):
         ctx = G.Context(mesh, p, precision=prec)
         apply_launch_options(G, ctx, args)
+        tr = tuned_options(prec)
+        if tr:
+            T.apply(ctx, T.option_ids(tr["options"]))
         if shell:
             if args.sh_basis == "recur":
                 if args.sh_cells:
@@ -673,7 +749,11 @@ def main():
                       (("one stencil kernel per step (patches of <= %d cells, %d lanes per cell, %d workgroups)"
                         % (ctx.option(G.OPT_PATCH_CELLS), ctx.option(G.OPT_STEP_LANES),
                            ctx.option(G.OPT_STEP_BLOCKS))) if one else "face + cell kernel per step"),
-            "graphs": bool(args.graphs),
+            "graphs": bool(ctx.option(G.OPT_GRAPHS)),
+            "autotune": ({("fp64" if k == G.FP64 else "fp32"): {
+                x: v[x] for x in ("options", "default", "ms_per_step", "default_ms_per_step", "speedup",
+                                  "candidates", "cached", "tune_s")} for k, v in tuned.items()}
+                         if tuned else "off (launch flags given or --no-tune)"),
             "kernel_ms": {k: r["ms"] for k, r in kernels.items()},
             "step_events_ms": tm["step_ms"],
             "alg_bytes": alg,
@@ -760,6 +840,9 @@ def main():
     if not args.no_extra and not args.no_uneven:
         out["config"]["uneven_fp32"] = uneven_run(G, args)
 
+    if not args.no_extra and not args.no_converged:
+        out["config"]["time_to_converged"] = converged_leg(G, args)
+
     if not args.no_cpu_baseline:
         import oracle as O
         threads = min(16, os.cpu_count() or 1)
@@ -799,7 +882,6 @@ def main():
         O.set_threads(1)
         n1, t1 = cpu_sample(args.cpu_seconds / 3.0)
         O.set_threads(used)
-        host = host_cpu_info()
         out["cpu_baseline"] = {
             "value": mesh.N * n / t_cpu,
             "unit": "cell-steps/s",

@@ -1801,6 +1801,7 @@ struct odis_ctx {
   int step_blocks = 0;         // workgroups of the one-kernel step (ODIS_OPT_STEP_BLOCKS; 0: one per patch)
   int step_lanes = 4;          // lanes per cell of the one-kernel step (ODIS_OPT_STEP_LANES: 2 or 4)
   int il_opt = 2;              // ODIS_OPT_INTERLEAVE: 0 off, 1 on, 2 auto (batches of >= kInterleaveMin)
+  int il_unroll = 0;           // ODIS_OPT_IL_UNROLL: 0 default, 1 (1, 1), 2 (2, 1), 3 (4, 2) faces / cells per lane
   int il_Bp = 0;               // instances of the interleaved copy, padded to kInterleaveGroup (0: none yet)
   void *il_buf = nullptr;      // interleaved v[2], eta[2], ring_v, ring_e, ts[2]
   int il_fresh = 0;            // the interleaved copy holds the current state
@@ -3080,6 +3081,29 @@ static void launch_pair(odis_ctx *ctx, const FaceArgs<T> &fa, const CellArgs<T> 
 // (n0 = ctx->step): per step the SH projection kernels (when a basis is
 // set), the face kernel and the cell kernel.  evs: dispatch-stamped events on
 // the kernels of every ctx->timing-th step (null: none).
+// One interleaved step launch: FU faces and CU (cell, instance) pairs per
+// lane in flight; the first launch of a chunk has no pending dissipation.
+template <typename T, int FU, int CU>
+static void launch_step_il(odis_ctx *ctx, const StepArgs<T> &a, const double *table, int s, hipEvent_t *ev) {
+  constexpr int IB = kInterleaveGroup;
+  const dim3 gp(a.n_blocks, ctx->il_Bp / IB), bp(256);
+  if (s == 0) {
+    if (ev)
+      hipExtLaunchKernelGGL((k_step_il<T, false, 64, IB, FU, CU>), gp, bp, 0, ctx->stream, ev[0], ev[1], 0, a,
+                            ctx->energy_quad, table, s);
+    else
+      hipLaunchKernelGGL((k_step_il<T, false, 64, IB, FU, CU>), gp, bp, 0, ctx->stream, a, ctx->energy_quad, table,
+                         s);
+  } else {
+    if (ev)
+      hipExtLaunchKernelGGL((k_step_il<T, true, 64, IB, FU, CU>), gp, bp, 0, ctx->stream, ev[0], ev[1], 0, a,
+                            ctx->energy_quad, table, s);
+    else
+      hipLaunchKernelGGL((k_step_il<T, true, 64, IB, FU, CU>), gp, bp, 0, ctx->stream, a, ctx->energy_quad, table,
+                         s);
+  }
+}
+
 template <typename T>
 static int launch_chunk(odis_ctx *ctx, Ops<T> &o, const double *table, int count, ChunkEvents *evs,
                         size_t *ev_used) {
@@ -3113,7 +3137,6 @@ static int launch_chunk(odis_ctx *ctx, Ops<T> &o, const double *table, int count
       if (rc) return rc;
     }
     if (one && il) {
-      constexpr int IB = kInterleaveGroup, FU = IlUnroll<T>::faces, CU = IlUnroll<T>::cells;
       StepArgs<T> a = step_args_il(ctx, o);
       hipEvent_t *ev = nullptr;
       if (stamp) {
@@ -3121,21 +3144,12 @@ static int launch_chunk(odis_ctx *ctx, Ops<T> &o, const double *table, int count
         ev = &ctx->ev[*ev_used];
         *ev_used += 2;
       }
-      const dim3 gp(a.n_blocks, ctx->il_Bp / IB), bp(256);
-      if (s == 0) {
-        if (ev)
-          hipExtLaunchKernelGGL((k_step_il<T, false, 64, IB, FU, CU>), gp, bp, 0, ctx->stream, ev[0], ev[1], 0, a,
-                                ctx->energy_quad, table, s);
-        else
-          hipLaunchKernelGGL((k_step_il<T, false, 64, IB, FU, CU>), gp, bp, 0, ctx->stream, a, ctx->energy_quad,
-                             table, s);
-      } else {
-        if (ev)
-          hipExtLaunchKernelGGL((k_step_il<T, true, 64, IB, FU, CU>), gp, bp, 0, ctx->stream, ev[0], ev[1], 0, a,
-                                ctx->energy_quad, table, s);
-        else
-          hipLaunchKernelGGL((k_step_il<T, true, 64, IB, FU, CU>), gp, bp, 0, ctx->stream, a, ctx->energy_quad,
-                             table, s);
+      // faces and (cell, instance) pairs per lane in flight (ODIS_OPT_IL_UNROLL)
+      switch (ctx->il_unroll) {
+        case 1: launch_step_il<T, 1, 1>(ctx, a, table, s, ev); break;
+        case 2: launch_step_il<T, 2, 1>(ctx, a, table, s, ev); break;
+        case 3: launch_step_il<T, 4, 2>(ctx, a, table, s, ev); break;
+        default: launch_step_il<T, IlUnroll<T>::faces, IlUnroll<T>::cells>(ctx, a, table, s, ev); break;
       }
     } else if (one) {
       // the face phase reads eta_eff (face_args) when a basis is set
@@ -3666,6 +3680,13 @@ static int set_option_impl(odis_ctx *ctx, int option, int value) {
     ctx->il_opt = value;
     return ODIS_OK;
   }
+  if (option == ODIS_OPT_IL_UNROLL) {
+    if (value < 0 || value > 3) return fail(ctx, ODIS_E_INVALID, "interleaved unroll: 0 (default), 1, 2 or 3");
+    CK(hipStreamSynchronize(ctx->stream));
+    drop_graphs(ctx);
+    ctx->il_unroll = value;
+    return ODIS_OK;
+  }
   if (option == ODIS_OPT_STEP_LANES) {
     if (value != 2 && value != 4) return fail(ctx, ODIS_E_INVALID, "step lanes must be 2 or 4");
     CK(hipStreamSynchronize(ctx->stream));
@@ -3709,6 +3730,7 @@ extern "C" int odis_get_option(odis_ctx *ctx, int option, int *value) {
   else if (option == ODIS_OPT_SH_CELLS) *value = ctx->sh_cpt;
   else if (option == ODIS_OPT_INTERLEAVE) *value = ctx->il_opt;
   else if (option == ODIS_OPT_INTERLEAVE_ACTIVE) *value = il_wanted(ctx) ? 1 : 0;
+  else if (option == ODIS_OPT_IL_UNROLL) *value = ctx->il_unroll;
   else return fail(ctx, ODIS_E_INVALID, "unknown option");
   return ODIS_OK;
 }

@@ -294,7 +294,11 @@ int odis_request_stop(odis_ctx *ctx);
  *                    same bit for bit; dissipation sums differ by rounding
  *                    (another grouping of the per-face terms).
  *  ODIS_OPT_INTERLEAVE_ACTIVE (read only) 1 when the next step would run
- *                    interleaved. */
+ *                    interleaved.
+ *  ODIS_OPT_IL_UNROLL faces and (cell, instance) pairs each lane of the
+ *                    interleaved step keeps in flight: 0 (default: 4 and 2
+ *                    in FP64, 2 and 1 in FP32), 1 (1, 1), 2 (2, 1), 3 (4, 2).
+ *                    The state is the same bit for bit. */
 #define ODIS_OPT_GRAPHS     1
 #define ODIS_OPT_COMPACT    2
 #define ODIS_OPT_ONE_KERNEL 3
@@ -304,6 +308,7 @@ int odis_request_stop(odis_ctx *ctx);
 #define ODIS_OPT_SH_CELLS    7
 #define ODIS_OPT_INTERLEAVE  8
 #define ODIS_OPT_INTERLEAVE_ACTIVE 9
+#define ODIS_OPT_IL_UNROLL 10
 int odis_set_option(odis_ctx *ctx, int option, int value);
 int odis_get_option(odis_ctx *ctx, int option, int *value);
 /* Timing for benchmarks: with stride T > 0, the kernels of every step n of
