@@ -476,6 +476,49 @@ def converge_run(G, args):
     return out
 
 
+def shell_orbit_run(G, args):
+    """The shell workload (163842 cells, 20 km LID_MEMBR shell, SH coupling
+    to degree 32, ECC, rest start) stepped args.orbits whole orbits (296000
+    steps each) in plain FP64, in the FP32 form (FP32 stencils and
+    recurrence) and in FP32 + split-BF16 MFMA products; the two fast forms'
+    relative L2 errors in eta and v and the relative error of the dissipation
+    against FP64, after every orbit."""
+    import numpy as np
+    grid = generated_grid(G, args.level)
+    p, steps_per_orbit, beta = shell_setup(G)
+    mesh = G.Mesh(grid, p.radius)
+    forms = (("fp64", G.FP64, G.SH_RECUR), ("fp32", G.FP32, G.SH_RECUR), ("fp32_mfma", G.FP32, G.SH_MFMA))
+    ctxs = {}
+    for name, prec, mode in forms:
+        c = G.Context(mesh, p, precision=prec)
+        c.set_sh(SH_DEGREE, beta, mode=mode)
+        ctxs[name] = c
+    n_orbits = max(1, int(round(args.orbits)))
+    rows, walls = [], {k: 0.0 for k in ctxs}
+    chunk = steps_per_orbit // 8
+    for k in range(n_orbits):
+        for name, c in ctxs.items():
+            t0 = time.perf_counter()
+            for q in range(8):
+                c.step(chunk)
+                print("bench: shell-orbit %s orbit %d, %d / 8" % (name, k + 1, q + 1), file=sys.stderr, flush=True)
+            walls[name] += time.perf_counter() - t0
+        ref = ctxs["fp64"].get_state()
+        d_ref = float(ctxs["fp64"].dissipation()[0])
+        row = {"orbit": k + 1, "steps": (k + 1) * steps_per_orbit, "dissipation_fp64_W_m2": d_ref}
+        for name in ("fp32", "fp32_mfma"):
+            st = ctxs[name].get_state()
+            d = float(ctxs[name].dissipation()[0])
+            row[name] = {"eta": rel_l2(st["eta"], ref["eta"]), "v": rel_l2(st["v"], ref["v"]),
+                         "dissipation": abs(d - d_ref) / abs(d_ref)}
+        rows.append(row)
+    for c in ctxs.values():
+        c.close()
+    return {"workload": "shell, 163842 cells, degree 32, %d orbit(s) of %d steps, rest start" % (n_orbits,
+                                                                                              steps_per_orbit),
+            "errors_vs_fp64": rows, "wall_s": walls}
+
+
 WARMSTART_MODEL = os.path.join(ROOT, "geodesicodis_amd", "data", "warmstart_c5.npz")
 
 
@@ -514,13 +557,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--workload", choices=["shell", "free", "converge", "sweep", "uneven", "warmstart"],
+    ap.add_argument("--workload", choices=["shell", "free", "converge", "sweep", "uneven", "warmstart",
+                                           "shell-orbit"],
                     default="shell",
                     help="shell: ocean under a 20 km ice shell with SH coupling to degree 32 (default); "
                          "free: free-surface ocean, stencils only; converge: time to converged dissipation "
                          "of the plain-method case, FP64 and FP32; sweep / uneven: the batched extras alone; "
                          "warmstart: time to the converged dissipation band of held-out sweep instances "
-                         "from the learned warm start and from rest")
+                         "from the learned warm start and from rest; shell-orbit: the shell workload's FP32 "
+                         "and FP32 + split-BF16 MFMA forms against FP64 after whole orbits")
+    ap.add_argument("--orbits", type=float, default=1.0,
+                    help="orbits of the shell-orbit workload (fast forms against FP64 after whole orbits)")
     ap.add_argument("--no-converged", action="store_true",
                     help="skip the time-to-converged leg (learned warm start vs rest, held-out instances)")
     ap.add_argument("--converged-instances", type=int, default=64,
@@ -600,6 +647,9 @@ def main():
         return
     if args.workload == "warmstart":
         print(json.dumps(converged_leg(G, args)), flush=True)
+        return
+    if args.workload == "shell-orbit":
+        print(json.dumps(shell_orbit_run(G, args)), flush=True)
         return
     if args.workload in ("sweep", "uneven"):
         r = sweep_run(G, args) if args.workload == "sweep" else uneven_run(G, args)
