@@ -616,9 +616,10 @@ def main():
                     help="lanes per cell of the one-kernel step (2 or 4; 0: library default)")
     ap.add_argument("--step-blocks", type=int, default=0,
                     help="workgroups of the one-kernel step (0: library default, one per patch)")
-    ap.add_argument("--timing-stride", type=int, default=16,
+    ap.add_argument("--timing-stride", type=int, default=-1,
                     help="dispatch-stamped HIP events on the kernels of every T-th step (0: none, "
-                         "roofline achieved then null)")
+                         "roofline achieved then null; default: 16, or fewer for short runs so that at "
+                         "least 5 steps are sampled)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (20 times), to calibrate "
                          "the FETCH_SIZE / WRITE_SIZE counters, and exit")
@@ -633,6 +634,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": n * 8, "launches": 20}), flush=True)
         return
+    if args.timing_stride < 0:
+        args.timing_stride = max(1, min(16, args.steps // 5))
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU; multi-GPU runs are not part of it "
               "(requested --gpus %d)" % args.gpus, file=sys.stderr)

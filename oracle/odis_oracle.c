@@ -25,8 +25,16 @@
  *                                 src/energy.cpp:13-62
  *   dt quantisation, period       src/mesh.cpp:1566-1623,           Enceladus: P=118394 s, dt=29.5985 s, N_s=4000
  *                                 src/globals.cpp:209-214
- *   membraneNuBeta                src/membraneConstants.cpp:8-144   shipped beta table l<=18 at h_s=20 km (in-container)
- *   PLANET / FULL2 forcing        src/tidalPotentials.cpp:133-225   parity unpinned (no closed-form check beyond formula)
+ *   membraneNuBeta                src/membraneConstants.cpp:8-144   shipped beta table l<=18 at h_s=20 km (in-container);
+ *                                                                     limits: no shell and no ocean = FREE_LOADING,
+ *                                                                     fluid core, rigid core (cubic in h_s); the
+ *                                                                     core-shell coupling terms only loosely
+ *   PLANET forcing                src/tidalPotentials.cpp:176-225   pure degree 2; G m R^2 / d^3 towards the companion
+ *   FULL2 forcing                 src/tidalPotentials.cpp:133-156   first order: ECC at zero obliquity, OBLIQ a
+ *                                                                     quarter period later at zero eccentricity;
+ *                                                                     second-order terms parity unpinned
+ *   OBLIQ_WEST closed form        src/analyticalLTE.cpp:133-180     exact continuity by centred differences
+ *   warm-start network            (no reference counterpart)        closed forms; gradient by centred differences
  */
 #include "odis_oracle.h"
 
