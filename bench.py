@@ -259,7 +259,9 @@ def sweep_run(G, args):
     n_steps, n_warm = max(20, args.steps // 10), 10
     tr = None
     ctx = G.Context(mesh, ps, precision=G.FP32)
-    if not args.no_tune:
+    if args.il_order >= 0:
+        ctx.set_option(G.OPT_IL_ORDER, args.il_order)
+    elif not args.no_tune:
         from geodesicodis_amd import tune as T
         t0 = time.perf_counter()
         tr = T.tune(mesh, ps, precision=G.FP32, log=lambda m: print("bench: " + m, file=sys.stderr, flush=True))
@@ -280,7 +282,7 @@ def sweep_run(G, args):
             "plain_fp64_single_instance": single, "batched_over_single": batched / single,
             "finite": ok,
             "autotune": ({x: tr[x] for x in ("options", "default", "ms_per_step", "default_ms_per_step", "speedup",
-                                             "candidates", "cached", "tune_s")} if tr else "off")}
+                                             "candidates", "cached", "tune_s", "timings_ms")} if tr else "off")}
 
 
 def mfma_flops(n_cells, l_max, n_inst):
@@ -607,6 +609,9 @@ def main():
     ap.add_argument("--face-cell", action="store_true",
                     help="a face and a cell kernel per step instead of the one-kernel step")
     ap.add_argument("--graphs", action="store_true", help="replay 64-step chunks as HIP graphs")
+    ap.add_argument("--il-order", type=int, default=-1,
+                    help="workgroup order of the sweep's interleaved step (ODIS_OPT_IL_ORDER; implies --no-tune "
+                         "for the sweep)")
     ap.add_argument("--no-tune", action="store_true",
                     help="launch options from the flags / library defaults instead of the autotuner's choice "
                          "(the tuner runs when no launch flag is given)")

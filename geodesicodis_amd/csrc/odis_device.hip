@@ -437,7 +437,13 @@ __global__ __launch_bounds__(LPC * P) void k_step(StepArgs<T> a, int energy_quad
 // the state is the same bit for bit; an instance's dissipation partial adds
 // its faces in another grouping.  Padding instances (b >= B) compute with
 // the last instance's parameters and store nothing.
-template <typename T, bool PENDING, int P, int IB, int FU, int CU>
+// GF: one-dimensional grid of n_blocks x groups workgroups, each XCD a
+// contiguous range of (patch, group) tiles with the groups fastest, so that
+// the groups of a patch (which read its operators) and neighbouring patches
+// (which read each other's boundary state) run on one XCD close in time;
+// else a grid (n_blocks, groups) with each group's patches spread as
+// contiguous ranges over the XCDs.
+template <typename T, bool PENDING, int P, int IB, int FU, int CU, bool GF = false>
 __global__ __launch_bounds__(256) void k_step_il(StepArgs<T> a, int energy_quad, const double *table, int s) {
   constexpr int NT = 256, SL = NT / IB;
   __shared__ T lv[6 * P * IB];
@@ -446,10 +452,19 @@ __global__ __launch_bounds__(256) void k_step_il(StepArgs<T> a, int energy_quad,
   __shared__ double sh[NT / 64];
   const int B = a.f.B, E = a.f.E, N = a.f.N;
   const int ib = threadIdx.x % IB, sl = threadIdx.x / IB;
-  const int b = blockIdx.y * IB + ib;
+  const int n_grp = (B + IB - 1) / IB;
+  int j, grp;
+  if (GF) {
+    const int t = xcd_tile(blockIdx.x, a.n_blocks * n_grp);
+    j = t / n_grp;
+    grp = t - j * n_grp;
+  } else {
+    j = xcd_tile(blockIdx.x, a.n_blocks);
+    grp = blockIdx.y;
+  }
+  const int b = grp * IB + ib;
   const bool live = b < B;
   const long long n = (long long)table[0] + s;
-  const int j = xcd_tile(blockIdx.x, a.n_blocks);
   const int lo = a.own_lo[j], n_own = a.own_lo[j + 1] - lo;
   const int flo = a.for_ptr[j], n_loc = n_own + (a.for_ptr[j + 1] - flo);
   double local = 0.0;
@@ -523,13 +538,13 @@ __global__ __launch_bounds__(256) void k_step_il(StepArgs<T> a, int energy_quad,
   if (threadIdx.x < IB) {
     double sum = 0.0;
     for (int u = 0; u < SL; ++u) sum += red[u * IB + threadIdx.x];
-    const int bb = blockIdx.y * IB + threadIdx.x;
+    const int bb = grp * IB + threadIdx.x;
     if (bb < B) a.f.partials[((n & 1) * B + bb) * a.n_blocks + j] = sum;
   }
-  if (!PENDING || blockIdx.x != 0) return;
+  if (!PENDING || j != 0) return;
   // dissipation of step n - 1 of the group's instances: partials of the previous launch
   for (int u = 0; u < IB; ++u) {
-    const int bb = blockIdx.y * IB + u;
+    const int bb = grp * IB + u;
     if (bb >= B) break;  // uniform in the workgroup
     __syncthreads();     // sh is free
     const double tot = sum_partials<NT / 64>(a.f.partials + (((n - 1) & 1) * B + bb) * size_t(a.n_blocks),
@@ -1802,6 +1817,7 @@ struct odis_ctx {
   int step_lanes = 4;          // lanes per cell of the one-kernel step (ODIS_OPT_STEP_LANES: 2 or 4)
   int il_opt = 2;              // ODIS_OPT_INTERLEAVE: 0 off, 1 on, 2 auto (batches of >= kInterleaveMin)
   int il_unroll = 0;           // ODIS_OPT_IL_UNROLL: 0 default, 1 (1, 1), 2 (2, 1), 3 (4, 2) faces / cells per lane
+  int il_order = 0;            // ODIS_OPT_IL_ORDER: 0 patches per group over the XCDs, 1 groups fastest
   int il_Bp = 0;               // instances of the interleaved copy, padded to kInterleaveGroup (0: none yet)
   void *il_buf = nullptr;      // interleaved v[2], eta[2], ring_v, ring_e, ts[2]
   int il_fresh = 0;            // the interleaved copy holds the current state
@@ -3083,25 +3099,34 @@ static void launch_pair(odis_ctx *ctx, const FaceArgs<T> &fa, const CellArgs<T> 
 // the kernels of every ctx->timing-th step (null: none).
 // One interleaved step launch: FU faces and CU (cell, instance) pairs per
 // lane in flight; the first launch of a chunk has no pending dissipation.
-template <typename T, int FU, int CU>
-static void launch_step_il(odis_ctx *ctx, const StepArgs<T> &a, const double *table, int s, hipEvent_t *ev) {
+template <typename T, int FU, int CU, bool GF>
+static void launch_step_il_order(odis_ctx *ctx, const StepArgs<T> &a, const double *table, int s, hipEvent_t *ev) {
   constexpr int IB = kInterleaveGroup;
-  const dim3 gp(a.n_blocks, ctx->il_Bp / IB), bp(256);
+  const int n_grp = (ctx->B + IB - 1) / IB;
+  const dim3 gp = GF ? dim3(unsigned(a.n_blocks) * unsigned(n_grp), 1) : dim3(a.n_blocks, n_grp), bp(256);
   if (s == 0) {
     if (ev)
-      hipExtLaunchKernelGGL((k_step_il<T, false, 64, IB, FU, CU>), gp, bp, 0, ctx->stream, ev[0], ev[1], 0, a,
+      hipExtLaunchKernelGGL((k_step_il<T, false, 64, IB, FU, CU, GF>), gp, bp, 0, ctx->stream, ev[0], ev[1], 0, a,
                             ctx->energy_quad, table, s);
     else
-      hipLaunchKernelGGL((k_step_il<T, false, 64, IB, FU, CU>), gp, bp, 0, ctx->stream, a, ctx->energy_quad, table,
-                         s);
+      hipLaunchKernelGGL((k_step_il<T, false, 64, IB, FU, CU, GF>), gp, bp, 0, ctx->stream, a, ctx->energy_quad,
+                         table, s);
   } else {
     if (ev)
-      hipExtLaunchKernelGGL((k_step_il<T, true, 64, IB, FU, CU>), gp, bp, 0, ctx->stream, ev[0], ev[1], 0, a,
+      hipExtLaunchKernelGGL((k_step_il<T, true, 64, IB, FU, CU, GF>), gp, bp, 0, ctx->stream, ev[0], ev[1], 0, a,
                             ctx->energy_quad, table, s);
     else
-      hipLaunchKernelGGL((k_step_il<T, true, 64, IB, FU, CU>), gp, bp, 0, ctx->stream, a, ctx->energy_quad, table,
-                         s);
+      hipLaunchKernelGGL((k_step_il<T, true, 64, IB, FU, CU, GF>), gp, bp, 0, ctx->stream, a, ctx->energy_quad,
+                         table, s);
   }
+}
+
+template <typename T, int FU, int CU>
+static void launch_step_il(odis_ctx *ctx, const StepArgs<T> &a, const double *table, int s, hipEvent_t *ev) {
+  if (ctx->il_order)
+    launch_step_il_order<T, FU, CU, true>(ctx, a, table, s, ev);
+  else
+    launch_step_il_order<T, FU, CU, false>(ctx, a, table, s, ev);
 }
 
 template <typename T>
@@ -3680,6 +3705,13 @@ static int set_option_impl(odis_ctx *ctx, int option, int value) {
     ctx->il_opt = value;
     return ODIS_OK;
   }
+  if (option == ODIS_OPT_IL_ORDER) {
+    if (value < 0 || value > 1) return fail(ctx, ODIS_E_INVALID, "interleaved order: 0 or 1");
+    CK(hipStreamSynchronize(ctx->stream));
+    drop_graphs(ctx);
+    ctx->il_order = value;
+    return ODIS_OK;
+  }
   if (option == ODIS_OPT_IL_UNROLL) {
     if (value < 0 || value > 3) return fail(ctx, ODIS_E_INVALID, "interleaved unroll: 0 (default), 1, 2 or 3");
     CK(hipStreamSynchronize(ctx->stream));
@@ -3731,6 +3763,7 @@ extern "C" int odis_get_option(odis_ctx *ctx, int option, int *value) {
   else if (option == ODIS_OPT_INTERLEAVE) *value = ctx->il_opt;
   else if (option == ODIS_OPT_INTERLEAVE_ACTIVE) *value = il_wanted(ctx) ? 1 : 0;
   else if (option == ODIS_OPT_IL_UNROLL) *value = ctx->il_unroll;
+  else if (option == ODIS_OPT_IL_ORDER) *value = ctx->il_order;
   else return fail(ctx, ODIS_E_INVALID, "unknown option");
   return ODIS_OK;
 }

@@ -19,12 +19,12 @@ import json
 import os
 import time
 
-from . import (FP64, OPT_GRAPHS, OPT_IL_UNROLL, OPT_INTERLEAVE, OPT_INTERLEAVE_ACTIVE, OPT_ONE_KERNEL,
-               OPT_PATCH_CELLS, OPT_STEP_LANES, SH_OFF, Context, OdisError)
+from . import (FP64, OPT_GRAPHS, OPT_IL_ORDER, OPT_IL_UNROLL, OPT_INTERLEAVE, OPT_INTERLEAVE_ACTIVE,
+               OPT_ONE_KERNEL, OPT_PATCH_CELLS, OPT_STEP_LANES, SH_OFF, Context, OdisError)
 
-TUNED = (OPT_ONE_KERNEL, OPT_PATCH_CELLS, OPT_STEP_LANES, OPT_INTERLEAVE, OPT_IL_UNROLL, OPT_GRAPHS)
+TUNED = (OPT_ONE_KERNEL, OPT_PATCH_CELLS, OPT_STEP_LANES, OPT_INTERLEAVE, OPT_IL_UNROLL, OPT_IL_ORDER, OPT_GRAPHS)
 NAMES = {OPT_ONE_KERNEL: "one_kernel", OPT_PATCH_CELLS: "patch_cells", OPT_STEP_LANES: "step_lanes",
-         OPT_INTERLEAVE: "interleave", OPT_IL_UNROLL: "il_unroll", OPT_GRAPHS: "graphs"}
+         OPT_INTERLEAVE: "interleave", OPT_IL_UNROLL: "il_unroll", OPT_IL_ORDER: "il_order", OPT_GRAPHS: "graphs"}
 
 _MEMO = {}
 
@@ -78,8 +78,8 @@ def candidates(ctx):
                     OPT_INTERLEAVE: 0}
             out.append(dict(base))
             if patch == 64 and ctx.B >= 2:
-                for unroll in (0, 1, 2, 3):
-                    out.append({**base, OPT_INTERLEAVE: 1, OPT_IL_UNROLL: unroll})
+                for unroll, order in itertools.product((0, 1, 2, 3), (0, 1)):
+                    out.append({**base, OPT_INTERLEAVE: 1, OPT_IL_UNROLL: unroll, OPT_IL_ORDER: order})
     return out
 
 
@@ -97,7 +97,8 @@ def apply(ctx, options):
     """Set option values (a dict option -> value) on a context; options the
     context refuses are left as they are.  Returns the options set."""
     done = {}
-    order = (OPT_ONE_KERNEL, OPT_PATCH_CELLS, OPT_STEP_LANES, OPT_INTERLEAVE, OPT_IL_UNROLL, OPT_GRAPHS)
+    order = (OPT_ONE_KERNEL, OPT_PATCH_CELLS, OPT_STEP_LANES, OPT_INTERLEAVE, OPT_IL_UNROLL, OPT_IL_ORDER,
+             OPT_GRAPHS)
     for o in order:
         if o in options:
             try:

@@ -298,7 +298,12 @@ int odis_request_stop(odis_ctx *ctx);
  *  ODIS_OPT_IL_UNROLL faces and (cell, instance) pairs each lane of the
  *                    interleaved step keeps in flight: 0 (default: 4 and 2
  *                    in FP64, 2 and 1 in FP32), 1 (1, 1), 2 (2, 1), 3 (4, 2).
- *                    The state is the same bit for bit. */
+ *                    The state is the same bit for bit.
+ *  ODIS_OPT_IL_ORDER  workgroup order of the interleaved step: 0 (default)
+ *                    each group of 16 instances has its patches spread over
+ *                    the XCDs as contiguous ranges; 1 the groups of a patch
+ *                    run next to each other on one XCD (its operators read
+ *                    from L2 once for all groups).  Same state bit for bit. */
 #define ODIS_OPT_GRAPHS     1
 #define ODIS_OPT_COMPACT    2
 #define ODIS_OPT_ONE_KERNEL 3
@@ -309,6 +314,7 @@ int odis_request_stop(odis_ctx *ctx);
 #define ODIS_OPT_INTERLEAVE  8
 #define ODIS_OPT_INTERLEAVE_ACTIVE 9
 #define ODIS_OPT_IL_UNROLL 10
+#define ODIS_OPT_IL_ORDER 11
 int odis_set_option(odis_ctx *ctx, int option, int value);
 int odis_get_option(odis_ctx *ctx, int option, int *value);
 /* Timing for benchmarks: with stride T > 0, the kernels of every step n of
