@@ -48,7 +48,7 @@ H5_DISPLACEMENT, H5_VELOCITY, H5_VELOCITY_XYZ, H5_DISSIPATION, H5_DISS_AVG = 1, 
 SH_OFF, SH_FP64, SH_SPLIT_BF16, SH_RECUR, SH_MFMA = 0, 1, 2, 3, 4
 OPT_GRAPHS, OPT_COMPACT, OPT_ONE_KERNEL, OPT_PATCH_CELLS, OPT_STEP_BLOCKS, OPT_STEP_LANES, OPT_SH_CELLS = \
     1, 2, 3, 4, 5, 6, 7
-OPT_INTERLEAVE, OPT_INTERLEAVE_ACTIVE, OPT_IL_UNROLL, OPT_IL_ORDER = 8, 9, 10, 11
+OPT_INTERLEAVE, OPT_INTERLEAVE_ACTIVE, OPT_IL_UNROLL, OPT_IL_ORDER, OPT_IL_RECORDS = 8, 9, 10, 11, 12
 
 
 class OdisError(RuntimeError):

@@ -324,6 +324,9 @@ struct StepArgs {
   const unsigned short *cyc_loc;  // [6][N] cycle faces, patch-local index
   int n_patch;
   int n_blocks;                 // workgroups per instance (= gridDim.x); each runs a contiguous run of patches
+  int max_loc;                  // most faces (owned + foreign) of a patch
+  const T *face_rec;            // [E][16] operand records of k_step_il (kRecSlots), or null
+  const T *cell_rec;            // [N][16]
 };
 
 // One patch of the one-kernel step; returns this lane's dissipation
@@ -427,6 +430,90 @@ __global__ __launch_bounds__(LPC * P) void k_step(StepArgs<T> a, int energy_quad
   }
 }
 
+// Operand records of the interleaved step.  Each face and each cell keeps
+// its operands in one record of kRecSlots values of the state type, so
+// that the 16 lanes of a group (one DPP row: lanes 16r .. 16r + 15 serve
+// the 16 instances of one face or cell) load the record with one vector
+// load, lane k slot k, and take each operand from its slot with a row
+// broadcast (v_mov_b32_dpp row_newbcast), instead of all 16 lanes loading
+// every operand.  Integer operands are kept as bit patterns.
+//   face e: 0, 1 gradient cells c0, c1; 2, 3 their coefficients; 4 Coriolis;
+//           5 area; 6 d_e; 7 l_e; 8 + k forcing field k at c0, 12 + k at c1
+//           (k < kRecFields)
+//   cell i: 0 cycle info; 1-3 the six patch-local cycle faces, two 16-bit
+//           indices per slot; 4-9 kite fractions; 10 area
+constexpr int kRecSlots = 16;
+constexpr int kRecFields = 4;  // forcing fields a face record holds
+
+template <typename T>
+__device__ __forceinline__ T int_slot(int x) {
+  if constexpr (sizeof(T) == 8)
+    return __longlong_as_double((long long)x);
+  else
+    return __int_as_float(x);
+}
+
+template <typename T>
+__device__ __forceinline__ int slot_int(T x) {
+  if constexpr (sizeof(T) == 8)
+    return int(__double_as_longlong(x));
+  else
+    return __float_as_int(x);
+}
+
+// slot S of the lane's DPP row (every lane of the row active)
+template <int S>
+__device__ __forceinline__ float row_slot(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x150 + S, 0xF, 0xF, false));
+}
+template <int S>
+__device__ __forceinline__ double row_slot(double x) {
+  const long long v = __double_as_longlong(x);
+  const unsigned lo = unsigned(__builtin_amdgcn_update_dpp(0, int(v), 0x150 + S, 0xF, 0xF, false));
+  const unsigned hi = unsigned(__builtin_amdgcn_update_dpp(0, int(v >> 32), 0x150 + S, 0xF, 0xF, false));
+  return __longlong_as_double((long long)((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pack_face_rec(int E, int N, int nf, const int *__restrict__ g_idx,
+                                                          const T *__restrict__ g_coef, const T *__restrict__ fcor,
+                                                          const T *__restrict__ farea, const T *__restrict__ fdist,
+                                                          const T *__restrict__ flen, const T *__restrict__ cellS,
+                                                          T *__restrict__ rec) {
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= E) return;
+  const int c0 = g_idx[e], c1 = g_idx[E + e];
+  T *r = rec + size_t(e) * kRecSlots;
+  r[0] = int_slot<T>(c0);
+  r[1] = int_slot<T>(c1);
+  r[2] = g_coef[e];
+  r[3] = g_coef[E + e];
+  r[4] = fcor[e];
+  r[5] = farea[e];
+  r[6] = fdist[e];
+  r[7] = flen[e];
+  for (int k = 0; k < kRecFields; ++k) {
+    r[8 + k] = k < nf ? cellS[size_t(k) * N + c0] : T(0);
+    r[12 + k] = k < nf ? cellS[size_t(k) * N + c1] : T(0);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pack_cell_rec(int N, const int *__restrict__ info,
+                                                          const unsigned short *__restrict__ cyc_loc,
+                                                          const T *__restrict__ kite, const T *__restrict__ area,
+                                                          T *__restrict__ rec) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= N) return;
+  T *r = rec + size_t(i) * kRecSlots;
+  r[0] = int_slot<T>(info[i]);
+  for (int q = 0; q < 3; ++q)
+    r[1 + q] = int_slot<T>(int(cyc_loc[size_t(2 * q) * N + i]) | (int(cyc_loc[size_t(2 * q + 1) * N + i]) << 16));
+  for (int q = 0; q < 6; ++q) r[4 + q] = kite[size_t(q) * N + i];
+  r[10] = area[i];
+  for (int q = 11; q < kRecSlots; ++q) r[q] = T(0);
+}
+
 // One-kernel step of a batch held interleaved: every per-instance array is
 // held in groups of IB instances, each group element-major with its
 // instances innermost (sidx<true>).  Workgroup (patch j, group of IB
@@ -437,19 +524,27 @@ __global__ __launch_bounds__(LPC * P) void k_step(StepArgs<T> a, int energy_quad
 // the state is the same bit for bit; an instance's dissipation partial adds
 // its faces in another grouping.  Padding instances (b >= B) compute with
 // the last instance's parameters and store nothing.
+// LDS (dynamic, il_lds_bytes): the dissipation partials, then per local
+// face l_e v_e^{n+1} of the group and the face's id, sized for the largest
+// patch (a.max_loc faces) rather than 6 P, so that more workgroups fit a CU.
+// REC: operands from the records (a.face_rec, a.cell_rec; at most
+// kRecFields forcing fields), else from the operand arrays.  The per-face
+// arithmetic is face_update's, written out in the same order.
 // GF: one-dimensional grid of n_blocks x groups workgroups, each XCD a
 // contiguous range of (patch, group) tiles with the groups fastest, so that
 // the groups of a patch (which read its operators) and neighbouring patches
 // (which read each other's boundary state) run on one XCD close in time;
 // else a grid (n_blocks, groups) with each group's patches spread as
 // contiguous ranges over the XCDs.
-template <typename T, bool PENDING, int P, int IB, int FU, int CU, bool GF = false>
+template <typename T, bool PENDING, int P, int IB, int FU, int CU, bool GF = false, bool REC = false>
 __global__ __launch_bounds__(256) void k_step_il(StepArgs<T> a, int energy_quad, const double *table, int s) {
+  static_assert(!REC || IB == 16, "a record is shared through one 16-lane DPP row");
   constexpr int NT = 256, SL = NT / IB;
-  __shared__ T lv[6 * P * IB];
-  __shared__ int gid[6 * P];
-  __shared__ double red[SL * IB];
-  __shared__ double sh[NT / 64];
+  extern __shared__ __align__(16) unsigned char il_lds[];
+  double *red = reinterpret_cast<double *>(il_lds);  // [SL * IB]
+  double *sh = red + SL * IB;                         // [NT / 64]
+  T *lv = reinterpret_cast<T *>(sh + NT / 64);        // [max_loc * IB]
+  int *gid = reinterpret_cast<int *>(lv + size_t(a.max_loc) * IB);  // [max_loc]
   const int B = a.f.B, E = a.f.E, N = a.f.N;
   const int ib = threadIdx.x % IB, sl = threadIdx.x / IB;
   const int n_grp = (B + IB - 1) / IB;
@@ -470,68 +565,195 @@ __global__ __launch_bounds__(256) void k_step_il(StepArgs<T> a, int energy_quad,
   double local = 0.0;
   // FU faces per lane are updated before any of them is stored, so that
   // their loads are in flight together (the stores could alias them)
-  for (int q0 = sl; q0 < n_loc; q0 += SL * FU) {
-    FaceOut<T> r[FU];
-    int e[FU];
+  if constexpr (REC) {
+    // the instance's parameters once (face_update reads them per face)
+    const int bp = min(b, B - 1);
+    const double *ip = a.f.inst + bp * 8;
+    const double *coef = table + 8 + size_t(s) * B * 8;
+    const T g = T(ip[0]), alpha = T(ip[1]), dt = T(ip[2]), qd = T(ip[4]);
+    const bool ab3 = (n >= 2) || (ip[3] != 0.0);
+    const int nf = a.f.nfields;
+    T ck[kRecFields];
 #pragma unroll
-    for (int u = 0; u < FU; ++u) {
-      const int q = min(q0 + u * SL, n_loc - 1);  // past the end: the last face again, not stored
-      e[u] = q < n_own ? lo + q : a.for_idx[flo + q - n_own];
-      r[u] = face_update<T, false, true, true>(a.f, e[u], b, energy_quad, table, s);
-    }
+    for (int k = 0; k < kRecFields; ++k) ck[k] = k < nf ? T(coef[bp * 8 + k]) : T(0);
+    const int s0 = int(n % 3), s1 = int((n + 2) % 3), s2 = int((n + 1) % 3);
+    for (int q0 = sl; q0 < n_loc; q0 += SL * FU) {
+      T vn[FU], f0o[FU], fl[FU];
+      double en_u[FU];
+      int e[FU];
 #pragma unroll
-    for (int u = 0; u < FU; ++u) {
-      const int q = q0 + u * SL;
-      if (q >= n_loc) break;
-      if (q < n_own && live) {
-        local += r[u].energy;
-        a.f.ring[sidx<true>(b, a.f.ring_stride, (long long)int(n % 3) * E + e[u])] = r[u].f0;
-        a.f.v_nxt[sidx<true>(b, a.f.v_stride, e[u])] = r[u].vn;
+      for (int u = 0; u < FU; ++u) {
+        const int q = min(q0 + u * SL, n_loc - 1);  // past the end: the last face again, not stored
+        e[u] = q < n_own ? lo + q : a.for_idx[flo + q - n_own];
+        const T mine = a.face_rec[size_t(e[u]) * kRecSlots + ib];
+        const int c0 = min(unsigned(slot_int(row_slot<0>(mine))), unsigned(N - 1));
+        const int c1 = min(unsigned(slot_int(row_slot<1>(mine))), unsigned(N - 1));
+        const T gc0 = row_slot<2>(mine), gc1 = row_slot<3>(mine), fcor = row_slot<4>(mine);
+        const T farea = row_slot<5>(mine), fdist = row_slot<6>(mine);
+        fl[u] = row_slot<7>(mine);
+        const T S0[kRecFields] = {row_slot<8>(mine), row_slot<9>(mine), row_slot<10>(mine), row_slot<11>(mine)};
+        const T S1[kRecFields] = {row_slot<12>(mine), row_slot<13>(mine), row_slot<14>(mine), row_slot<15>(mine)};
+        // face_update<T, false, true, true>, operation for operation
+        const T vt = (a.f.ts[sidx<true>(b, a.f.ts_stride, e[u])] -
+                      a.f.ts[sidx<true>(b, a.f.ts_stride, E + e[u])]) / fdist;
+        const T ve = a.f.v_cur[sidx<true>(b, a.f.v_stride, e[u])];
+        const T sp = ve * ve + vt * vt;
+        const T en = energy_quad ? sqrt(sp) * sp : sp;
+        en_u[u] = double(farea) * double(en);
+        const T ge = gc0 * a.f.eta[sidx<true>(b, a.f.eta_stride, c0)] + gc1 * a.f.eta[sidx<true>(b, a.f.eta_stride, c1)];
+        T f0 = -g * ge + fcor * vt;
+        if (a.f.quad_drag) f0 -= qd * sqrt(sp) * ve;
+        T u0 = T(0), u1 = T(0);
+#pragma unroll
+        for (int k = 0; k < kRecFields; ++k)
+          if (k < nf) {
+            u0 += ck[k] * S0[k];
+            u1 += ck[k] * S1[k];
+          }
+        const T fu = gc0 * u0 + gc1 * u1;
+        const T drag = -alpha * ve + fu;
+        T upd;
+        if (ab3)
+          upd = (T(23. / 12.) * f0 + T(-16. / 12.) * a.f.ring[sidx<true>(b, a.f.ring_stride, (long long)s1 * E + e[u])] +
+                 T(5. / 12.) * a.f.ring[sidx<true>(b, a.f.ring_stride, (long long)s2 * E + e[u])]) *
+                dt;
+        else
+          upd = f0 * dt;
+        f0o[u] = f0;
+        vn[u] = (ve + upd) + dt * drag;
       }
-      lv[q * IB + ib] = a.c.flen[e[u]] * r[u].vn;
-      if (ib == 0) gid[q] = e[u];
+#pragma unroll
+      for (int u = 0; u < FU; ++u) {
+        const int q = q0 + u * SL;
+        if (q >= n_loc) break;
+        if (q < n_own && live) {
+          local += en_u[u];
+          a.f.ring[sidx<true>(b, a.f.ring_stride, (long long)s0 * E + e[u])] = f0o[u];
+          a.f.v_nxt[sidx<true>(b, a.f.v_stride, e[u])] = vn[u];
+        }
+        lv[q * IB + ib] = fl[u] * vn[u];
+        if (ib == 0) gid[q] = e[u];
+      }
+    }
+  } else {
+    for (int q0 = sl; q0 < n_loc; q0 += SL * FU) {
+      FaceOut<T> r[FU];
+      int e[FU];
+  #pragma unroll
+      for (int u = 0; u < FU; ++u) {
+        const int q = min(q0 + u * SL, n_loc - 1);  // past the end: the last face again, not stored
+        e[u] = q < n_own ? lo + q : a.for_idx[flo + q - n_own];
+        r[u] = face_update<T, false, true, true>(a.f, e[u], b, energy_quad, table, s);
+      }
+  #pragma unroll
+      for (int u = 0; u < FU; ++u) {
+        const int q = q0 + u * SL;
+        if (q >= n_loc) break;
+        if (q < n_own && live) {
+          local += r[u].energy;
+          a.f.ring[sidx<true>(b, a.f.ring_stride, (long long)int(n % 3) * E + e[u])] = r[u].f0;
+          a.f.v_nxt[sidx<true>(b, a.f.v_stride, e[u])] = r[u].vn;
+        }
+        lv[q * IB + ib] = a.c.flen[e[u]] * r[u].vn;
+        if (ib == 0) gid[q] = e[u];
+      }
     }
   }
   __syncthreads();
   const int c0 = a.cell_lo[j], nc = a.cell_lo[j + 1] - c0;
   // CU (cell, instance) pairs per lane loaded together, then updated
-  if (live)
+  if constexpr (REC) {
+    // every lane of a row takes part in the broadcasts; padding instances
+    // of the last group then skip their state
     for (int p0 = threadIdx.x; p0 < nc * IB; p0 += NT * CU) {  // p % IB == ib
       int info[CU], lq[CU][6];
       T R[CU][6], eta_old[CU], r1[CU], r2[CU], area[CU];
 #pragma unroll
       for (int u = 0; u < CU; ++u) {
         const int i = c0 + min(p0 + u * NT, nc * IB - 1) / IB;
-        info[u] = a.c.info[i];
+        const T mine = a.cell_rec[size_t(i) * kRecSlots + ib];
+        info[u] = slot_int(row_slot<0>(mine));
+        const int w0 = slot_int(row_slot<1>(mine)), w1 = slot_int(row_slot<2>(mine)), w2 = slot_int(row_slot<3>(mine));
+        const int wl[3] = {w0, w1, w2};
 #pragma unroll
-        for (int q = 0; q < 6; ++q) {
-          lq[u][q] = a.cyc_loc[q * N + i];
-          R[u][q] = a.c.kite[q * N + i];
+        for (int q = 0; q < 3; ++q) {
+          lq[u][2 * q] = min(wl[q] & 0xffff, a.max_loc - 1);
+          lq[u][2 * q + 1] = min((wl[q] >> 16) & 0xffff, a.max_loc - 1);
         }
-        area[u] = a.c.area[i];
-        eta_old[u] = a.c.eta[sidx<true>(b, a.c.eta_stride, i)];
-        r1[u] = a.c.ring[sidx<true>(b, a.c.ring_stride, (long long)int((n + 2) % 3) * N + i)];
-        r2[u] = a.c.ring[sidx<true>(b, a.c.ring_stride, (long long)int((n + 1) % 3) * N + i)];
+        R[u][0] = row_slot<4>(mine);
+        R[u][1] = row_slot<5>(mine);
+        R[u][2] = row_slot<6>(mine);
+        R[u][3] = row_slot<7>(mine);
+        R[u][4] = row_slot<8>(mine);
+        R[u][5] = row_slot<9>(mine);
+        area[u] = row_slot<10>(mine);
+        eta_old[u] = r1[u] = r2[u] = T(0);
+        if (live) {
+          eta_old[u] = a.c.eta[sidx<true>(b, a.c.eta_stride, i)];
+          r1[u] = a.c.ring[sidx<true>(b, a.c.ring_stride, (long long)int((n + 2) % 3) * N + i)];
+          r2[u] = a.c.ring[sidx<true>(b, a.c.ring_stride, (long long)int((n + 1) % 3) * N + i)];
+        }
       }
+      if (live) {
 #pragma unroll
-      for (int u = 0; u < CU; ++u) {
-        const int p = p0 + u * NT;
-        if (p >= nc * IB) break;
-        const int i = c0 + p / IB;
-        int f[6];
-        T F[6];
+        for (int u = 0; u < CU; ++u) {
+          const int p = p0 + u * NT;
+          if (p >= nc * IB) break;
+          const int i = c0 + p / IB;
+          int f[6];
+          T F[6];
 #pragma unroll
-        for (int q = 0; q < 6; ++q) {
-          const int l = lq[u][q];
-          f[q] = gid[l];
-          const T x = lv[l * IB + ib];
-          F[q] = ((info[u] >> (3 + q)) & 1) ? -x : x;
+          for (int q = 0; q < 6; ++q) {
+            const int l = lq[u][q];
+            f[q] = gid[l];
+            const T x = lv[l * IB + ib];
+            F[q] = ((info[u] >> (3 + q)) & 1) ? -x : x;
+          }
+          const int k = info[u] & 7, db = info[u] >> 3;
+          continuity_loaded<T, true>(a.c, i, b, n, divergence(F, k, area[u]), eta_old[u], r1[u], r2[u]);
+          cell_outputs<T, true>(a.c, b, k, db, f, F, R[u]);
         }
-        const int k = info[u] & 7, db = info[u] >> 3;
-        continuity_loaded<T, true>(a.c, i, b, n, divergence(F, k, area[u]), eta_old[u], r1[u], r2[u]);
-        cell_outputs<T, true>(a.c, b, k, db, f, F, R[u]);
       }
     }
+  } else {
+    if (live)
+      for (int p0 = threadIdx.x; p0 < nc * IB; p0 += NT * CU) {  // p % IB == ib
+        int info[CU], lq[CU][6];
+        T R[CU][6], eta_old[CU], r1[CU], r2[CU], area[CU];
+  #pragma unroll
+        for (int u = 0; u < CU; ++u) {
+          const int i = c0 + min(p0 + u * NT, nc * IB - 1) / IB;
+          info[u] = a.c.info[i];
+  #pragma unroll
+          for (int q = 0; q < 6; ++q) {
+            lq[u][q] = a.cyc_loc[q * N + i];
+            R[u][q] = a.c.kite[q * N + i];
+          }
+          area[u] = a.c.area[i];
+          eta_old[u] = a.c.eta[sidx<true>(b, a.c.eta_stride, i)];
+          r1[u] = a.c.ring[sidx<true>(b, a.c.ring_stride, (long long)int((n + 2) % 3) * N + i)];
+          r2[u] = a.c.ring[sidx<true>(b, a.c.ring_stride, (long long)int((n + 1) % 3) * N + i)];
+        }
+  #pragma unroll
+        for (int u = 0; u < CU; ++u) {
+          const int p = p0 + u * NT;
+          if (p >= nc * IB) break;
+          const int i = c0 + p / IB;
+          int f[6];
+          T F[6];
+  #pragma unroll
+          for (int q = 0; q < 6; ++q) {
+            const int l = lq[u][q];
+            f[q] = gid[l];
+            const T x = lv[l * IB + ib];
+            F[q] = ((info[u] >> (3 + q)) & 1) ? -x : x;
+          }
+          const int k = info[u] & 7, db = info[u] >> 3;
+          continuity_loaded<T, true>(a.c, i, b, n, divergence(F, k, area[u]), eta_old[u], r1[u], r2[u]);
+          cell_outputs<T, true>(a.c, b, k, db, f, F, R[u]);
+        }
+      }
+  }
   // the patch's dissipation partial of each instance: its face slots in order
   red[sl * IB + ib] = local;
   __syncthreads();
@@ -1754,6 +1976,8 @@ struct Ops {
   // patches of the one-kernel step
   int *cell_lo = nullptr, *own_lo = nullptr, *for_ptr = nullptr, *for_idx = nullptr;
   unsigned short *cyc_loc = nullptr;  // [6][N] cycle faces as patch-local indices
+  // operand records of the interleaved step ([E][kRecSlots], [N][kRecSlots]), made on first use
+  T *face_rec = nullptr, *cell_rec = nullptr;
 };
 
 // Device-order operands of the cell-local TRiSK form.
@@ -1763,6 +1987,7 @@ struct CompactHost {
   // patches of the one-kernel step (empty: not available)
   std::vector<int> cell_lo, own_lo, for_ptr, for_idx;        // [P + 1] x 3, [*]
   std::vector<unsigned short> cyc_loc;                       // [6][N]
+  int max_loc = 0;                                           // most faces (owned + foreign) of a patch
 };
 
 uint64_t hilbert3(uint32_t X0, uint32_t X1, uint32_t X2, int bits) {
@@ -1812,12 +2037,14 @@ struct odis_ctx {
   int one_kernel_ok = 0;       // patches for the one-kernel step are built
   int one_kernel = 0;          // use it (ODIS_OPT_ONE_KERNEL; compact form)
   int n_patch = 0;
+  int patch_max_loc = 0;       // most faces (owned + foreign) of a patch
   int patch = 64;              // cells per patch (ODIS_OPT_PATCH_CELLS: 64, 128, 256)
   int step_blocks = 0;         // workgroups of the one-kernel step (ODIS_OPT_STEP_BLOCKS; 0: one per patch)
   int step_lanes = 4;          // lanes per cell of the one-kernel step (ODIS_OPT_STEP_LANES: 2 or 4)
   int il_opt = 2;              // ODIS_OPT_INTERLEAVE: 0 off, 1 on, 2 auto (batches of >= kInterleaveMin)
   int il_unroll = 0;           // ODIS_OPT_IL_UNROLL: 0 default, 1 (1, 1), 2 (2, 1), 3 (4, 2) faces / cells per lane
   int il_order = 0;            // ODIS_OPT_IL_ORDER: 0 patches per group over the XCDs, 1 groups fastest
+  int il_rec = 1;              // ODIS_OPT_IL_RECORDS: operand records of the interleaved step (when they fit)
   int il_Bp = 0;               // instances of the interleaved copy, padded to kInterleaveGroup (0: none yet)
   void *il_buf = nullptr;      // interleaved v[2], eta[2], ring_v, ring_e, ts[2]
   int il_fresh = 0;            // the interleaved copy holds the current state
@@ -2067,6 +2294,7 @@ static std::string build_patches(int N, int E, const std::vector<int> &gi, int P
     h.own_lo[j] = r;
   }
   std::vector<int> loc(E, -1), foreign;
+  h.max_loc = 0;
   for (int j = 0; j < n_patch; j++) {
     const int c_lo = h.cell_lo[j], c_hi = h.cell_lo[j + 1];
     if (c_hi - c_lo > P) return "patch larger than its workgroup";
@@ -2082,6 +2310,7 @@ static std::string build_patches(int N, int E, const std::vector<int> &gi, int P
     foreign.erase(std::unique(foreign.begin(), foreign.end()), foreign.end());
     const int n_loc = (hi - lo) + int(foreign.size());
     if (n_loc > 6 * P) return "patch has more faces than its local store";
+    h.max_loc = std::max(h.max_loc, n_loc);
     for (int f = lo; f < hi; f++) loc[f] = f - lo;
     for (size_t q = 0; q < foreign.size(); q++) loc[foreign[q]] = (hi - lo) + int(q);
     for (int c = c_lo; c < c_hi; c++)
@@ -2102,13 +2331,16 @@ static std::string build_patches(int N, int E, const std::vector<int> &gi, int P
 // (Re)place the patch arrays of the one-kernel step.
 template <typename T>
 static int upload_patches(odis_ctx *ctx, Ops<T> &o, const CompactHost &h) {
-  for (void *p : {(void *)o.cell_lo, (void *)o.own_lo, (void *)o.for_ptr, (void *)o.for_idx, (void *)o.cyc_loc}) {
+  // the cell records hold patch-local indices: both records are made again on next use
+  for (void *p : {(void *)o.cell_lo, (void *)o.own_lo, (void *)o.for_ptr, (void *)o.for_idx, (void *)o.cyc_loc,
+                  (void *)o.face_rec, (void *)o.cell_rec}) {
     if (!p) continue;
     (void)hipFree(p);
     ctx->allocs.erase(std::remove(ctx->allocs.begin(), ctx->allocs.end(), p), ctx->allocs.end());
   }
   o.cell_lo = o.own_lo = o.for_ptr = o.for_idx = nullptr;
   o.cyc_loc = nullptr;
+  o.face_rec = o.cell_rec = nullptr;
   int rc;
   if ((rc = upload(ctx, &o.cell_lo, h.cell_lo))) return rc;
   if ((rc = upload(ctx, &o.own_lo, h.own_lo))) return rc;
@@ -2331,6 +2563,7 @@ static int create_impl(const odis_mesh_desc *m, const odis_params *params, int n
       ctx->one_kernel = ctx->one_kernel_ok;
       ctx->one_kernel_why = pw;
       ctx->n_patch = (N + ctx->patch - 1) / ctx->patch;
+      ctx->patch_max_loc = ch.max_loc;
       ctx->gi_host = gi;
       ctx->cyc_host = ch.cyc;
     } else {
@@ -2917,6 +3150,8 @@ static StepArgs<T> step_args(odis_ctx *ctx, Ops<T> &o) {
   a.cyc_loc = o.cyc_loc;
   a.n_patch = ctx->n_patch;
   a.n_blocks = step_blocks(ctx);
+  a.max_loc = ctx->patch_max_loc;
+  a.face_rec = a.cell_rec = nullptr;
   return a;
 }
 
@@ -2986,6 +3221,25 @@ static int il_alloc(odis_ctx *ctx) {
   return ODIS_OK;
 }
 
+// Operand records for the interleaved step: used when on and the tide has
+// at most kRecFields forcing fields.
+static bool il_rec_wanted(const odis_ctx *ctx) { return ctx->il_rec == 1 && ctx->nfields <= kRecFields; }
+
+template <typename T>
+static int il_records(odis_ctx *ctx, Ops<T> &o) {
+  if (!il_rec_wanted(ctx) || (o.face_rec && o.cell_rec)) return ODIS_OK;
+  const int E = ctx->E, N = ctx->N;
+  int rc;
+  if ((rc = dev_alloc(ctx, &o.face_rec, size_t(E) * kRecSlots))) return rc;
+  if ((rc = dev_alloc(ctx, &o.cell_rec, size_t(N) * kRecSlots))) return rc;
+  hipLaunchKernelGGL(k_pack_face_rec<T>, dim3((E + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, E, N,
+                     ctx->nfields, o.g_idx, o.g_coef, o.fcor, o.farea, o.fdist, o.flen, o.cellS, o.face_rec);
+  hipLaunchKernelGGL(k_pack_cell_rec<T>, dim3((N + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, N, o.info,
+                     o.cyc_loc, o.kite, o.area, o.cell_rec);
+  CK(hipGetLastError());
+  return ODIS_OK;
+}
+
 // the current state between the layouts (v, eta and ts of parity cur, both rings)
 template <typename T>
 static int il_convert(odis_ctx *ctx, Ops<T> &o, bool to_il) {
@@ -3041,6 +3295,10 @@ static StepArgs<T> step_args_il(odis_ctx *ctx, Ops<T> &o) {
   a.c.eta_nxt = q.eta[c ^ 1];
   a.c.ring = q.ring_e;
   a.c.ts = q.ts[c ^ 1];
+  if (il_rec_wanted(ctx)) {
+    a.face_rec = o.face_rec;
+    a.cell_rec = o.cell_rec;
+  }
   return a;
 }
 
@@ -3099,34 +3357,50 @@ static void launch_pair(odis_ctx *ctx, const FaceArgs<T> &fa, const CellArgs<T> 
 // the kernels of every ctx->timing-th step (null: none).
 // One interleaved step launch: FU faces and CU (cell, instance) pairs per
 // lane in flight; the first launch of a chunk has no pending dissipation.
-template <typename T, int FU, int CU, bool GF>
+// Dynamic LDS of k_step_il (its layout is in the kernel's comment).
+template <typename T>
+static size_t il_lds_bytes(int max_loc) {
+  constexpr int IB = kInterleaveGroup, NT = 256;
+  return sizeof(double) * (NT + NT / 64) + size_t(max_loc) * (IB * sizeof(T) + sizeof(int));
+}
+
+template <typename T, int FU, int CU, bool GF, bool REC>
 static void launch_step_il_order(odis_ctx *ctx, const StepArgs<T> &a, const double *table, int s, hipEvent_t *ev) {
   constexpr int IB = kInterleaveGroup;
   const int n_grp = (ctx->B + IB - 1) / IB;
   const dim3 gp = GF ? dim3(unsigned(a.n_blocks) * unsigned(n_grp), 1) : dim3(a.n_blocks, n_grp), bp(256);
+  const size_t lds = il_lds_bytes<T>(a.max_loc);
   if (s == 0) {
     if (ev)
-      hipExtLaunchKernelGGL((k_step_il<T, false, 64, IB, FU, CU, GF>), gp, bp, 0, ctx->stream, ev[0], ev[1], 0, a,
-                            ctx->energy_quad, table, s);
+      hipExtLaunchKernelGGL((k_step_il<T, false, 64, IB, FU, CU, GF, REC>), gp, bp, lds, ctx->stream, ev[0], ev[1],
+                            0, a, ctx->energy_quad, table, s);
     else
-      hipLaunchKernelGGL((k_step_il<T, false, 64, IB, FU, CU, GF>), gp, bp, 0, ctx->stream, a, ctx->energy_quad,
-                         table, s);
+      hipLaunchKernelGGL((k_step_il<T, false, 64, IB, FU, CU, GF, REC>), gp, bp, lds, ctx->stream, a,
+                         ctx->energy_quad, table, s);
   } else {
     if (ev)
-      hipExtLaunchKernelGGL((k_step_il<T, true, 64, IB, FU, CU, GF>), gp, bp, 0, ctx->stream, ev[0], ev[1], 0, a,
-                            ctx->energy_quad, table, s);
+      hipExtLaunchKernelGGL((k_step_il<T, true, 64, IB, FU, CU, GF, REC>), gp, bp, lds, ctx->stream, ev[0], ev[1],
+                            0, a, ctx->energy_quad, table, s);
     else
-      hipLaunchKernelGGL((k_step_il<T, true, 64, IB, FU, CU, GF>), gp, bp, 0, ctx->stream, a, ctx->energy_quad,
-                         table, s);
+      hipLaunchKernelGGL((k_step_il<T, true, 64, IB, FU, CU, GF, REC>), gp, bp, lds, ctx->stream, a,
+                         ctx->energy_quad, table, s);
   }
 }
 
 template <typename T, int FU, int CU>
 static void launch_step_il(odis_ctx *ctx, const StepArgs<T> &a, const double *table, int s, hipEvent_t *ev) {
-  if (ctx->il_order)
-    launch_step_il_order<T, FU, CU, true>(ctx, a, table, s, ev);
-  else
-    launch_step_il_order<T, FU, CU, false>(ctx, a, table, s, ev);
+  const bool rec = a.face_rec != nullptr && a.cell_rec != nullptr;
+  if (ctx->il_order) {
+    if (rec)
+      launch_step_il_order<T, FU, CU, true, true>(ctx, a, table, s, ev);
+    else
+      launch_step_il_order<T, FU, CU, true, false>(ctx, a, table, s, ev);
+  } else {
+    if (rec)
+      launch_step_il_order<T, FU, CU, false, true>(ctx, a, table, s, ev);
+    else
+      launch_step_il_order<T, FU, CU, false, false>(ctx, a, table, s, ev);
+  }
 }
 
 template <typename T>
@@ -3269,6 +3543,7 @@ static int step_t(odis_ctx *ctx, Ops<T> &o, int n_steps) {
       if ((rc = ensure_ts(ctx, o)) || (rc = il_alloc<T>(ctx)) || (rc = il_convert(ctx, o, true))) return rc;
       ctx->il_fresh = 1;
     }
+    if ((rc = il_records(ctx, o))) return rc;
     ctx->std_fresh = 0;
   } else {
     if ((rc = il_out(ctx, o)) || (rc = ensure_ts(ctx, o))) return rc;
@@ -3688,6 +3963,7 @@ static int set_option_impl(odis_ctx *ctx, int option, int value) {
     if (rc) return rc;
     ctx->patch = value;
     ctx->n_patch = (ctx->N + value - 1) / value;
+    ctx->patch_max_loc = h.max_loc;
     return ODIS_OK;
   }
   if (option == ODIS_OPT_SH_CELLS) {
@@ -3710,6 +3986,13 @@ static int set_option_impl(odis_ctx *ctx, int option, int value) {
     CK(hipStreamSynchronize(ctx->stream));
     drop_graphs(ctx);
     ctx->il_order = value;
+    return ODIS_OK;
+  }
+  if (option == ODIS_OPT_IL_RECORDS) {
+    if (value < 0 || value > 1) return fail(ctx, ODIS_E_INVALID, "interleaved records: 0 or 1");
+    CK(hipStreamSynchronize(ctx->stream));
+    drop_graphs(ctx);
+    ctx->il_rec = value;
     return ODIS_OK;
   }
   if (option == ODIS_OPT_IL_UNROLL) {
@@ -3764,6 +4047,7 @@ extern "C" int odis_get_option(odis_ctx *ctx, int option, int *value) {
   else if (option == ODIS_OPT_INTERLEAVE_ACTIVE) *value = il_wanted(ctx) ? 1 : 0;
   else if (option == ODIS_OPT_IL_UNROLL) *value = ctx->il_unroll;
   else if (option == ODIS_OPT_IL_ORDER) *value = ctx->il_order;
+  else if (option == ODIS_OPT_IL_RECORDS) *value = ctx->il_rec;
   else return fail(ctx, ODIS_E_INVALID, "unknown option");
   return ODIS_OK;
 }

@@ -2,8 +2,9 @@
 
 The step kernels have launch shapes that only the device can rank: the
 one-kernel step or the face + cell pair, cells per patch, lanes per cell,
-the interleaved batch layout and the faces / (cell, instance) pairs each
-lane keeps in flight, and HIP-graph replay.  Every candidate gives the same
+the interleaved batch layout with the faces / (cell, instance) pairs each
+lane keeps in flight, its workgroup order and its operand records, and
+HIP-graph replay.  Every candidate gives the same
 state bit for bit (dissipation sums differ by rounding), so tuning never
 changes results.
 
@@ -19,12 +20,14 @@ import json
 import os
 import time
 
-from . import (FP64, OPT_GRAPHS, OPT_IL_ORDER, OPT_IL_UNROLL, OPT_INTERLEAVE, OPT_INTERLEAVE_ACTIVE,
-               OPT_ONE_KERNEL, OPT_PATCH_CELLS, OPT_STEP_LANES, SH_OFF, Context, OdisError)
+from . import (FP64, OPT_GRAPHS, OPT_IL_ORDER, OPT_IL_RECORDS, OPT_IL_UNROLL, OPT_INTERLEAVE,
+               OPT_INTERLEAVE_ACTIVE, OPT_ONE_KERNEL, OPT_PATCH_CELLS, OPT_STEP_LANES, SH_OFF, Context, OdisError)
 
-TUNED = (OPT_ONE_KERNEL, OPT_PATCH_CELLS, OPT_STEP_LANES, OPT_INTERLEAVE, OPT_IL_UNROLL, OPT_IL_ORDER, OPT_GRAPHS)
+TUNED = (OPT_ONE_KERNEL, OPT_PATCH_CELLS, OPT_STEP_LANES, OPT_INTERLEAVE, OPT_IL_UNROLL, OPT_IL_ORDER, OPT_IL_RECORDS,
+         OPT_GRAPHS)
 NAMES = {OPT_ONE_KERNEL: "one_kernel", OPT_PATCH_CELLS: "patch_cells", OPT_STEP_LANES: "step_lanes",
-         OPT_INTERLEAVE: "interleave", OPT_IL_UNROLL: "il_unroll", OPT_IL_ORDER: "il_order", OPT_GRAPHS: "graphs"}
+         OPT_INTERLEAVE: "interleave", OPT_IL_UNROLL: "il_unroll", OPT_IL_ORDER: "il_order",
+         OPT_IL_RECORDS: "il_records", OPT_GRAPHS: "graphs"}
 
 _MEMO = {}
 
@@ -77,9 +80,10 @@ def candidates(ctx):
             base = {OPT_ONE_KERNEL: 1, OPT_PATCH_CELLS: patch, OPT_STEP_LANES: lanes, OPT_GRAPHS: graphs,
                     OPT_INTERLEAVE: 0}
             out.append(dict(base))
-            if patch == 64 and ctx.B >= 2:
-                for unroll, order in itertools.product((0, 1, 2, 3), (0, 1)):
-                    out.append({**base, OPT_INTERLEAVE: 1, OPT_IL_UNROLL: unroll, OPT_IL_ORDER: order})
+            if patch == 64 and lanes == 4 and ctx.B >= 2:  # lanes per cell do not apply to it
+                for unroll, order, rec in itertools.product((0, 1, 2, 3), (0, 1), (1, 0)):
+                    out.append({**base, OPT_INTERLEAVE: 1, OPT_IL_UNROLL: unroll, OPT_IL_ORDER: order,
+                                OPT_IL_RECORDS: rec})
     return out
 
 
@@ -98,7 +102,7 @@ def apply(ctx, options):
     context refuses are left as they are.  Returns the options set."""
     done = {}
     order = (OPT_ONE_KERNEL, OPT_PATCH_CELLS, OPT_STEP_LANES, OPT_INTERLEAVE, OPT_IL_UNROLL, OPT_IL_ORDER,
-             OPT_GRAPHS)
+             OPT_IL_RECORDS, OPT_GRAPHS)
     for o in order:
         if o in options:
             try:

@@ -303,7 +303,12 @@ int odis_request_stop(odis_ctx *ctx);
  *                    each group of 16 instances has its patches spread over
  *                    the XCDs as contiguous ranges; 1 the groups of a patch
  *                    run next to each other on one XCD (its operators read
- *                    from L2 once for all groups).  Same state bit for bit. */
+ *                    from L2 once for all groups).  Same state bit for bit.
+ *  ODIS_OPT_IL_RECORDS operands of the interleaved step: 1 (default) from
+ *                    per-face and per-cell records that each group of 16
+ *                    lanes loads once and shares by row broadcasts (tides
+ *                    with at most 4 forcing fields), 0 every lane loads
+ *                    every operand.  Same state bit for bit. */
 #define ODIS_OPT_GRAPHS     1
 #define ODIS_OPT_COMPACT    2
 #define ODIS_OPT_ONE_KERNEL 3
@@ -315,6 +320,7 @@ int odis_request_stop(odis_ctx *ctx);
 #define ODIS_OPT_INTERLEAVE_ACTIVE 9
 #define ODIS_OPT_IL_UNROLL 10
 #define ODIS_OPT_IL_ORDER 11
+#define ODIS_OPT_IL_RECORDS 12
 int odis_set_option(odis_ctx *ctx, int option, int value);
 int odis_get_option(odis_ctx *ctx, int option, int *value);
 /* Timing for benchmarks: with stride T > 0, the kernels of every step n of

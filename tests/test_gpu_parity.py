@@ -703,17 +703,21 @@ def test_one_kernel_step_matches_face_cell_pair(product, precision):
             _assert_same_state(one, pair, (0, 1), (level, "patch", cells, lanes, blocks))
 
 
-@pytest.mark.parametrize("precision,n,shell,level", [(0, 20, False, 5), (1, 33, False, 5), (1, 20, True, 5),
-                                                   (1, 17, False, 7)])
-def test_interleaved_batch_step(product, precision, n, shell, level):
+@pytest.mark.parametrize("precision,n,shell,level,records,tide", [
+    (0, 20, False, 5, 1, "FULL"), (1, 33, False, 5, 1, "FULL"), (1, 20, True, 5, 1, "FULL"),
+    (1, 17, False, 7, 1, "FULL"), (1, 33, False, 5, 0, "FULL"), (0, 20, False, 5, 0, "FULL"),
+    (1, 18, False, 5, 1, "FULL2"), (1, 18, False, 5, 1, "PLANET")])
+def test_interleaved_batch_step(product, precision, n, shell, level, records, tide):
     """A batch stepping with its state held interleaved (16 instances per
     workgroup, the last group padded) leaves the same state bit for bit as
     the instance-major step, across state reads between calls, a set_state
     of one instance in mid-run, and graph replays; dissipation and its
-    series agree to 1e-12 (another grouping of the face terms).  With shells
-    coupled through the batched split-BF16 products the products read and
-    write the interleaved eta; a context with another SH mode does not
-    interleave.  40962 cells: the size from which a batch interleaves by
+    series agree to 1e-12 (another grouping of the face terms).  With the
+    operand records (the default for tides of at most 4 forcing fields:
+    FULL 3, PLANET 4) and without them (records off, or FULL2's 7 fields).
+    With shells coupled through the batched split-BF16 products the products
+    read and write the interleaved eta; a context with another SH mode does
+    not interleave.  40962 cells: the size from which a batch interleaves by
     default."""
     G = product
     g, _ = generated_grid(level)
@@ -722,12 +726,15 @@ def test_interleaved_batch_step(product, precision, n, shell, level):
         mesh = G.Mesh(g, 232.1e3)
         ps, beta = _shell_batch(G, mesh, n, 8)
     else:
+        extra = dict(semimajor=671100000.0) if tide == "PLANET" else {}
         ps = [G.make_params(time_step=30.0 if level == 5 else 14.0, h=float(10 ** rng.uniform(3.5, 4.6)),
-                            alpha=float(10 ** rng.uniform(-10, -7)), tide=G.FULL,
-                            obliquity=float(np.radians(rng.uniform(0, 1))))[0] for _ in range(n)]
+                            alpha=float(10 ** rng.uniform(-10, -7)), tide=getattr(G, tide),
+                            obliquity=float(np.radians(rng.uniform(0, 1))), **extra)[0] for _ in range(n)]
         mesh = G.Mesh(g, ps[0].radius)
     il = G.Context(mesh, ps, precision=precision)
     im = G.Context(mesh, ps, precision=precision)
+    il.set_option(G.OPT_IL_RECORDS, records)
+    assert il.option(G.OPT_IL_RECORDS) == records
     if shell:
         for c in (il, im):
             c.set_sh(8, beta, mode=G.SH_MFMA)
