@@ -327,6 +327,7 @@ struct StepArgs {
   int max_loc;                  // most faces (owned + foreign) of a patch
   const T *face_rec;            // [E][16] operand records of k_step_il (kRecSlots), or null
   const T *cell_rec;            // [N][16]
+  int il_band;                  // k_step_il in band order: groups per band
 };
 
 // One patch of the one-kernel step; returns this lane's dissipation
@@ -530,12 +531,14 @@ __global__ __launch_bounds__(kBlock) void k_pack_cell_rec(int N, const int *__re
 // REC: operands from the records (a.face_rec, a.cell_rec; at most
 // kRecFields forcing fields), else from the operand arrays.  The per-face
 // arithmetic is face_update's, written out in the same order.
-// GF: one-dimensional grid of n_blocks x groups workgroups, each XCD a
-// contiguous range of (patch, group) tiles with the groups fastest, so that
-// the groups of a patch (which read its operators) and neighbouring patches
-// (which read each other's boundary state) run on one XCD close in time;
-// else a grid (n_blocks, groups) with each group's patches spread as
-// contiguous ranges over the XCDs.
+// GF: the groups in bands of a.il_band; a one-dimensional grid of tiles
+// (band, patch, group of the band), each XCD a contiguous range of them, so
+// that the band's groups of a patch (which read its operands) and
+// neighbouring patches (which read each other's boundary state) run on one
+// XCD close in time, while the XCD streams through the state of only a
+// band's groups; a band of all groups is groups-fastest order.  Else a grid
+// (n_blocks, groups) with each group's patches spread as contiguous ranges
+// over the XCDs.
 template <typename T, bool PENDING, int P, int IB, int FU, int CU, bool GF = false, bool REC = false>
 __global__ __launch_bounds__(256) void k_step_il(StepArgs<T> a, int energy_quad, const double *table, int s) {
   static_assert(!REC || IB == 16, "a record is shared through one 16-lane DPP row");
@@ -550,9 +553,12 @@ __global__ __launch_bounds__(256) void k_step_il(StepArgs<T> a, int energy_quad,
   const int n_grp = (B + IB - 1) / IB;
   int j, grp;
   if (GF) {
-    const int t = xcd_tile(blockIdx.x, a.n_blocks * n_grp);
-    j = t / n_grp;
-    grp = t - j * n_grp;
+    const int GB = a.il_band, per_band = a.n_blocks * GB;
+    const int t = xcd_tile(blockIdx.x, per_band * ((n_grp + GB - 1) / GB));
+    const int band = t / per_band, r = t - band * per_band;
+    j = r / GB;
+    grp = band * GB + (r - j * GB);
+    if (grp >= n_grp) return;  // whole workgroup: a group the last band does not have
   } else {
     j = xcd_tile(blockIdx.x, a.n_blocks);
     grp = blockIdx.y;
@@ -2043,7 +2049,7 @@ struct odis_ctx {
   int step_lanes = 4;          // lanes per cell of the one-kernel step (ODIS_OPT_STEP_LANES: 2 or 4)
   int il_opt = 2;              // ODIS_OPT_INTERLEAVE: 0 off, 1 on, 2 auto (batches of >= kInterleaveMin)
   int il_unroll = 0;           // ODIS_OPT_IL_UNROLL: 0 default, 1 (1, 1), 2 (2, 1), 3 (4, 2) faces / cells per lane
-  int il_order = 0;            // ODIS_OPT_IL_ORDER: 0 patches per group over the XCDs, 1 groups fastest
+  int il_order = 0;            // ODIS_OPT_IL_ORDER: 0 patches per group over the XCDs, 1 groups fastest, 2/4/8 bands
   int il_rec = 1;              // ODIS_OPT_IL_RECORDS: operand records of the interleaved step (when they fit)
   int il_Bp = 0;               // instances of the interleaved copy, padded to kInterleaveGroup (0: none yet)
   void *il_buf = nullptr;      // interleaved v[2], eta[2], ring_v, ring_e, ts[2]
@@ -3152,6 +3158,7 @@ static StepArgs<T> step_args(odis_ctx *ctx, Ops<T> &o) {
   a.n_blocks = step_blocks(ctx);
   a.max_loc = ctx->patch_max_loc;
   a.face_rec = a.cell_rec = nullptr;
+  a.il_band = 1;
   return a;
 }
 
@@ -3299,6 +3306,8 @@ static StepArgs<T> step_args_il(odis_ctx *ctx, Ops<T> &o) {
     a.face_rec = o.face_rec;
     a.cell_rec = o.cell_rec;
   }
+  const int n_grp = (ctx->B + kInterleaveGroup - 1) / kInterleaveGroup;
+  a.il_band = ctx->il_order == 1 ? n_grp : std::max(1, std::min(ctx->il_order, n_grp));
   return a;
 }
 
@@ -3368,7 +3377,9 @@ template <typename T, int FU, int CU, bool GF, bool REC>
 static void launch_step_il_order(odis_ctx *ctx, const StepArgs<T> &a, const double *table, int s, hipEvent_t *ev) {
   constexpr int IB = kInterleaveGroup;
   const int n_grp = (ctx->B + IB - 1) / IB;
-  const dim3 gp = GF ? dim3(unsigned(a.n_blocks) * unsigned(n_grp), 1) : dim3(a.n_blocks, n_grp), bp(256);
+  const unsigned n_band = unsigned((n_grp + a.il_band - 1) / a.il_band);
+  const dim3 gp = GF ? dim3(unsigned(a.n_blocks) * unsigned(a.il_band) * n_band, 1) : dim3(a.n_blocks, n_grp),
+             bp(256);
   const size_t lds = il_lds_bytes<T>(a.max_loc);
   if (s == 0) {
     if (ev)
@@ -3982,7 +3993,8 @@ static int set_option_impl(odis_ctx *ctx, int option, int value) {
     return ODIS_OK;
   }
   if (option == ODIS_OPT_IL_ORDER) {
-    if (value < 0 || value > 1) return fail(ctx, ODIS_E_INVALID, "interleaved order: 0 or 1");
+    if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8)
+      return fail(ctx, ODIS_E_INVALID, "interleaved order: 0, 1 (all groups), or bands of 2, 4 or 8 groups");
     CK(hipStreamSynchronize(ctx->stream));
     drop_graphs(ctx);
     ctx->il_order = value;

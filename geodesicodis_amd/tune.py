@@ -81,7 +81,7 @@ def candidates(ctx):
                     OPT_INTERLEAVE: 0}
             out.append(dict(base))
             if patch == 64 and lanes == 4 and ctx.B >= 2:  # lanes per cell do not apply to it
-                for unroll, order, rec in itertools.product((0, 1, 2, 3), (0, 1), (1, 0)):
+                for unroll, order, rec in itertools.product((0, 2), (0, 1, 2, 4), (1, 0)):
                     out.append({**base, OPT_INTERLEAVE: 1, OPT_IL_UNROLL: unroll, OPT_IL_ORDER: order,
                                 OPT_IL_RECORDS: rec})
     return out

@@ -302,8 +302,11 @@ int odis_request_stop(odis_ctx *ctx);
  *  ODIS_OPT_IL_ORDER  workgroup order of the interleaved step: 0 (default)
  *                    each group of 16 instances has its patches spread over
  *                    the XCDs as contiguous ranges; 1 the groups of a patch
- *                    run next to each other on one XCD (its operators read
- *                    from L2 once for all groups).  Same state bit for bit.
+ *                    run next to each other on one XCD (its operands read
+ *                    from L2 once for all groups); 2, 4 or 8 the same for
+ *                    bands of that many groups, band after band (each XCD
+ *                    streams through the state of fewer groups at a time).
+ *                    Same state bit for bit.
  *  ODIS_OPT_IL_RECORDS operands of the interleaved step: 1 (default) from
  *                    per-face and per-cell records that each group of 16
  *                    lanes loads once and shares by row broadcasts (tides
