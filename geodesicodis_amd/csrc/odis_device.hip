@@ -202,13 +202,11 @@ __device__ __forceinline__ void cell_outputs(const CellArgs<T> &a, int b, int k,
 // Continuity + AB3 of cell i: eta^{n+1} = eta^n + dt AB3(h Div v^{n+1})
 // (src/updateEta.cpp, src/temporalOperators.cpp:36-47), history slot n % 3.
 // eta_old = eta^n, r1 / r2 = history slots (n+2) % 3 and (n+1) % 3, already
-// loaded.
+// loaded; h, dt and the AB3 flag are the instance's (continuity_loaded reads
+// them).
 template <typename T, bool IL = false>
-__device__ __forceinline__ T continuity_loaded(const CellArgs<T> &a, int i, int b, long long n, T div,
-                                               T eta_old, T r1, T r2) {
-  const double *ip = a.inst + b * 8;
-  const T h = T(ip[5]), dt = T(ip[2]);
-  const bool ab3 = (n >= 2) || (ip[3] != 0.0);
+__device__ __forceinline__ T continuity_with(const CellArgs<T> &a, int i, int b, long long n, T div, T eta_old,
+                                             T r1, T r2, T h, T dt, bool ab3) {
   const T f0 = h * div;
   T upd;
   if (ab3)
@@ -219,6 +217,14 @@ __device__ __forceinline__ T continuity_loaded(const CellArgs<T> &a, int i, int 
   const T eta_new = eta_old + upd;
   a.eta_nxt[sidx<IL>(b, a.eta_stride, i)] = eta_new;
   return eta_new;
+}
+
+template <typename T, bool IL = false>
+__device__ __forceinline__ T continuity_loaded(const CellArgs<T> &a, int i, int b, long long n, T div,
+                                               T eta_old, T r1, T r2) {
+  const double *ip = a.inst + b * 8;
+  const bool ab3 = (n >= 2) || (ip[3] != 0.0);
+  return continuity_with<T, IL>(a, i, b, n, div, eta_old, r1, r2, T(ip[5]), T(ip[2]), ab3);
 }
 
 template <typename T>
@@ -475,6 +481,16 @@ __device__ __forceinline__ double row_slot(double x) {
   return __longlong_as_double((long long)((static_cast<unsigned long long>(hi) << 32) | lo));
 }
 
+// u0 += c_K S_K(c0), u1 += c_K S_K(c1) from a face record, for field K of
+// the tide's nf (nf is uniform, so the row is whole for the broadcasts)
+template <int K, typename T>
+__device__ __forceinline__ void rec_forcing(T mine, const T *ck, int nf, T &u0, T &u1) {
+  if (K < nf) {
+    u0 += ck[K] * row_slot<8 + K>(mine);
+    u1 += ck[K] * row_slot<12 + K>(mine);
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pack_face_rec(int E, int N, int nf, const int *__restrict__ g_idx,
                                                           const T *__restrict__ g_coef, const T *__restrict__ fcor,
@@ -583,22 +599,51 @@ __global__ __launch_bounds__(256) void k_step_il(StepArgs<T> a, int energy_quad,
 #pragma unroll
     for (int k = 0; k < kRecFields; ++k) ck[k] = k < nf ? T(coef[bp * 8 + k]) : T(0);
     const int s0 = int(n % 3), s1 = int((n + 2) % 3), s2 = int((n + 1) % 3);
+    // the face ids and records of a lane's next FU faces are loaded one
+    // iteration ahead, so that their latency overlaps this iteration's work
+    // (past the end: the last face again, not stored)
+    int e_n[FU];
+    T rec_n[FU];
+    if (n_loc > 0) {
+#pragma unroll
+      for (int u = 0; u < FU; ++u) {
+        const int q = min(sl + u * SL, n_loc - 1);
+        e_n[u] = q < n_own ? lo + q : a.for_idx[flo + q - n_own];
+        rec_n[u] = a.face_rec[size_t(e_n[u]) * kRecSlots + ib];
+      }
+    }
     for (int q0 = sl; q0 < n_loc; q0 += SL * FU) {
       T vn[FU], f0o[FU], fl[FU];
       double en_u[FU];
       int e[FU];
+      T rec[FU];
 #pragma unroll
       for (int u = 0; u < FU; ++u) {
-        const int q = min(q0 + u * SL, n_loc - 1);  // past the end: the last face again, not stored
-        e[u] = q < n_own ? lo + q : a.for_idx[flo + q - n_own];
-        const T mine = a.face_rec[size_t(e[u]) * kRecSlots + ib];
+        e[u] = e_n[u];
+        rec[u] = rec_n[u];
+      }
+      if (q0 + SL * FU < n_loc) {  // the same for the whole row
+#pragma unroll
+        for (int u = 0; u < FU; ++u) {
+          const int q = min(q0 + SL * FU + u * SL, n_loc - 1);
+          e_n[u] = q < n_own ? lo + q : a.for_idx[flo + q - n_own];
+          rec_n[u] = a.face_rec[size_t(e_n[u]) * kRecSlots + ib];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < FU; ++u) {
+        const T mine = rec[u];
         const int c0 = min(unsigned(slot_int(row_slot<0>(mine))), unsigned(N - 1));
         const int c1 = min(unsigned(slot_int(row_slot<1>(mine))), unsigned(N - 1));
         const T gc0 = row_slot<2>(mine), gc1 = row_slot<3>(mine), fcor = row_slot<4>(mine);
         const T farea = row_slot<5>(mine), fdist = row_slot<6>(mine);
         fl[u] = row_slot<7>(mine);
-        const T S0[kRecFields] = {row_slot<8>(mine), row_slot<9>(mine), row_slot<10>(mine), row_slot<11>(mine)};
-        const T S1[kRecFields] = {row_slot<12>(mine), row_slot<13>(mine), row_slot<14>(mine), row_slot<15>(mine)};
+        // forcing sums at the two cells, broadcasting only the fields the tide has
+        T u0 = T(0), u1 = T(0);
+        rec_forcing<0>(mine, ck, nf, u0, u1);
+        rec_forcing<1>(mine, ck, nf, u0, u1);
+        rec_forcing<2>(mine, ck, nf, u0, u1);
+        rec_forcing<3>(mine, ck, nf, u0, u1);
         // face_update<T, false, true, true>, operation for operation
         const T vt = (a.f.ts[sidx<true>(b, a.f.ts_stride, e[u])] -
                       a.f.ts[sidx<true>(b, a.f.ts_stride, E + e[u])]) / fdist;
@@ -609,13 +654,6 @@ __global__ __launch_bounds__(256) void k_step_il(StepArgs<T> a, int energy_quad,
         const T ge = gc0 * a.f.eta[sidx<true>(b, a.f.eta_stride, c0)] + gc1 * a.f.eta[sidx<true>(b, a.f.eta_stride, c1)];
         T f0 = -g * ge + fcor * vt;
         if (a.f.quad_drag) f0 -= qd * sqrt(sp) * ve;
-        T u0 = T(0), u1 = T(0);
-#pragma unroll
-        for (int k = 0; k < kRecFields; ++k)
-          if (k < nf) {
-            u0 += ck[k] * S0[k];
-            u1 += ck[k] * S1[k];
-          }
         const T fu = gc0 * u0 + gc1 * u1;
         const T drag = -alpha * ve + fu;
         T upd;
@@ -671,6 +709,9 @@ __global__ __launch_bounds__(256) void k_step_il(StepArgs<T> a, int energy_quad,
   if constexpr (REC) {
     // every lane of a row takes part in the broadcasts; padding instances
     // of the last group then skip their state
+    const double *ipc = a.c.inst + min(b, B - 1) * 8;
+    const T h_c = T(ipc[5]), dt_c = T(ipc[2]);
+    const bool ab3_c = (n >= 2) || (ipc[3] != 0.0);
     for (int p0 = threadIdx.x; p0 < nc * IB; p0 += NT * CU) {  // p % IB == ib
       int info[CU], lq[CU][6];
       T R[CU][6], eta_old[CU], r1[CU], r2[CU], area[CU];
@@ -716,7 +757,8 @@ __global__ __launch_bounds__(256) void k_step_il(StepArgs<T> a, int energy_quad,
             F[q] = ((info[u] >> (3 + q)) & 1) ? -x : x;
           }
           const int k = info[u] & 7, db = info[u] >> 3;
-          continuity_loaded<T, true>(a.c, i, b, n, divergence(F, k, area[u]), eta_old[u], r1[u], r2[u]);
+          continuity_with<T, true>(a.c, i, b, n, divergence(F, k, area[u]), eta_old[u], r1[u], r2[u], h_c, dt_c,
+                                   ab3_c);
           cell_outputs<T, true>(a.c, b, k, db, f, F, R[u]);
         }
       }
