@@ -1934,8 +1934,46 @@ __global__ __launch_bounds__(256) void k_sh_syn_mfma(int N, int NT16, int KB, co
     }
   }
   // result tile (C^T Y): rows = instances j0 + 16 it + 4 g + r, column =
-  // cell 16 t + r16, so 16 lanes store 16 consecutive cells of a row; the
-  // base values are loaded together first (clamped into range)
+  // cell 16 t + r16.  Held interleaved (FP32), a lane's four rows are four
+  // consecutive instances of one group at one cell: 16 contiguous bytes,
+  // loaded and stored whole.
+  if constexpr (IL && sizeof(T) == 4) {
+    const int last_grp = (nb - 1) / 16;
+    float4 add4[CT][NIT];
+#pragma unroll
+    for (int u = 0; u < CT; ++u)
+#pragma unroll
+      for (int it = 0; it < NIT; ++it) {
+        const int c = min(16 * (t0 + u) + r16, N - 1), grp = min((j0 + it * 16) / 16, last_grp);
+        add4[u][it] = base ? *reinterpret_cast<const float4 *>(base + (size_t(grp) * stride + c) * 16 + 4 * g)
+                           : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+#pragma unroll
+    for (int u = 0; u < CT; ++u) {
+      const int t = t0 + u, c = 16 * t + r16;
+      if (t >= NT16 || c >= N) continue;
+#pragma unroll
+      for (int it = 0; it < NIT; ++it) {
+        const int j = j0 + it * 16 + 4 * g;
+        if (j >= nb) continue;
+        const float4 a4 = add4[u][it];
+        const float4 v = make_float4(float(double(a4.x) + double(acc[u][it][0])),
+                                     float(double(a4.y) + double(acc[u][it][1])),
+                                     float(double(a4.z) + double(acc[u][it][2])),
+                                     float(double(a4.w) + double(acc[u][it][3])));
+        float *dst = reinterpret_cast<float *>(out) + (size_t(j / 16) * stride + c) * 16 + 4 * g;
+        if (j + 3 < nb) {
+          *reinterpret_cast<float4 *>(dst) = v;
+        } else {
+          dst[0] = v.x;
+          if (j + 1 < nb) dst[1] = v.y;
+          if (j + 2 < nb) dst[2] = v.z;
+        }
+      }
+    }
+    return;
+  }
+  // the base values are loaded together first (clamped into range)
   T add[CT][NIT][4];
 #pragma unroll
   for (int u = 0; u < CT; ++u)
