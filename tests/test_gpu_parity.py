@@ -703,6 +703,35 @@ def test_one_kernel_step_matches_face_cell_pair(product, precision):
             _assert_same_state(one, pair, (0, 1), (level, "patch", cells, lanes, blocks))
 
 
+def test_interleaved_records_follow_patch_changes(product):
+    """The interleaved step's operand records hold patch-local face indices:
+    they are made again after the patches change.  A batch stepping
+    interleaved switches to 128-cell patches (it then steps instance-major)
+    and back to 64, and steps interleaved again; its state stays bit for
+    bit that of an instance-major copy throughout, with the records on, off
+    and on again."""
+    G = product
+    g, _ = generated_grid(5)
+    rng = np.random.default_rng(11)
+    ps = [G.make_params(time_step=30.0, h=float(10 ** rng.uniform(3.5, 4.6)),
+                        alpha=float(10 ** rng.uniform(-10, -7)))[0] for _ in range(20)]
+    mesh = G.Mesh(g, ps[0].radius)
+    il = G.Context(mesh, ps, precision=G.FP32)
+    im = G.Context(mesh, ps, precision=G.FP32)
+    il.set_option(G.OPT_INTERLEAVE, 1)
+    im.set_option(G.OPT_INTERLEAVE, 0)
+    insts = (0, 7, 19)
+    for cells, rec, active in ((64, 1, 1), (128, 1, 0), (64, 1, 1), (64, 0, 1), (64, 1, 1)):
+        il.set_option(G.OPT_PATCH_CELLS, cells)
+        il.set_option(G.OPT_IL_RECORDS, rec)
+        assert il.option(G.OPT_INTERLEAVE_ACTIVE) == active
+        il.step(37)
+        im.step(37)
+        _assert_same_state(il, im, insts, ("patch", cells, "records", rec))
+    il.close()
+    im.close()
+
+
 @pytest.mark.parametrize("precision,n,shell,level,records,tide", [
     (0, 20, False, 5, 1, "FULL"), (1, 33, False, 5, 1, "FULL"), (1, 20, True, 5, 1, "FULL"),
     (1, 17, False, 7, 1, "FULL"), (1, 33, False, 5, 0, "FULL"), (0, 20, False, 5, 0, "FULL"),
