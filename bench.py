@@ -490,6 +490,8 @@ def shell_orbit_run(G, args):
     p, steps_per_orbit, beta = shell_setup(G)
     mesh = G.Mesh(grid, p.radius)
     forms = (("fp64", G.FP64, G.SH_RECUR), ("fp32", G.FP32, G.SH_RECUR), ("fp32_mfma", G.FP32, G.SH_MFMA))
+    wanted = {"fp64"} | set(args.orbit_forms.split(","))
+    forms = tuple(f for f in forms if f[0] in wanted)
     ctxs = {}
     for name, prec, mode in forms:
         c = G.Context(mesh, p, precision=prec)
@@ -508,7 +510,7 @@ def shell_orbit_run(G, args):
         ref = ctxs["fp64"].get_state()
         d_ref = float(ctxs["fp64"].dissipation()[0])
         row = {"orbit": k + 1, "steps": (k + 1) * steps_per_orbit, "dissipation_fp64_W_m2": d_ref}
-        for name in ("fp32", "fp32_mfma"):
+        for name in (f for f in ("fp32", "fp32_mfma") if f in ctxs):
             st = ctxs[name].get_state()
             d = float(ctxs[name].dissipation()[0])
             row[name] = {"eta": rel_l2(st["eta"], ref["eta"]), "v": rel_l2(st["v"], ref["v"]),
@@ -570,6 +572,8 @@ def main():
                          "and FP32 + split-BF16 MFMA forms against FP64 after whole orbits")
     ap.add_argument("--orbits", type=float, default=1.0,
                     help="orbits of the shell-orbit workload (fast forms against FP64 after whole orbits)")
+    ap.add_argument("--orbit-forms", default="fp32,fp32_mfma",
+                    help="fast forms the shell-orbit workload steps beside FP64 (comma-separated: fp32, fp32_mfma)")
     ap.add_argument("--no-converged", action="store_true",
                     help="skip the time-to-converged leg (learned warm start vs rest, held-out instances)")
     ap.add_argument("--converged-instances", type=int, default=64,
