@@ -500,7 +500,15 @@ def shell_orbit_run(G, args):
     n_orbits = max(1, int(round(args.orbits)))
     rows, walls = [], {k: 0.0 for k in ctxs}
     chunk = steps_per_orbit // 8
-    for k in range(n_orbits):
+    keys = ("v", "dvdt", "eta", "detadt")
+    k0 = 0
+    if args.orbit_resume:  # continue runs saved after whole orbits
+        for name, c in ctxs.items():
+            st = {x: np.load(os.path.join(args.orbit_resume, "%s_%s.npy" % (name, x))) for x in keys}
+            it = int(np.load(os.path.join(args.orbit_resume, "%s_iter.npy" % name)))
+            c.set_state(0, st["v"], st["dvdt"], st["eta"], st["detadt"], it)
+            k0 = it // steps_per_orbit
+    for k in range(k0, k0 + n_orbits):
         for name, c in ctxs.items():
             t0 = time.perf_counter()
             for q in range(8):
@@ -516,10 +524,17 @@ def shell_orbit_run(G, args):
             row[name] = {"eta": rel_l2(st["eta"], ref["eta"]), "v": rel_l2(st["v"], ref["v"]),
                          "dissipation": abs(d - d_ref) / abs(d_ref)}
         rows.append(row)
+    if args.orbit_save:
+        os.makedirs(args.orbit_save, exist_ok=True)
+        for name, c in ctxs.items():
+            st = c.get_state()
+            for x in keys:
+                np.save(os.path.join(args.orbit_save, "%s_%s.npy" % (name, x)), st[x])
+            np.save(os.path.join(args.orbit_save, "%s_iter.npy" % name), np.array(st["iter"]))
     for c in ctxs.values():
         c.close()
-    return {"workload": "shell, 163842 cells, degree 32, %d orbit(s) of %d steps, rest start" % (n_orbits,
-                                                                                              steps_per_orbit),
+    return {"workload": "shell, 163842 cells, degree 32, orbits %d-%d of %d steps, rest start" % (
+                k0 + 1, k0 + n_orbits, steps_per_orbit),
             "errors_vs_fp64": rows, "wall_s": walls}
 
 
@@ -572,6 +587,10 @@ def main():
                          "and FP32 + split-BF16 MFMA forms against FP64 after whole orbits")
     ap.add_argument("--orbits", type=float, default=1.0,
                     help="orbits of the shell-orbit workload (fast forms against FP64 after whole orbits)")
+    ap.add_argument("--orbit-save", default="",
+                    help="shell-orbit: save each form's state after the last orbit into this directory")
+    ap.add_argument("--orbit-resume", default="",
+                    help="shell-orbit: continue from the states saved in this directory")
     ap.add_argument("--orbit-forms", default="fp32,fp32_mfma",
                     help="fast forms the shell-orbit workload steps beside FP64 (comma-separated: fp32, fp32_mfma)")
     ap.add_argument("--no-converged", action="store_true",
