@@ -3027,6 +3027,16 @@ static void launch_analysis_pair(odis_ctx *ctx, int nb, const T *x, long long st
 // registers.  The analysis ranges depend on N only, and every kernel sums
 // an instance's products in the same order whatever NIT, so an instance's
 // results do not depend on the rest of its batch.
+template <typename T, int NIT, int CT, bool IL>
+static void launch_syn_mfma(odis_ctx *ctx, int nb, const T *base, long long stride, T *out, hipEvent_t e0,
+                            hipEvent_t e1) {
+  const int NT16 = (ctx->N + 15) / 16, n_it = (nb + 15) / 16;
+  const dim3 grid((NT16 + 4 * CT - 1) / (4 * CT), (n_it + NIT - 1) / NIT);
+  hipExtLaunchKernelGGL((k_sh_syn_mfma<T, NIT, CT, IL>), grid, dim3(256), 0, ctx->stream, e0, e1, 0, ctx->N, NT16,
+                        ctx->sh_Kp / 32, ctx->sh_Ys, reinterpret_cast<const uint4 *>(ctx->sh_Cs), nb, base, stride,
+                        out);
+}
+
 template <typename T, int NIT, bool IL>
 static void launch_mfma_t(odis_ctx *ctx, bool ana, int nb, const T *x, const T *base, long long stride, T *out,
                           hipEvent_t e0, hipEvent_t e1) {
@@ -3043,11 +3053,18 @@ static void launch_mfma_t(odis_ctx *ctx, bool ana, int nb, const T *x, const T *
     const dim3 grid(ctx->sh_rng, (KT + 4 * W - 1) / (4 * W), (n_it + NIT - 1) / NIT);
     hipExtLaunchKernelGGL((k_sh_ana_mfma<NIT, W>), grid, dim3(256), 0, ctx->stream, e0, e1, 0, ctx->sh_K, KT, NB32,
                           ctx->sh_qpr, ctx->sh_rng, ctx->sh_Ya, ctx->sh_Xa, nb, ctx->sh_b);
+  } else if constexpr (NIT == 5) {
+    // cell tiles per wave by grid size (measured on the uneven batch's
+    // groups: 4 at 163842 cells, 3 at 40962, 2 at 2562 cells, where fewer
+    // tiles per wave keep more workgroups)
+    if (NT16 >= 8192)
+      launch_syn_mfma<T, NIT, 4, IL>(ctx, nb, base, stride, out, e0, e1);
+    else if (NT16 >= 1024)
+      launch_syn_mfma<T, NIT, 3, IL>(ctx, nb, base, stride, out, e0, e1);
+    else
+      launch_syn_mfma<T, NIT, 2, IL>(ctx, nb, base, stride, out, e0, e1);
   } else {
-    const dim3 grid((NT16 + 4 * CT - 1) / (4 * CT), (n_it + NIT - 1) / NIT);
-    hipExtLaunchKernelGGL((k_sh_syn_mfma<T, NIT, CT, IL>), grid, dim3(256), 0, ctx->stream, e0, e1, 0, N, NT16,
-                          ctx->sh_Kp / 32, ctx->sh_Ys, reinterpret_cast<const uint4 *>(ctx->sh_Cs), nb, base,
-                          stride, out);
+    launch_syn_mfma<T, NIT, CT, IL>(ctx, nb, base, stride, out, e0, e1);
   }
 }
 
