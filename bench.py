@@ -719,7 +719,8 @@ def main():
                 sh = (SH_DEGREE, beta, G.SH_RECUR if args.sh_basis == "recur" else
                       (G.SH_FP64 if prec == G.FP64 else G.SH_SPLIT_BF16))
             t0 = time.perf_counter()
-            tuned[prec] = T.tune(mesh, [p], precision=prec, sh=sh,
+            # a timed region that stamps kernels with events replays no graphs
+            tuned[prec] = T.tune(mesh, [p], precision=prec, sh=sh, graphs=args.timing_stride == 0,
                                  log=lambda m: print("bench: " + m, file=sys.stderr, flush=True))
             tuned[prec]["tune_s"] = time.perf_counter() - t0
         return tuned[prec]

@@ -67,12 +67,14 @@ def _device_name():
         return "gpu"
 
 
-def candidates(ctx):
-    """Option settings (dicts) valid for the context."""
+def candidates(ctx, graphs=True):
+    """Option settings (dicts) valid for the context; graphs=False leaves
+    out graph replay (for runs that stamp kernels with events, which replay
+    no graphs)."""
     out = []
     defaults = {o: ctx.option(o) for o in TUNED}
     one_ok = defaults[OPT_ONE_KERNEL] == 1 or _accepts(ctx, OPT_ONE_KERNEL, 1)
-    for graphs in (0, 1):
+    for graphs in ((0, 1) if graphs else (0,)):
         out.append({OPT_ONE_KERNEL: 0, OPT_GRAPHS: graphs, OPT_INTERLEAVE: 0})
         if not one_ok:
             continue
@@ -124,14 +126,15 @@ def _time(ctx, steps, warmup, repeats):
 
 
 def tune(mesh, params, precision=FP64, sh=None, steps=100, warmup=20, repeats=3, margin=0.02, use_cache=True,
-         log=None):
+         log=None, graphs=True):
     """The fastest launch options for stepping this batch; sh = (l_max, beta,
-    mode) of the SH coupling or None.  Returns a dict: "options" (name ->
-    value, to pass to apply() through option_ids()), "ms_per_step" tuned and
-    "default_ms_per_step", "speedup", "candidates" timed, "cached"."""
+    mode) of the SH coupling or None; graphs=False leaves graph replay out.
+    Returns a dict: "options" (name -> value, to pass to apply() through
+    option_ids()), "ms_per_step" tuned and "default_ms_per_step", "speedup",
+    "candidates" timed, "cached"."""
     params = list(params)
-    key = "%d/%d/%d/%s/%d/%s" % (mesh.N, len(params), precision, sh[2] if sh else SH_OFF,
-                                 sh[0] if sh else 0, _device_name())
+    key = "%d/%d/%d/%s/%d/%s%s" % (mesh.N, len(params), precision, sh[2] if sh else SH_OFF,
+                                   sh[0] if sh else 0, _device_name(), "" if graphs else "/no-graphs")
     if use_cache:
         if key in _MEMO:
             return dict(_MEMO[key], cached=True)
@@ -146,7 +149,7 @@ def tune(mesh, params, precision=FP64, sh=None, steps=100, warmup=20, repeats=3,
     t_default = _time(ctx, steps, warmup, repeats)
     rows = []
     best, t_best = dict(default), t_default
-    for cand in candidates(ctx):
+    for cand in candidates(ctx, graphs):
         apply(ctx, default)
         set_ = apply(ctx, cand)
         if set_ != {o: int(v) for o, v in cand.items()}:
