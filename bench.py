@@ -163,8 +163,11 @@ def measured_traffic(kname, tname, n_cells):
 def timed_run(ctx, steps, warmup, stride):
     """Warm-up, then `steps` timed steps between synchronisations; returns
     (seconds, kernel times)."""
-    ctx.step(warmup, sync=True)
+    # the warm-up steps are sampled too (their times dropped), so that the
+    # events' first use is not inside the timed steps
     ctx.enable_timing(stride)
+    ctx.step(warmup, sync=True)
+    ctx.timing()
     ctx.sync()
     t1 = time.perf_counter()
     ctx.step(steps, sync=True)
@@ -646,8 +649,8 @@ def main():
                     help="workgroups of the one-kernel step (0: library default, one per patch)")
     ap.add_argument("--timing-stride", type=int, default=-1,
                     help="dispatch-stamped HIP events on the kernels of every T-th step (0: none, "
-                         "roofline achieved then null; default: 16, or fewer for short runs so that at "
-                         "least 5 steps are sampled)")
+                         "roofline achieved then null; default: 16, or half the steps for short runs, "
+                         "which samples two)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (20 times), to calibrate "
                          "the FETCH_SIZE / WRITE_SIZE counters, and exit")
@@ -663,7 +666,9 @@ def main():
         print(json.dumps({"copy_calibration_bytes": n * 8, "launches": 20}), flush=True)
         return
     if args.timing_stride < 0:
-        args.timing_stride = max(1, min(16, args.steps // 5))
+        # every stamped step costs ~25 us (its kernels cannot overlap their
+        # neighbours): short runs sample two steps, long ones every 16th
+        args.timing_stride = max(1, min(16, args.steps // 2))
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU; multi-GPU runs are not part of it "
               "(requested --gpus %d)" % args.gpus, file=sys.stderr)

@@ -2105,6 +2105,13 @@ uint64_t hilbert3(uint32_t X0, uint32_t X1, uint32_t X2, int bits) {
 
 }  // namespace
 
+// Event pairs (start, end) recorded for the kernels of sampled steps.
+struct ChunkEvents {
+  std::vector<int> mom;   // momentum kernel (k_face, or the one-kernel k_step) of each sampled step
+  std::vector<int> cont;  // continuity kernel (k_cell) of each sampled face + cell step
+  std::vector<int> sh;    // 3 event pairs (analysis, coefficients, synthesis) of each sampled SH step
+};
+
 struct odis_ctx {
   int N = 0, E = 0, B = 0, precision = ODIS_FP64;
   int nb_face = 0, nb_cell = 0;
@@ -2171,6 +2178,7 @@ struct odis_ctx {
   std::vector<hipEvent_t> ev;
   double t_face = 0, t_cell = 0, t_step = 0;
   double t_sh[3] = {0, 0, 0};  // SH analysis, coefficients, synthesis
+  std::vector<ChunkEvents> pending;  // sampled steps of the last odis_step call, read on demand
   // spherical-harmonic coupling
   int sh_mode = ODIS_SH_OFF, sh_L = 0, sh_K = 0;
   double *sh_Yt = nullptr, *sh_Minv = nullptr, *sh_D = nullptr, *sh_b = nullptr, *sh_c = nullptr;
@@ -3207,12 +3215,6 @@ static int sh_effective_height(odis_ctx *ctx, Ops<T> &o, hipEvent_t *ev) {
                      static_cast<T *>(ctx->eta_eff), ctx->sh_c, true, true, ev);
 }
 
-// Event pairs (start, end) recorded for the kernels of sampled steps.
-struct ChunkEvents {
-  std::vector<int> mom;   // momentum kernel (k_face, or the one-kernel k_step) of each sampled step
-  std::vector<int> cont;  // continuity kernel (k_cell) of each sampled face + cell step
-  std::vector<int> sh;    // 3 event pairs (analysis, coefficients, synthesis) of each sampled SH step
-};
 
 static int reserve_events(odis_ctx *ctx, size_t n) {
   while (ctx->ev.size() < n) {
@@ -3638,58 +3640,13 @@ static void drop_graphs(odis_ctx *ctx) {
       }
 }
 
-template <typename T>
-static int step_t(odis_ctx *ctx, Ops<T> &o, int n_steps) {
-  // Timing mode (ctx->timing = T > 0): the kernels of every T-th step get
-  // dispatch-stamped start/end events (chunks with a sampled step are not
-  // replayed from graphs).  Events are read once, after the last chunk.
-  std::vector<ChunkEvents> sampled;
-  size_t ev_used = 0;
-  int done = 0;
-  int rc = ODIS_OK;
-  const bool il = il_wanted(ctx);
-  if (il) {
-    if (!ctx->il_fresh) {  // the instance-major arrays are current: copy them in
-      if ((rc = ensure_ts(ctx, o)) || (rc = il_alloc<T>(ctx)) || (rc = il_convert(ctx, o, true))) return rc;
-      ctx->il_fresh = 1;
-    }
-    if ((rc = il_records(ctx, o))) return rc;
-    ctx->std_fresh = 0;
-  } else {
-    if ((rc = il_out(ctx, o)) || (rc = ensure_ts(ctx, o))) return rc;
-    ctx->il_fresh = 0;
-  }
-  while (done < n_steps) {
-    if (ctx->stop_flag) {
-      ctx->stop_flag = 0;
-      return fail(ctx, ODIS_E_STOPPED, "stop requested");
-    }
-    const int left = n_steps - done;
-    const bool graph_fits = ctx->use_graphs && left >= ctx->graph_steps;
-    const int count = graph_fits ? ctx->graph_steps : std::min(ctx->chunk, left);
-    // does [step, step + count) hold a multiple of the timing stride?
-    const bool sample = ctx->timing > 0 && ((ctx->step + ctx->timing - 1) / ctx->timing) * ctx->timing <
-                                               ctx->step + count;
-    const bool graph = graph_fits && !sample;
-    const double *table = nullptr;
-    if ((rc = stage_coefficients(ctx, ctx->step, count, &table))) return rc;
-    if (graph) {
-      hipGraphExec_t g = nullptr;
-      if ((rc = get_graph(ctx, o, ctx->coef_last, &g))) return rc;
-      CK(hipGraphLaunch(g, ctx->stream));
-      if (count & 1) ctx->cur ^= 1;
-    } else {
-      ChunkEvents *evs = nullptr;
-      if (sample) {
-        sampled.emplace_back();
-        evs = &sampled.back();
-      }
-      if ((rc = launch_chunk(ctx, o, table, count, evs, &ev_used))) return rc;
-    }
-    ctx->step += count;
-    done += count;
-  }
-  if (!sampled.empty()) {
+// Average kernel times of the sampled steps of the last odis_step call (its
+// events, read once: after the step's work, never inside it).
+static int read_kernel_times(odis_ctx *ctx) {
+  if (ctx->pending.empty()) return ODIS_OK;
+  const std::vector<ChunkEvents> sampled = std::move(ctx->pending);
+  ctx->pending.clear();
+  {
     CK(hipStreamSynchronize(ctx->stream));
     double tm = 0, tc = 0, ts = 0;
     long long ns = 0;
@@ -3736,6 +3693,65 @@ static int step_t(odis_ctx *ctx, Ops<T> &o, int n_steps) {
       ctx->t_step = k ? ctx->t_step / k : 0.0;
     }
   }
+  return ODIS_OK;
+}
+
+template <typename T>
+static int step_t(odis_ctx *ctx, Ops<T> &o, int n_steps) {
+  // Timing mode (ctx->timing = T > 0): the kernels of every T-th step get
+  // dispatch-stamped start/end events (chunks with a sampled step are not
+  // replayed from graphs).  The events are read by read_kernel_times when
+  // the times are asked for, or before this call reuses them.
+  int rc = read_kernel_times(ctx);
+  if (rc) return rc;
+  std::vector<ChunkEvents> sampled;
+  size_t ev_used = 0;
+  int done = 0;
+  const bool il = il_wanted(ctx);
+  if (il) {
+    if (!ctx->il_fresh) {  // the instance-major arrays are current: copy them in
+      if ((rc = ensure_ts(ctx, o)) || (rc = il_alloc<T>(ctx)) || (rc = il_convert(ctx, o, true))) return rc;
+      ctx->il_fresh = 1;
+    }
+    if ((rc = il_records(ctx, o))) return rc;
+    ctx->std_fresh = 0;
+  } else {
+    if ((rc = il_out(ctx, o)) || (rc = ensure_ts(ctx, o))) return rc;
+    ctx->il_fresh = 0;
+  }
+  while (done < n_steps) {
+    if (ctx->stop_flag) {
+      ctx->stop_flag = 0;
+      return fail(ctx, ODIS_E_STOPPED, "stop requested");
+    }
+    const int left = n_steps - done;
+    const bool graph_fits = ctx->use_graphs && left >= ctx->graph_steps;
+    const int count = graph_fits ? ctx->graph_steps : std::min(ctx->chunk, left);
+    // does [step, step + count) hold a multiple of the timing stride?
+    const bool sample = ctx->timing > 0 && ((ctx->step + ctx->timing - 1) / ctx->timing) * ctx->timing <
+                                               ctx->step + count;
+    const bool graph = graph_fits && !sample;
+    const double *table = nullptr;
+    if ((rc = stage_coefficients(ctx, ctx->step, count, &table))) return rc;
+    if (graph) {
+      hipGraphExec_t g = nullptr;
+      if ((rc = get_graph(ctx, o, ctx->coef_last, &g))) return rc;
+      CK(hipGraphLaunch(g, ctx->stream));
+      if (count & 1) ctx->cur ^= 1;
+    } else {
+      ChunkEvents *evs = nullptr;
+      if (sample) {
+        sampled.emplace_back();
+        evs = &sampled.back();
+      }
+      if ((rc = launch_chunk(ctx, o, table, count, evs, &ev_used))) return rc;
+    }
+    ctx->step += count;
+    done += count;
+  }
+  // the events are read when the kernel times are asked for, outside the
+  // caller's timed steps
+  ctx->pending = std::move(sampled);
   return ODIS_OK;
 }
 
@@ -4038,6 +4054,9 @@ extern "C" int odis_request_stop(odis_ctx *ctx) {
 extern "C" int odis_enable_timing(odis_ctx *ctx, int stride) {
   if (!ctx || stride < 0) return ODIS_E_INVALID;
   ctx->timing = stride;
+  // the events of a whole chunk's sampled steps, created here rather than
+  // inside the steps they time
+  if (stride > 0) return guarded(ctx, [&] { return reserve_events(ctx, 10 * size_t(ctx->chunk / stride + 2)); });
   return ODIS_OK;
 }
 
@@ -4165,6 +4184,8 @@ extern "C" int odis_get_option(odis_ctx *ctx, int option, int *value) {
 
 extern "C" int odis_get_kernel_times(odis_ctx *ctx, double *ms, int n) {
   if (!ctx || !ms || n < 0) return ODIS_E_INVALID;
+  const int rc = guarded(ctx, [&] { return read_kernel_times(ctx); });
+  if (rc) return rc;
   const double v[ODIS_T_COUNT] = {ctx->t_face, ctx->t_cell, ctx->t_step, ctx->t_sh[0], ctx->t_sh[1], ctx->t_sh[2]};
   for (int q = 0; q < n && q < ODIS_T_COUNT; q++) ms[q] = v[q];
   return ODIS_OK;
@@ -4172,6 +4193,8 @@ extern "C" int odis_get_kernel_times(odis_ctx *ctx, double *ms, int n) {
 
 extern "C" int odis_get_timing(odis_ctx *ctx, double *face_ms, double *cell_ms, double *step_ms) {
   if (!ctx) return ODIS_E_INVALID;
+  const int rc = guarded(ctx, [&] { return read_kernel_times(ctx); });
+  if (rc) return rc;
   if (face_ms) *face_ms = ctx->t_face;
   if (cell_ms) *cell_ms = ctx->t_cell;
   if (step_ms) *step_ms = ctx->t_step;
