@@ -331,7 +331,10 @@ int odis_get_option(odis_ctx *ctx, int option, int *value);
  * start/stop HIP events on the context stream; odis_get_timing returns the
  * average duration (ms) of the face kernel, of the cell kernel and of a whole
  * sampled step (face start to cell end), over the sampled steps of the last
- * odis_step call.  T = 0 disables. */
+ * odis_step call.  T = 0 disables.  odis_enable_timing creates the events a
+ * chunk of steps needs; the events are read (waiting for the stream) when
+ * odis_get_timing or odis_get_kernel_times asks for them, or when the next
+ * odis_step starts, so a timed odis_step adds no host work of its own. */
 int odis_enable_timing(odis_ctx *ctx, int stride);
 int odis_get_timing(odis_ctx *ctx, double *face_ms, double *cell_ms, double *step_ms);
 /* The same averages and those of the SH kernels of sampled steps, indexed
