@@ -32,7 +32,10 @@
  *   PLANET forcing                src/tidalPotentials.cpp:176-225   pure degree 2; G m R^2 / d^3 towards the companion
  *   FULL2 forcing                 src/tidalPotentials.cpp:133-156   first order: ECC at zero obliquity, OBLIQ a
  *                                                                     quarter period later at zero eccentricity;
- *                                                                     second-order terms parity unpinned
+ *                                                                     second order: the exact tide of an eccentric,
+ *                                                                     inclined orbit (the theta^2 sectoral term as
+ *                                                                     the reference writes it); third- and
+ *                                                                     fourth-order terms parity unpinned
  *   OBLIQ_WEST closed form        src/analyticalLTE.cpp:133-180     exact continuity by centred differences
  *   warm-start network            (no reference counterpart)        closed forms; gradient by centred differences
  */

@@ -205,8 +205,8 @@ def test_forcing_is_degree_two(oracle, product, tide):
     158-174) are pure degree-2 spherical harmonics: a least-squares fit on the
     9 real harmonics of degree <= 2 leaves no residual and no degree 0/1 part.
     (FULL2, src/tidalPotentials.cpp:133-156, carries a cos^2(lat) cos(lon - M)
-    term that is not degree 2 as written; it is checked only against the
-    GPU path, parity unpinned beyond the restated formula.)"""
+    term that is not degree 2 as written; it is checked against the exact
+    tide of an eccentric, inclined orbit instead.)"""
     m, arrs = _omesh(oracle, 4)
     op, _ = _params(oracle, product, tide=getattr(product, tide), obliquity=0.02, ecc=0.01)
     sim = oracle.Sim(m, op)
@@ -501,6 +501,58 @@ def test_full2_reduces_to_ecc_and_obliq(oracle, product):
             ref = so.forcing(t + P / 4)
             d.append(np.linalg.norm(s2.forcing(t) - ref) / np.linalg.norm(so.forcing(0.0)))
         assert d[0] < 2e-3 and 1.8 < d[0] / d[1] < 2.2, d
+
+
+def _exact_tide(op, ecc, obl, t, lat, lon):
+    """Degree-2 tide of a point mass on a Keplerian orbit about a
+    synchronously rotating body, U = w^2 R^2 (a/r)^3 P_2(cos gamma) (times
+    the Love reduction), with G M = w^2 a^3.  The orbit's pericentre lies on
+    the node of its plane on the body's equator, tilted by the obliquity; the
+    body's prime meridian sits at the mean anomaly M = w t from that node.
+    The sub-planet point is at latitude asin(sin f sin obl) and longitude
+    atan2(sin f cos obl, cos f) - M (f the true anomaly, from Kepler's
+    equation by Newton's method)."""
+    M = op.omega * t
+    E = M
+    for _ in range(50):
+        E -= (E - ecc * np.sin(E) - M) / (1.0 - ecc * np.cos(E))
+    f = 2.0 * np.arctan2(np.sqrt(1.0 + ecc) * np.sin(E / 2), np.sqrt(1.0 - ecc) * np.cos(E / 2))
+    lam = np.arctan2(np.sin(f) * np.cos(obl), np.cos(f)) - M
+    beta = np.arcsin(np.sin(f) * np.sin(obl))
+    cg = np.cos(lat) * np.cos(beta) * np.cos(lon - lam) + np.sin(lat) * np.sin(beta)
+    a_r = 1.0 / (1.0 - ecc * np.cos(E))
+    return op.love_reduct * op.omega ** 2 * op.radius ** 2 * a_r ** 3 * (1.5 * cg * cg - 0.5)
+
+
+def test_full2_second_order_against_the_exact_tide(oracle, product):
+    """FULL2 (src/tidalPotentials.cpp:133-156) is the time-varying part of
+    the exact degree-2 tide of an eccentric, inclined orbit (`_exact_tide`),
+    expanded to second order in eccentricity e and obliquity t, with one
+    exception: its obliquity-squared sectoral term reads
+    (3/8) t^2 cos^2(lat) cos(lon - M) where the expansion gives
+    (3/8) t^2 cos^2(lat) cos(2 lon + 2 M).  After adding that difference, the
+    two fields, each less its mean over an orbit (FULL2 has a static
+    e t term, the exact tide static e^2 and t^2 terms), differ by third-order
+    terms only: the difference falls 8x when e and t halve.  A dropped or
+    mistyped second-order term (e^2 cos 2M, e t, t^2) leaves a difference
+    that falls only 4x."""
+    G = product
+    for e, ob in ((1e-2, 0.0), (0.0, 1e-2), (1e-2, 1e-2), (1e-2, 3e-3)):
+        d = []
+        for s in (1.0, 0.5):
+            sim, op, arrs = _sim_for_tide(oracle, product, G.FULL2, ecc=e * s, obliquity=ob * s)
+            la, lo = arrs["lat"], arrs["lon"]
+            ts = np.arange(48) * (2 * np.pi / op.omega / 48)
+            F = np.array([sim.forcing(t) for t in ts])
+            X = np.array([_exact_tide(op, e * s, ob * s, t, la, lo) for t in ts])
+            M = op.omega * ts[:, None]
+            X += (op.love_reduct * op.omega ** 2 * op.radius ** 2 * 0.375 * (ob * s) ** 2 * np.cos(la) ** 2
+                  * (np.cos(lo - M) - np.cos(2 * lo + 2 * M)))
+            F -= F.mean(0)
+            X -= X.mean(0)
+            d.append(np.linalg.norm(F - X) / np.linalg.norm(X))
+        # the relative difference is second order (third order over first)
+        assert d[0] < 1e-3 and 3.6 < d[0] / d[1] < 4.4, (e, ob, d)
 
 
 def test_planet_potential_degree_two_and_amplitude(oracle, product):
