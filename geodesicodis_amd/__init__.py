@@ -510,7 +510,8 @@ class Context:
         self._check(lib().odis_request_stop(self._h))
 
     def enable_timing(self, stride=1):
-        """Sample every stride-th chunk of steps with HIP events (0 = off)."""
+        """Stamp the kernels of every stride-th step with HIP events (0 =
+        off); timing() reads them."""
         self._check(lib().odis_enable_timing(self._h, int(stride)))
 
     def set_option(self, option, value):
@@ -537,7 +538,7 @@ class Context:
 
     def timing(self):
         """Average kernel times (ms) of the sampled steps of the last step()
-        call (odis_get_kernel_times)."""
+        call (odis_get_kernel_times; waits for the context stream)."""
         t = np.zeros(6)
         self._check(lib().odis_get_kernel_times(self._h, _dp(t), 6))
         return dict(face_ms=t[0], cell_ms=t[1], step_ms=t[2], sh_analysis_ms=t[3], sh_coef_ms=t[4],
