@@ -37,8 +37,8 @@ converged orbit-averaged dissipation, plain FP64 and FP32; `--workload
 sweep` and `--workload uneven` run one batched extra alone (for profiling).
 
 Prints one JSON line: cell-steps/s of the whole job; the roofline of the
-dominant kernel (HIP events stamped on the kernel dispatches of every 16th
-step, inside the timed region); and a CPU baseline (the FP64 C oracle with
+dominant kernel (HIP events stamped on the kernel dispatches of every 64th
+step of a long run, inside the timed region); and a CPU baseline (the FP64 C oracle with
 OpenMP, with the same projection as dense products, on a bounded sample of
 the same workload).
 """
@@ -649,8 +649,8 @@ def main():
                     help="workgroups of the one-kernel step (0: library default, one per patch)")
     ap.add_argument("--timing-stride", type=int, default=-1,
                     help="dispatch-stamped HIP events on the kernels of every T-th step (0: none, "
-                         "roofline achieved then null; default: 16, or half the steps for short runs, "
-                         "which samples two)")
+                         "roofline achieved then null; default: 64 from 1024 steps on, else 16, or half "
+                         "the steps for short runs, which samples two)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (20 times), to calibrate "
                          "the FETCH_SIZE / WRITE_SIZE counters, and exit")
@@ -667,8 +667,9 @@ def main():
         return
     if args.timing_stride < 0:
         # every stamped step costs ~25 us (its kernels cannot overlap their
-        # neighbours): short runs sample two steps, long ones every 16th
-        args.timing_stride = max(1, min(16, args.steps // 2))
+        # neighbours): short runs sample two steps, long ones every 64th (16
+        # samples or more)
+        args.timing_stride = 64 if args.steps >= 1024 else max(1, min(16, args.steps // 2))
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU; multi-GPU runs are not part of it "
               "(requested --gpus %d)" % args.gpus, file=sys.stderr)
