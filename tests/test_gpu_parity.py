@@ -388,17 +388,43 @@ def test_sh_synthesis_and_projection(product, oracle, mode, tol, L, level):
     assert rel_l2(c, c_true) < tol
 
 
-def _shell_batch(G, mesh, n, L):
+def _shell_batch(G, mesh, n, L, dt=2.0):
     """n instances under shells of 5..30 km over oceans of 20..40 km, with
     their LID_MEMBR degree factors."""
     ps, betas = [], []
     for i in range(n):
         hs, h = 5e3 + 25e3 * i / max(1, n - 1), 20e3 + 20e3 * ((7 * i) % n) / n
-        p, _ = G.make_params(time_step=2.0, radius=232.1e3, g=0.1166, h=h, surface=G.LID_MEMBR,
+        p, _ = G.make_params(time_step=dt, radius=232.1e3, g=0.1166, h=h, surface=G.LID_MEMBR,
                              shell_thickness=hs, love_reduct=0.99)
         ps.append(p)
         betas.append(G.surface_beta(G.LID_MEMBR, L, 252.1e3, 0.113, hs, h))
     return ps, np.array(betas)
+
+
+@pytest.mark.parametrize("level,L", [(4, 12), (4, 16), (5, 32)])
+def test_sh_coefficients_alone_and_in_batch(product, level, L):
+    """The SH analysis, partial sums and coefficients of an instance do not
+    depend on the rest of its batch: an instance's FP64 trajectory under a
+    shell is the same bit for bit alone and inside a batch of three, for the
+    per-degree analysis (L = 12) and the pair analysis (L = 16, 32; the
+    shells' degree-32 waves need a step of 0.25 s)."""
+    G = product
+    g, _ = generated_grid(level)
+    mesh = G.Mesh(g, 232.1e3)
+    ps, beta = _shell_batch(G, mesh, 3, L, dt=0.25 if L > 16 else 2.0)
+    batch = G.Context(mesh, ps, precision=0)
+    batch.set_sh(L, beta, mode=G.SH_RECUR)
+    one = G.Context(mesh, [ps[1]], precision=0)
+    one.set_sh(L, beta[1:2], mode=G.SH_RECUR)
+    for steps in (1, 2, 37):
+        batch.step(steps)
+        one.step(steps)
+        a, b = one.get_state(0), batch.get_state(1)
+        for k in ("v", "eta", "dvdt", "detadt"):
+            assert np.array_equal(a[k], b[k]), (steps, k)
+    assert np.all(np.isfinite(a["eta"])) and np.abs(a["eta"]).max() > 0
+    batch.close()
+    one.close()
 
 
 @pytest.mark.parametrize("level,L,n", [(5, 16, 20), (4, 8, 70), (3, 6, 150)])
