@@ -65,7 +65,7 @@ def test_reference_grids_match_fixtures(oracle, level):
     _check_trisk_antisymmetry(m, 1e-12 * 16 ** (level - 3) * 10)
 
 
-def test_fixture_values_are_the_surveyed_ones():
+def test_fixture_values_are_the_reference_grid_sums():
     fx = _fixture("grid_invariants.json")
     assert round(fx["grid_l4"]["sum_de_le_over_4pi"], 5) == 1.99873
     assert round(fx["grid_l6"]["sum_de_le_over_4pi"], 5) == 1.99992
