@@ -59,8 +59,8 @@ def run_to_convergence(ctx, steps_per_orbit, tol=1e-4, stable=3, max_orbits=2000
     on_stop(slot, b), if given, runs when instance b (context slot `slot`)
     stops, before it is retired (to read its final state).  Returns one
     InstanceRecord per instance, in the context's original order; ctx ends
-    holding only the instances that never stopped (none, unless retire is
-    False)."""
+    holding the instances that stopped at the last orbit end (a context keeps
+    at least one), or all of them when retire is False."""
     ns = np.asarray(steps_per_orbit, np.int64)
     B = len(ns)
     if ctx.B != B:
