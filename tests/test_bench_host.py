@@ -1,7 +1,10 @@
-"""The bench line's roofline arithmetic without a GPU: the algorithmic
-flops and bytes per launch that `bench.py` divides by the measured kernel
-times, rebuilt here from their per-unit counts, and the HBM traffic it reads
-from the committed PMC summary."""
+"""The bench line's host logic without a GPU: the algorithmic flops and
+bytes per launch that `bench.py` divides by the measured kernel times,
+rebuilt here from their per-unit counts; the HBM traffic it reads from the
+committed PMC summary; and the seeded draws of the sweep (256 instances)
+and of the uneven batch (512)."""
+import numpy as np
+
 import bench
 
 N, E, L = 163842, 491520, 32
@@ -42,3 +45,46 @@ def test_traffic_from_the_committed_pmc_summary():
     t = bench.measured_traffic("k_sh_analysis_pair", "double", N)
     assert t == round((4532.7 * 1.9998 + 265.7 * 1.0) * 1024)
     assert bench.measured_traffic("k_sh_analysis_pair", "double", N + 1) is None
+
+
+def test_sweep_instances_follow_the_seeded_draws(product):
+    """The 256-instance sweep: thickness 10^U(3, 5) m and linear drag
+    10^U(-11, -7) s^-1, drawn in that order per instance from
+    default_rng(20261203); eccentricity tide; each dt the quantised step
+    below 0.35 / omega_max(h)."""
+    G = product
+    ps = bench.sweep_params(G, 256)
+    rng = np.random.default_rng(20261203)
+    for p in ps:
+        h = 10.0 ** rng.uniform(3.0, 5.0)
+        a = 10.0 ** rng.uniform(-11.0, -7.0)
+        assert (p.h, p.alpha, p.tide) == (h, a, G.ECC)
+        assert p.dt * bench.OMEGA_MAX_40962_38KM * (h / 38e3) ** 0.5 <= 0.35
+    hs = np.array([p.h for p in ps])
+    assert 1e3 <= hs.min() and hs.max() <= 1e5 and len(set(hs)) == 256
+
+
+def test_uneven_groups_follow_the_seeded_draws(product):
+    """The 512-instance uneven batch: (level, obliquity, quadratic-drag
+    coefficient, surface) drawn in that order per instance from
+    default_rng(20261204), grouped by (level, shell); FULL tide, quadratic
+    drag, the 20 km LID_MEMBR shell on half of them."""
+    G = product
+    groups, beta = bench.uneven_groups(G, 512)
+    rng = np.random.default_rng(20261204)
+    want = {}
+    for _ in range(512):
+        level = int(rng.choice([4, 5, 6, 7]))
+        theta, cd, lid = rng.uniform(0.0, 1.0), rng.uniform(1e-3, 5e-3), bool(rng.uniform() < 0.5)
+        want.setdefault((level, lid), []).append((theta, cd))
+    assert sorted(groups) == sorted(want) and len(groups) == 8
+    for key, ps in groups.items():
+        got = np.array([(np.degrees(p.obliquity), p.alpha) for p in ps])
+        assert got.shape == (len(want[key]), 2) and np.allclose(got, want[key], rtol=1e-14, atol=0)
+        for p in ps:
+            assert p.tide == G.FULL and p.friction == G.QUADRATIC and p.quad_drag == 1
+            assert (p.surface == G.LID_MEMBR) == key[1]
+            if key[1]:
+                assert p.shell_thickness == 20e3
+    assert sum(len(v) for v in groups.values()) == 512
+    assert beta.shape == (bench.UNEVEN_SH_DEGREE + 1,) and np.all(np.diff(beta[2:]) > 0)
