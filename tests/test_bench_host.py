@@ -88,3 +88,15 @@ def test_uneven_groups_follow_the_seeded_draws(product):
                 assert p.shell_thickness == 20e3
     assert sum(len(v) for v in groups.values()) == 512
     assert beta.shape == (bench.UNEVEN_SH_DEGREE + 1,) and np.all(np.diff(beta[2:]) > 0)
+
+
+def test_more_than_one_gpu_is_refused():
+    """`bench.py --gpus N` with N > 1 exits with status 2 and says that this
+    build runs on one GPU, before touching any device."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "exactly one GPU" in r.stderr and r.stdout == ""
